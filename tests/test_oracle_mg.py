@@ -1,0 +1,189 @@
+"""Pins the multigrid / flexible CG oracle (oracle/mgref.py): the reference's own CPU test
+cases (tests/test_mg_cpu.py), the golden V-cycle / spectral / fcg vectors written from the
+reference in this container, the reference's stored smoke-q4 GPU solutions and known answers,
+and closed forms for the Chebyshev smoother."""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import hexref as hx
+from oracle import mgref as mg
+from oracle import simpref as sr
+
+
+@pytest.fixture(scope="module")
+def KE(golden):
+    return np.load(golden / "smoke-q4.npz")["ke"]
+
+
+def smoke(golden, q=4, seed=None):
+    """tests/test_mg_cpu.py:80-95 of the reference."""
+    d = np.load(golden / "smoke-q4.npz")
+    if q == 4 and seed is None:
+        return sr.fixture_case(d)[:4]
+    shape = (2 * q, q, q)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed or 0)
+    rho = 0.05 + 0.95 * rng.random(nx * ny * nz) ** 2
+    m = np.ones((nz + 1, ny + 1, nx + 1, 3)); m[:, :, 0, :] = 0
+    f = np.zeros((nz + 1, ny + 1, nx + 1, 3)); f[0, :, nx, 2] = -1.0 / (ny + 1)
+    E = hx.to_author(1e-6 + (1 - 1e-6) * rho ** 3, shape)
+    return shape, E, hx.to_author(m.ravel(), shape, True), hx.to_author(f.ravel(), shape, True)
+
+
+def apply_fine(KE, shape, E, mask):
+    prod = hx.NumpyProduct(shape, KE)
+    return lambda u: prod.raw(u * mask, E) * mask + u * (1 - mask)
+
+
+def P_matrix(coarse):
+    n = 3 * hx.n_nodes(coarse)
+    return np.stack([mg.prolong(np.eye(n)[i], coarse) for i in range(n)], axis=1)
+
+
+def test_restriction_is_the_exact_transpose_of_prolongation():
+    rng = np.random.default_rng(1)
+    coarse, fine = (2, 1, 1), (4, 2, 2)
+    c, f = rng.standard_normal(3 * 3 * 2 * 2), rng.standard_normal(3 * 5 * 3 * 3)
+    assert np.isclose(mg.prolong(c, coarse) @ f, c @ mg.restrict(f, fine), rtol=1e-13)
+
+
+def test_prolongation_interpolates_linear_fields_exactly():
+    coarse = (3, 2, 2)
+    X = np.stack(np.meshgrid(*[np.arange(n + 1) * 2.0 for n in coarse], indexing="ij"), -1)
+    Xf = np.stack(np.meshgrid(*[np.arange(2 * n + 1) * 1.0 for n in coarse], indexing="ij"), -1)
+    A = np.array([[1.0, 2, -1], [0.5, 0, 3], [-2, 1, 1]])
+    uc = (X.reshape(-1, 3) @ A.T).ravel()
+    assert np.abs(mg.prolong(uc, coarse) - (Xf.reshape(-1, 3) @ A.T).ravel()).max() < 1e-13
+
+
+def test_rediscretisation_equals_galerkin_for_uniform_modulus(KE):
+    fine, coarse = (4, 2, 2), (2, 1, 1)
+    Kf = mg.assemble_dense(fine, KE, np.ones(16), np.ones(3 * 5 * 3 * 3))
+    Kc = mg.assemble_dense(coarse, 2 * KE, np.ones(2), np.ones(3 * 3 * 2 * 2))
+    P = P_matrix(coarse)
+    assert np.abs(P.T @ Kf @ P - Kc).max() / np.abs(Kc).max() < 1e-13
+
+
+def test_vcycle_is_symmetric(golden, KE):
+    shape, E, mask, _ = smoke(golden)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
+    rng = np.random.default_rng(2)
+    r1, r2 = rng.standard_normal(mask.size) * mask, rng.standard_normal(mask.size) * mask
+    a, b = M(r1) @ r2, r1 @ M(r2)
+    assert abs(a - b) / abs(a) < 1e-10
+
+
+def test_mg_fcg_on_the_real_smoke_input(golden, KE):
+    shape, E, mask, force = smoke(golden)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
+    x, it, ok, rel, _ = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 200)
+    assert ok and rel <= 1e-8 and it <= 30, (it, rel)
+    assert np.all(x[mask == 0] == 0)
+
+
+@pytest.mark.parametrize("q", [4, 8])
+def test_iteration_count_is_roughly_mesh_independent(golden, KE, q):
+    shape, E, mask, force = smoke(golden, q=q, seed=3)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
+    _, it, ok, rel, _ = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 300)
+    assert ok and it <= 60, (q, it, rel)
+
+
+def test_fp32_preconditioner_still_reaches_1e8_in_fp64_outer(golden, KE):
+    shape, E, mask, force = smoke(golden, q=8, seed=4)
+    _, M32 = mg.build_hierarchy(shape, KE, E, mask, dtype=np.float32, max_coarse_x=2)
+    _, it, ok, rel, _ = mg.fcg(apply_fine(KE, shape, E, mask), force, M32, 1e-8, 300)
+    assert ok and rel <= 1e-8, (it, rel)
+
+
+def test_fixed_iterations_do_exactly_the_requested_work(golden, KE):
+    shape, E, mask, force = smoke(golden)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
+    _, it, _, _, mv = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 999, fixed_iterations=7)
+    assert it == 7 and mv == 8
+
+
+def test_single_level_degenerates_to_a_direct_solve(golden, KE):
+    shape, E, mask, force = smoke(golden)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=64)
+    _, it, ok, rel, _ = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 10, check_every=1)
+    assert ok and it <= 2, (it, rel)
+
+
+def test_fixed_omega_0_6_is_unstable_on_heterogeneous_density(golden, KE):
+    shape, E, mask, _ = smoke(golden, q=8, seed=3)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
+    _, lambdas = mg.spectral_omegas(M.levels)
+    assert 0.6 * max(lambdas) > 2.0
+    assert all(w * lam == pytest.approx(4 / 3) for w, lam in zip(M.omegas, lambdas))
+
+
+@pytest.mark.parametrize("tag", ["smoke", "q8s3"])
+def test_matches_the_reference_golden_vectors(golden, KE, tag):
+    g = np.load(golden / "mg_golden.npz")
+    shape = tuple(int(s) for s in g[f"{tag}/shape"])
+    E, mask, force = g[f"{tag}/E"], g[f"{tag}/mask"], g[f"{tag}/force"]
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
+    _, lams = mg.spectral_omegas(M.levels)
+    assert np.allclose(lams, g[f"{tag}/lambdas"], rtol=1e-12, atol=0)
+    out = M(g[f"{tag}/vcycle_in"])
+    ref = g[f"{tag}/vcycle_out"]
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-12
+    x, it, ok, rel, mv = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 300)
+    st = g[f"{tag}/fcg_stats"]
+    assert (it, int(ok), mv) == (int(st[0]), int(st[1]), int(st[3]))
+    assert np.linalg.norm(x - g[f"{tag}/fcg_x"]) / np.linalg.norm(x) < 1e-10
+    if tag == "q8s3":
+        _, M32 = mg.build_hierarchy(shape, KE, E, mask, dtype=np.float32, max_coarse_x=2)
+        assert np.allclose(mg.spectral_omegas(M32.levels)[1], g[f"{tag}/lambdas32"], rtol=1e-5)
+        ref32 = g[f"{tag}/vcycle32_out"]
+        assert np.linalg.norm(M32(g[f"{tag}/vcycle_in"]) - ref32) / np.linalg.norm(ref32) < 1e-5
+
+
+def test_mg_solution_matches_the_reference_gpu_run(golden, KE):
+    """results/session-d-20260923 smoke-q4--mg64-modal8 (max_coarse_x = 16 -> one level):
+    one iteration; same state and compliance."""
+    ka = json.loads((golden / "known_answers.json").read_text())["session_d/mg64-modal8"]
+    d = np.load(golden / "smoke-q4.npz")
+    shape, E, mask, force = sr.fixture_case(d)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=16)
+    x, it, ok, rel, mv = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 200, check_every=1)
+    assert ok and it == ka["iterations"] and mv == ka["outer_matvecs"]
+    sol = hx.to_author(np.load(golden / "smoke-q4--mg64-modal8--tol-00.npz")["solution"], shape, True)
+    assert np.linalg.norm(x - sol) / np.linalg.norm(sol) < 1e-12
+    assert float(force @ x) == pytest.approx(ka["force_work"], rel=1e-12)
+
+
+def test_chebyshev_is_the_chebyshev_polynomial():
+    lam = np.linspace(0.05, 2.0, 40)
+    lmax, frac = 2.0, 1.0 / 30.0
+    for deg in (1, 2, 3, 5):
+        x = mg.chebyshev(lambda v: lam * v, np.ones_like(lam), np.zeros_like(lam),
+                         np.ones_like(lam), lmax, frac, deg)
+        th, de = lmax * (1 + frac) / 2, lmax * (1 - frac) / 2
+        T = np.polynomial.chebyshev.Chebyshev.basis(deg)
+        assert np.abs((1 - lam * x) - T((th - lam) / de) / T(th / de)).max() < 1e-13
+    # degree one is damped Jacobi with omega = 1/theta
+    x1 = mg.chebyshev(lambda v: lam * v, np.ones_like(lam), np.zeros_like(lam), np.ones_like(lam),
+                      lmax, frac, 1)
+    assert np.allclose(x1, 1.0 / (lmax * (1 + frac) / 2))
+
+
+def test_donor_chebyshev_variant_is_a_different_polynomial():
+    """The donor recurrence (multigrid_v4.py:860-889) is kept selectable; it departs from the
+    Chebyshev polynomial from degree two on."""
+    lam = np.linspace(0.05, 2.0, 40)
+    a = mg.chebyshev(lambda v: lam * v, np.ones_like(lam), np.zeros_like(lam), np.ones_like(lam),
+                     2.0, 1 / 30, 3, variant="donor")
+    b = mg.chebyshev(lambda v: lam * v, np.ones_like(lam), np.zeros_like(lam), np.ones_like(lam),
+                     2.0, 1 / 30, 3)
+    assert np.abs(a - b).max() > 1e-2
+
+
+def test_chebyshev_multigrid_converges(golden, KE):
+    shape, E, mask, force = smoke(golden, q=8, seed=3)
+    _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2, smoother="chebyshev", degree=3)
+    _, it, ok, rel, _ = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 300)
+    assert ok and it <= 40, (it, rel)

@@ -1,0 +1,145 @@
+"""Pins the Jacobi-PCG / mixed IR oracle (oracle/solvers.py) on the reference's own smoke-q4
+state and stored GPU solutions, and the SIMP pieces (oracle/simpref.py) by brute force,
+adjointness, invariants and finite differences."""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import hexref as hx
+from oracle import simpref as sr
+from oracle import solvers as sv
+
+
+@pytest.fixture(scope="module")
+def smoke_ctx(golden):
+    d = np.load(golden / "smoke-q4.npz")
+    shape, E, mask, force = sr.fixture_case(d)
+    prod = hx.NumpyProduct(shape, d["ke"])
+    diag = np.where(mask != 0, prod.diagonal(E), 1.0)
+    ka = json.loads((golden / "known_answers.json").read_text())
+    return dict(d=d, shape=shape, E=E, mask=mask, force=force, prod=prod, inv=1 / diag, ka=ka)
+
+
+def _action(ctx, dtype=np.float64):
+    prod32 = hx.NumpyProduct(ctx["shape"], ctx["d"]["ke"], dtype)
+    m = ctx["mask"].astype(dtype)
+    E = ctx["E"].astype(dtype)
+    return lambda u: prod32.raw(u * m, E) * m + u * (1 - m)
+
+
+def test_pcg_known_answers_on_smoke_q4(smoke_ctx, golden):
+    c = smoke_ctx
+    ka = c["ka"]["session_a/pcg64-modal8"]
+    x, it, ok, rel = sv.pcg(_action(c), c["force"], c["inv"], tol=1e-8, maxiter=20000,
+                            mask=c["mask"], check_every=50)
+    assert ok and it == ka["iterations"] and rel < 1e-12
+    sol = hx.to_author(np.load(golden / "smoke-q4--pcg64-modal8--tol-00.npz")["solution"],
+                       c["shape"], True)
+    assert np.linalg.norm(x - sol) / np.linalg.norm(sol) < 1e-10
+    assert float(c["force"] @ x) == pytest.approx(ka["force_work"], rel=1e-11)
+    assert np.all(x[c["mask"] == 0] == 0)
+
+
+def test_pcg_fixed_work_matvec_count(smoke_ctx):
+    """fixed-work solves run tol=0, maxiter=100: 104 matvecs (1 initial + 100 + 2 checks + 1 exit)."""
+    c = smoke_ctx
+    calls = [0]
+    A = _action(c)
+
+    def counted(u):
+        calls[0] += 1
+        return A(u)
+    _, it, ok, _ = sv.pcg(counted, c["force"], c["inv"], tol=0.0, maxiter=100, mask=c["mask"],
+                          check_every=50)
+    ka = c["ka"]["session_a/fixed/modal8"]
+    assert it == ka["iterations"] and calls[0] == ka["matvec_calls"] and not ok
+
+
+def test_pcg_ir_known_answers_on_smoke_q4(smoke_ctx, golden):
+    c = smoke_ctx
+    ka = c["ka"]["session_a/ir-modal8-modal8_32"]
+    x, outer, inner, ok, rel = sv.pcg_ir(_action(c), c["force"], _action(c, np.float32),
+                                         c["inv"].astype(np.float32), tol=1e-8,
+                                         mask32=c["mask"].astype(np.float32), inner_tol=0.1,
+                                         max_outer=40, inner_maxiter=4000, check_every=50)
+    assert ok and rel <= 1e-8
+    assert outer == ka["outer_iterations"] and inner == ka["inner_iterations"]
+    sol = hx.to_author(np.load(golden / "smoke-q4--pcg64-modal8--tol-00.npz")["solution"],
+                       c["shape"], True)
+    assert np.linalg.norm(x - sol) / np.linalg.norm(sol) < 5e-5      # protocol cross-state limit
+
+
+def test_pcg_reports_nonconvergence(smoke_ctx):
+    c = smoke_ctx
+    _, it, ok, rel = sv.pcg(_action(c), c["force"], c["inv"], tol=1e-8, maxiter=1, mask=c["mask"])
+    assert it == 1 and not ok and rel > 1e-8
+
+
+def test_cone_filter_matches_the_explicit_matrix():
+    shape = (5, 4, 3)
+    x = np.random.default_rng(0).random(60)
+    for rmin in (1.5, 2.2, 1.0):
+        H = sr.cone_matrix(shape, rmin)
+        assert np.abs(sr.cone_stencil(x, shape, rmin) - H @ x).max() < 1e-14
+    assert len(sr.cone_offsets(1.5)) == 19
+    w = sorted({round(o[3], 12) for o in sr.cone_offsets(1.5)})
+    assert np.allclose(w, [1.5 - np.sqrt(2), 0.5, 1.5])
+
+
+def test_cone_filter_adjointness_and_constants():
+    shape = (6, 5, 4)
+    f = sr.ConeFilter(shape, 1.5)
+    rng = np.random.default_rng(2)
+    a, b = rng.random(120), rng.random(120)
+    lhs, rhs = f.forward(a) @ b, a @ f.transpose(b)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+    assert np.allclose(f.forward(np.full(120, 0.37)), 0.37, rtol=1e-14)
+
+
+def test_oc_top88_invariants():
+    rng = np.random.default_rng(3)
+    n = 200
+    x = rng.uniform(0.2, 0.6, n)
+    dc = -rng.random(n)
+    dv = np.ones(n)
+    xn = sr.oc_top88(x, dc, dv, 0.4, 0.2)
+    assert np.all(xn >= np.maximum(x - 0.2, 0) - 1e-15) and np.all(xn <= np.minimum(x + 0.2, 1) + 1e-15)
+    assert abs(xn.mean() - 0.4) < 5e-3
+    # uniform sensitivities: the update is uniform scaling by the volume fraction ratio
+    x0 = np.full(n, 0.5)
+    xu = sr.oc_top88(x0, -np.ones(n), dv, 0.45, 0.2)
+    assert np.allclose(xu, xu[0]) and abs(xu[0] - 0.45) < 1e-3
+
+
+def test_sensitivity_matches_central_finite_differences():
+    shape = (4, 2, 2)
+    ke = 2 * hx.quadrature([1, 1, 1], 0.3)
+    rho, E, mask, force = sr.cantilever(shape, seed=9, rho_lo=0.3)
+    A = lambda r: hx.assemble_dense(shape, ke, sr.modulus(r), mask)  # noqa: E731
+
+    def compliance(r):
+        return force @ np.linalg.solve(A(r), force)
+    u = np.linalg.solve(A(rho), force)
+    dc = sr.sensitivity(rho, hx.element_energy(shape, ke, u))
+    h = 1e-6
+    for e in (0, 5, 11):
+        rp, rm = rho.copy(), rho.copy()
+        rp[e] += h; rm[e] -= h
+        fd = (compliance(rp) - compliance(rm)) / (2 * h)
+        assert fd == pytest.approx(dc[e], rel=1e-6)
+
+
+def test_regenerate_reproduces_the_fixture_boundary_data(golden):
+    for name in ("smoke-q4.npz", "optimized-q64.npz"):
+        d = np.load(golden / name)
+        m, f = sr.regenerate(d["shape"])
+        assert np.array_equal(m, d["mask"]) and np.array_equal(f, d["force"])
+
+
+def test_cantilever_load_is_unit_and_clamp_is_x0():
+    rho, E, mask, force = sr.cantilever((8, 4, 4), seed=1)
+    assert force.sum() == pytest.approx(-1.0)
+    m = mask.reshape(9, 5, 5, 3)
+    assert np.all(m[0] == 0) and np.all(m[1:] == 1)
+    assert np.all((rho >= 0.01) & (rho <= 1.0))
