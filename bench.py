@@ -1,0 +1,209 @@
+"""Benchmark of the voxel elasticity hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload (one "step" = one complete solve from x = 0 to the FP64 true relative residual
+||b - A x|| / ||b|| <= 1e-8, setup excluded as in the reference's timing boundary):
+  Hex8 voxel cantilever 256 x 128 x 128 elements (12,830,211 DOFs), E0 = 1, Emin = 1e-9,
+  SIMP p = 3, densities iid U[0.01, 1] (seed 3) smoothed by the radius-1.5 cone filter, x = 0
+  face clamped, unit downward line load on the x = nx, z = nz/2 edge; GMG-preconditioned
+  flexible CG (5 levels, 2x coarsening, Chebyshev-Jacobi degree 3, dense coarsest), mixed
+  precision: FP32 V-cycle inside the FP64 outer CG with FP64 true residual and FP64 dot
+  accumulation.  The plain FP64 solve of the same system is timed beside it.
+
+Printed JSON line: value = mixed-precision solves/s; extra fields carry the FP64 solve, the
+128 x 64 x 64 FP64 GMG solve, matvec GDOF/s at both sizes, the roofline of the dominant kernel
+(the FP64 parity product at 256 x 128 x 128, timed with HIP events on its launch stream), and
+the CPU baseline (the NumPy oracle on the host cores, a bounded sample: one flexible-CG
+iteration of the same solve, scaled by the GPU's iteration count).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "matrix-free-voxel-elasticity_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12           # MI355X HBM3E bytes/s
+
+
+def problem(shape, seed, rmin=1.5):
+    """Synthetic cantilever in the author layout (device tensors): rho, E, force, mask values."""
+    import torch
+    from voxelhip import ops
+    from voxelhip.simp import ConeFilter, modulus
+    nx, ny, nz = shape
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(seed)
+    rho = torch.from_numpy(rng.uniform(0.01, 1.0, nx * ny * nz)).to(dev)
+    if rmin:
+        rho = ConeFilter(shape, rmin, device=dev).forward(rho)
+    E = modulus(rho, 3.0, 1.0, 1e-9)
+    f = torch.zeros((nx + 1, ny + 1, nz + 1, 3), dtype=torch.float64, device=dev)
+    line = torch.full((ny + 1,), -1.0 / ny, dtype=torch.float64, device=dev)
+    line[0] = line[-1] = -0.5 / ny
+    f[nx, :, nz // 2, 2] = line
+    m = torch.ones_like(f)
+    m[0] = 0
+    return rho, E, f.reshape(-1).contiguous(), m.reshape(-1), ops
+
+
+def timed_solves(solver, b, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        solver.solve(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        _, it, ok, rel, mv = solver.solve(b)
+        stats.append((it, ok, rel, mv))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, stats
+
+
+def matvec_timing(shape, dtype, reps=20):
+    """Average duration of the parity product, HIP events on the launch stream."""
+    import torch
+    import voxel_kernels as vk
+    from voxelhip import ops
+    ke = 2 * vk.quadrature([1, 1, 1], 0.3)
+    blocks = vk.Modal(ke).blocks
+    ne, nd = int(np.prod(shape)), 3 * ops.n_nodes(shape)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    E = torch.rand(ne, generator=g, device="cuda", dtype=dtype)
+    us = [torch.randn(nd, generator=g, device="cuda", dtype=dtype) for _ in range(2)]
+    ys = [torch.empty(nd, device="cuda", dtype=dtype) for _ in range(2)]
+    for i in range(4):
+        ops.matvec(blocks, E, us[i % 2], ys[i % 2], shape)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream.synchronize()
+    e0.record(stream)
+    for i in range(reps):
+        ops.matvec(blocks, E, us[i % 2], ys[i % 2], shape)
+    e1.record(stream)
+    e1.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    word = 8 if dtype == torch.float64 else 4
+    algo = word * (2 * nd + ne)
+    return {"shape": list(shape), "dtype": str(dtype).replace("torch.", ""), "us": sec * 1e6,
+            "gdof_s": nd / sec / 1e9, "algorithmic_bytes": algo, "achieved_GBs": algo / sec / 1e9,
+            "frac_hbm": algo / sec / HBM_PEAK}
+
+
+def cpu_baseline(shape, E, mask, force, ke, lambdas, iterations):
+    """Oracle (NumPy, host cores): one flexible-CG iteration of the same FP64 GMG solve, scaled."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import hexref as hx
+    from oracle import mgref as mgr
+    E = E.cpu().numpy()
+    mask = mask.cpu().numpy()
+    force = force.cpu().numpy()
+    shapes = mgr.hierarchy_shapes(shape, 16)
+    prods = [hx.NumpyProduct(s, ke * 2 ** i) for i, s in enumerate(shapes)]
+    Ec, mc = E, mask
+    for i in range(1, len(shapes)):
+        Ec, mc = mgr.coarsen_modulus(Ec, shapes[i - 1]), mgr.coarsen_mask(mc, shapes[i - 1])
+    A = hx.assemble_dense(shapes[-1], ke * 2 ** (len(shapes) - 1), Ec, mc)
+    M = mgr.Multigrid(shapes, prods, E, mask, np.linalg.cholesky(A), np.float64, pre=3, post=3,
+                      smoother="chebyshev", lambdas=lambdas)
+    fine = hx.NumpyProduct(shape, ke)
+    apply = lambda u: fine.raw(u * mask, E) * mask + u * (1 - mask)  # noqa: E731
+    t0 = time.perf_counter()
+    mgr.fcg(apply, force, M, 1e-8, 1, check_every=1, fixed_iterations=1)   # setup z0, 1 iteration
+    sec = time.perf_counter() - t0
+    solve_sec = sec * iterations
+    return {"value": 1.0 / solve_sec, "unit": "solves/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"NumPy oracle FP64 GMG-FCG on {shape[0]}x{shape[1]}x{shape[2]}: initial V-cycle plus "
+                      f"one flexible-CG iteration (V-cycle, 2 fine products) timed = {sec:.2f} s, scaled by the "
+                      f"GPU iteration count {iterations}; BLAS threads = all host cores, np.add.at single-threaded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import voxel_kernels as vk
+    from voxelhip.engine import GMGSolver
+    torch.cuda.set_device(0)
+    ke = 2 * vk.quadrature([1, 1, 1], 0.3)          # the donor's KE_UNIT_3D (ke_unit_3d.npy)
+    shape = (256, 128, 128)
+    rho, E, force, mask, ops = problem(shape, seed=3)
+    ndof = force.numel()
+    out = {}
+    solvers = {}
+    for prec in ("mixed", "fp64"):
+        s = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision=prec, graph=bool(args.graph))
+        solvers[prec] = s
+        dt, st = timed_solves(s, force, args.steps, args.warmup)
+        its = sorted({x[0] for x in st})
+        out[prec] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "iterations": its,
+                     "converged": all(x[1] for x in st), "max_rel_residual": max(x[2] for x in st)}
+    # solution error of the mixed solve against the FP64 solve
+    x64 = solvers["fp64"].solve(force)[0].clone()
+    x32 = solvers["mixed"].solve(force)[0]
+    out["mixed"]["rel_diff_vs_fp64_solution"] = float((x32 - x64).norm() / x64.norm())
+    lambdas = solvers["fp64"].M.lambdas
+    iters64 = out["fp64"]["iterations"][0]
+    del solvers
+    torch.cuda.empty_cache()
+    # the FP64 GMG-FCG of the 128 x 64 x 64 configuration
+    shape2 = (128, 64, 64)
+    _, E2, f2, m2, _ = problem(shape2, seed=2)
+    s2 = GMGSolver(shape2, ke, m2, E2, smoother="chebyshev", degree=3, precision="fp64", coarse="galerkin",
+                   graph=bool(args.graph))
+    dt2, st2 = timed_solves(s2, f2, args.steps, args.warmup)
+    out["c2_fp64"] = {"shape": list(shape2), "seconds_per_solve": dt2, "solves_per_s": 1 / dt2,
+                      "iterations": sorted({x[0] for x in st2}), "converged": all(x[1] for x in st2)}
+    del s2
+    mv = {f"{d}@{'x'.join(map(str, s))}": matvec_timing(s, getattr(torch, d))
+          for s in ((256, 128, 128), (128, 64, 64), (384, 192, 192)) for d in ("float64", "float32")}
+    dom = mv["float64@256x128x128"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(shape, E, mask, force, ke, lambdas, iters64)
+    line = {
+        "metric": "CG solves/s at FP64-equiv rel.res 1e-8 (mixed vs FP64); matvec GDOF/s, %HBM",
+        "value": out["mixed"]["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": out["mixed"]["seconds_per_solve"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 V-cycle in f64 outer CG",
+        "data": "synthetic (seeded densities; no dataset)",
+        "config": {"workload": "cantilever 256x128x128 (12,830,211 DOF), GMG-FCG 5 levels Chebyshev deg 3, "
+                               "FP32 V-cycle in FP64 outer CG, rtol 1e-8 (FP64 true residual)",
+                   "ndof": ndof, "parallelism": "single GPU"},
+        "fp64_solves_per_s": out["fp64"]["solves_per_s"],
+        "mixed_speedup_vs_fp64": out["fp64"]["seconds_per_solve"] / out["mixed"]["seconds_per_solve"],
+        "solves": out,
+        "matvec": mv,
+        "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": dom["frac_hbm"], "traffic": None,
+                     "kernel": "parity_xmarch<double> (FP64 product, 256x128x128)",
+                     "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "avg_launch_us": dom["us"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

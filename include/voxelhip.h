@@ -1,0 +1,151 @@
+/* voxelhip: MI355X (gfx950) matrix-free Hex8 voxel elasticity — C ABI.
+ *
+ * Every entry point returns 0 on success and a negative code on failure (-1 HIP error,
+ * -2 bad argument, -3 outside the parity-block model); vh_last_error() gives the message of
+ * the calling thread's last failure.  Array arguments are DEVICE pointers (plain, contiguous)
+ * unless stated; `stream` is a hipStream_t (NULL = the null stream).  Launches are
+ * asynchronous on `stream`; nothing synchronises the device except vh_modal_blocks (host only).
+ *
+ * Layout ("author" layout of the reference, src/voxel_mg.py:20-21,32-38): element
+ * e = (ex*ny + ey)*nz + ez, node n = (ix*(ny+1) + iy)*(nz+1) + iz, DOF 3n + c.
+ * dtype: 0 = float64, 1 = float32; storage 2 = bfloat16 input vector with float32 compute.
+ * mask_kind: 0 = none (raw product), 1 = per-DOF uint8 array (1 = free), 2 = the x = 0 clamp
+ * (every DOF of node plane ix = 0 fixed; no array needed).
+ */
+#ifndef VOXELHIP_H
+#define VOXELHIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_ABI_VERSION 1
+
+enum { VH_F64 = 0, VH_F32 = 1, VH_BF16 = 2 };
+enum { VH_MASK_NONE = 0, VH_MASK_ARRAY = 1, VH_MASK_X0 = 2 };
+enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3, VH_MODE_DOT = 4 };
+enum { VH_RED_DOT = 0, VH_RED_DOT2 = 1, VH_RED_DOTDIFF = 2, VH_RED_NORM2 = 3 };
+enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4 };
+/* device scalar slots used by vh_finish / vh_cg_update / vh_xpby */
+enum { VH_S_RZ = 0, VH_S_PQ = 1, VH_S_ALPHA = 2, VH_S_BETA = 3, VH_S_FLAG = 4 };
+
+int vh_abi_version(void);
+const char* vh_last_error(void);
+
+/* Parity-block factorisation of a 24x24 element matrix (host pointers): blocks72 = the 8
+ * 3x3 blocks of C = T ke T^T / 64, block g coupling (mode g^1, x), (mode g^2, y), (mode g^4, z).
+ * Returns -3 if the Frobenius share outside the blocks exceeds `tolerance` (not an
+ * axis-aligned isotropic brick).
+ * Replaces: Modal(ke) code/hex_modal/modal.py:55-69 (host setup of the kernel's C[72]). */
+int vh_modal_blocks(const double* ke576, double tolerance, double* blocks72, double* discarded);
+
+/* y = K(E) u (mask_kind 0), or the constrained A u = M K M u + (I - M) u.
+ * blocks72 is a HOST pointer (FP64); E, u, y device arrays of `dtype` (u bf16 if
+ * storage == VH_BF16).  y is fully overwritten; no zeroing needed; y must not alias u.
+ * Replaces: modal8 / modal8_muladd kernels code/hex_modal_cuda/cuda.py:18-45,
+ * experiments/session_e/dependencies/cuda.py:87-108 with CudaModal.matvec_full
+ * code/hex_modal_cuda/cuda.py:96-101; NumpyProduct.raw src/voxel_mg.py:50-53;
+ * Level.apply src/voxel_mg.py:129-130. */
+int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
+              int nx, int ny, int nz, int mask_kind, const uint8_t* mask, void* stream);
+
+/* Fused epilogues over the same single pass (A as in vh_matvec; with a mask, fixed_zero makes
+ * the output 0 at fixed DOFs, i.e. M K M u for RAW/DOT and (b - A u) M for RESID):  RESID y = b - A u;  JACOBI y = u + omega dinv (b - A u);
+ * CHEB d = cheb_a (dinv (b - A u) + cheb_c d), y = u + d (d not read when cheb_c == 0);
+ * DOT y = A u and partial[t] = sum of u . A u over tile t (FP64, length vh_matvec_tiles()).
+ * Replaces: the smoothing sweep and residual of Multigrid.vcycle src/voxel_mg.py:166-174,
+ * the Chebyshev step of donor_yang_gmg multigrid_v4.py:860-889, the matvec + p.q of fcg
+ * src/voxel_mg.py:197-199 and of donor pcg simp_r2.py:258-262. */
+typedef struct vh_epilogue {
+  int mode;
+  int fixed_zero;
+  const void* b;
+  const void* dinv;
+  void* d;
+  double* partial;
+  double omega, cheb_a, cheb_c;
+  int lx; /* x planes per workgroup chunk, 0 = automatic */
+} vh_epilogue;
+int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
+                    int nx, int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep,
+                    void* stream);
+/* number of tiles (DOT partials) and workgroups of the product for a grid */
+int vh_matvec_tiles(int nx, int ny, int nz, int lx, int* n_tiles, int* n_workgroups);
+
+/* Node-owned product with the dense 24x24 ke (host pointer): the dense8 / node control.
+ * Replaces: dense8 code/hex_modal_cuda/cuda.py:47-79 and node_matvec_fp64
+ * experiments/session_a/donor/src/gpu_fem/cuda_operators.py:204-285. */
+int vh_dense_product(int dtype, const double* ke576, const void* E, const void* u, void* y, int nx, int ny, int nz,
+                     void* stream);
+
+/* Jacobi diagonal (invert = 0) or inverse diagonal with 1 at fixed DOFs (invert = 1).
+ * Replaces: jacobi_diag_fp64 cuda_operators.py:371-421 / OperatorSuite.diagonal :763-792;
+ * NumpyProduct.diagonal src/voxel_mg.py:55-58 and Level.inv_diag :126-127. */
+int vh_diagonal(int dtype, const double* ke576, const void* E, void* out, int nx, int ny, int nz, int mask_kind,
+                const uint8_t* mask, int invert, void* stream);
+
+/* Element energy ce[e] = u_e^T ke u_e of the masked u; if dc != NULL also
+ * dc[e] = -penal * dE * rho[e]^(penal-1) * ce[e].
+ * Replaces: elem_compliance_fp64 cuda_operators.py:304-355 and the sensitivity of
+ * donor_yang_gmg solver_v4.py:246-251. */
+int vh_element_energy(int dtype, const double* blocks72, const void* u, void* ce, int nx, int ny, int nz,
+                      int mask_kind, const uint8_t* mask, const void* rho, void* dc, double penal, double dE,
+                      void* stream);
+
+/* Restriction rc = Mc P^T (Mf rf) from the fine grid (nx, ny, nz even) to its 2x coarsening.
+ * Replaces: restrict src/voxel_mg.py:83-89,100-105 with the masks of :169-170. */
+int vh_restrict(int dtype, const void* rf, void* rc, int nx, int ny, int nz, int fmask_kind, const uint8_t* fmask,
+                int cmask_kind, const uint8_t* cmask, void* stream);
+/* Prolongation onto the fine grid: xf += Mf P ec (assign = 0) or xf = Mf P ec (assign = 1).
+ * Replaces: prolong src/voxel_mg.py:74-80,92-97 and the correction of :172. */
+int vh_prolong(int dtype, const void* ec, void* xf, int nx, int ny, int nz, int mask_kind, const uint8_t* mask,
+               int assign, void* stream);
+/* Ec = 2x2x2 mean of Ef.  Replaces: coarsen_modulus src/voxel_mg.py:108-111. */
+int vh_coarsen_modulus(int dtype, const void* Ef, void* Ec, int nx, int ny, int nz, void* stream);
+
+/* y = mask * (A r) for a dense row-major n x n matrix (the coarsest solve with the explicit
+ * inverse of the coarsest operator).  Replaces: Multigrid.coarse_solve src/voxel_mg.py:152-159. */
+int vh_dense_mv(int dtype, int n, const void* A, const void* r, void* y, const uint8_t* mask, void* stream);
+
+/* Deterministic two-stage reductions with device-resident FP64 scalars.
+ * vh_reduce writes FP64 partials (count = 2 for DOT2/DOTDIFF, laid out [j][part]) of length
+ * vh_reduce_parts(n); vh_finish sums them into scal[dst..] and applies a scalar op.
+ * Replaces: the cp.dot / cp.linalg.norm host reductions of fcg src/voxel_mg.py:187-211 and
+ * donor pcg simp_r2.py:242-285. */
+int vh_reduce_parts(size_t n);
+int vh_reduce(int dtype, int kind, size_t n, const void* x, const void* y, const void* w, double* partial,
+              void* stream);
+int vh_finish(const double* partial, int nparts, int count, double* scal, int dst, int op, void* stream);
+/* x += alpha p; if update_r: r -= alpha q and, with dinv, z = dinv r and partial r.z
+ * (alpha = scal[VH_S_ALPHA]).  Replaces: the update lines of donor pcg simp_r2.py:266-284. */
+int vh_cg_update(int dtype, size_t n, void* x, const void* p, void* r, const void* q, const void* dinv, void* z,
+                 const double* scal, int update_r, double* partial, void* stream);
+/* z = dinv r and partial r.z.  Replaces: z = M_inv * r; rz = r.z of simp_r2.py:239-242,282-283. */
+int vh_jacobi_dot(int dtype, size_t n, const void* r, const void* dinv, void* z, double* partial, void* stream);
+/* p = z + beta p (beta = scal[VH_S_BETA]); if zcopy != NULL also zcopy = z in the same pass.
+ * Replaces: p = z + beta p of fcg src/voxel_mg.py:212-213 / donor pcg simp_r2.py:286. */
+int vh_xpby(int dtype, size_t n, const void* z, void* p, const double* scal, void* zcopy, void* stream);
+/* y = a x (w) + b y with host scalars (w optional, elementwise). */
+int vh_axpby(int dtype, size_t n, double a, const void* x, const void* w, double b, void* y, void* stream);
+/* y = x * s (accumulate: y += x * s), s = scale or *dscale (device), inverted if inv_scale;
+ * converts between float64 and float32.  Replaces: the FP32 cast of the scaled residual and
+ * the FP64 update of donor pcg_ir simp_r2.py:372-379. */
+int vh_convert(int dtype_in, int dtype_out, size_t n, const void* x, void* y, double scale, const double* dscale,
+               int inv_scale, int accumulate, void* stream);
+
+/* Cone filter y = post_div^-1 H (x / pre_div) with radius rmin (pre_div/post_div optional).
+ * Replaces: cone_stencil_fp64 filter_r2.py:40-76, ConeFilter.forward/transpose :171-178. */
+int vh_cone_filter(int dtype, const void* x, const void* pre_div, const void* post_div, void* y, int nx, int ny,
+                   int nz, double rmin, void* stream);
+/* OC probe: xnew = clip(x sqrt(max(-dc/dv/lmid, 0)), max(x-move,0), min(x+move,1)) and FP64
+ * partials of sum xnew * w (nparts blocks).  Replaces: one bisection probe of oc_top88_gpu
+ * experiments/session_h/run_w1b.py:82-94 (with w = H(1/s)/n the filtered volume). */
+int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w, double lmid,
+                double move, void* xnew, double* partial, int nparts, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,279 @@
+// Streaming vector kernels of the Krylov solvers and smoothers, with device-resident
+// scalars.  Reductions are two-stage and deterministic: each block writes one FP64 partial per
+// dot product, then a single-block "finish" kernel sums the partials in a fixed order and
+// evaluates the scalar recurrence (alpha, beta, ...) in FP64 on the device, so a solver
+// iteration needs no host synchronisation and can be captured in a HIP graph.
+#include "common.h"
+#include "blas.h"
+
+namespace vh {
+
+// --------------------------------------------------------------------------- reductions
+// out partial[j * nblocks + block] for the dots requested by `kind`:
+//   RED_DOT      x.y
+//   RED_DOT2     x.y, x.w
+//   RED_DOTDIFF  x.y, x.(y - w)
+//   RED_NORM2    x.x
+template <typename T, int KIND>
+__global__ void __launch_bounds__(BLAS_BLOCK) reduce_kernel(size_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                                           const T* __restrict__ w, double* __restrict__ partial) {
+  __shared__ double red[2][BLAS_BLOCK / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    const double xi = double(x[i]);
+    if (KIND == RED_NORM2) {
+      s0 += xi * xi;
+    } else {
+      const double yi = double(y[i]);
+      s0 += xi * yi;
+      if (KIND == RED_DOT2) s1 += xi * double(w[i]);
+      if (KIND == RED_DOTDIFF) s1 += xi * (double(y[i] - w[i]));
+    }
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wv] = s0; red[1][wv] = s1; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < BLAS_BLOCK / 64; ++k) { a += red[0][k]; b += red[1][k]; }
+    partial[blockIdx.x] = a;
+    if (KIND == RED_DOT2 || KIND == RED_DOTDIFF) partial[gridDim.x + blockIdx.x] = b;
+  }
+}
+
+// Sum `count` partial arrays of length `nparts` into scal[dst..dst+count) and evaluate `op`.
+__global__ void __launch_bounds__(256) finish_kernel(const double* __restrict__ partial, int nparts, int count,
+                                                    double* __restrict__ s, int dst, int op) {
+  __shared__ double red[2][4];
+  double acc[2] = {0.0, 0.0};
+  for (int j = 0; j < count; ++j)
+    for (int i = threadIdx.x; i < nparts; i += 256) acc[j] += partial[j * nparts + i];
+  for (int j = 0; j < 2; ++j) acc[j] = wave_sum(acc[j]);
+  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = acc[0]; red[1][threadIdx.x >> 6] = acc[1]; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double v[2];
+  for (int j = 0; j < 2; ++j) v[j] = red[j][0] + red[j][1] + red[j][2] + red[j][3];
+  for (int j = 0; j < count; ++j) s[dst + j] = v[j];
+  // scalar recurrences (slot names in blas.h)
+  switch (op) {
+    case OP_ALPHA: {         // alpha = rz / pq  (pq just written to s[dst])
+      const double pq = v[0];
+      if (pq == 0.0 || !isfinite(pq)) { s[S_ALPHA] = 0.0; s[S_FLAG] = 1.0; }
+      else s[S_ALPHA] = s[S_RZ] / pq;
+      break;
+    }
+    case OP_BETA_FR: {       // Fletcher-Reeves: beta = rz_new / rz ; rz = rz_new
+      const double rzn = v[0];
+      const double rz = s[S_RZ];
+      s[S_BETA] = (rz == 0.0) ? 0.0 : rzn / rz;
+      if (rz == 0.0) s[S_FLAG] = 1.0;
+      s[S_RZ] = rzn;
+      break;
+    }
+    case OP_BETA_PR: {       // Polak-Ribiere: beta = r_new.(z_new - z) / rz ; rz = r_new.z_new
+      const double rzn = v[0], num = v[1];
+      const double rz = s[S_RZ];
+      s[S_BETA] = (rz == 0.0) ? 0.0 : num / rz;
+      if (rz == 0.0) s[S_FLAG] = 1.0;
+      s[S_RZ] = rzn;
+      break;
+    }
+    case OP_SET_RZ: s[S_RZ] = v[0]; break;
+    default: break;
+  }
+}
+
+// --------------------------------------------------------------------------- updates
+// x += a p ; r -= a q  (a = s[S_ALPHA] * sign); optional z = dinv * r and partial r.z
+template <typename T, int WITH_Z>
+__global__ void __launch_bounds__(BLAS_BLOCK) cg_update_kernel(size_t n, T* __restrict__ x, const T* __restrict__ p,
+                                                              T* __restrict__ r, const T* __restrict__ q,
+                                                              const T* __restrict__ dinv, T* __restrict__ z,
+                                                              const double* __restrict__ s, int update_r,
+                                                              double* __restrict__ partial) {
+  __shared__ double red[BLAS_BLOCK / 64];
+  const T alpha = T(s[S_ALPHA]);
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    x[i] = x[i] + alpha * p[i];
+    if (update_r) {
+      const T ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      if (WITH_Z) {
+        const T zi = dinv[i] * ri;
+        z[i] = zi;
+        acc += double(ri) * double(zi);
+      }
+    }
+  }
+  if (WITH_Z) {
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int k = 0; k < BLAS_BLOCK / 64; ++k) t += red[k];
+      partial[blockIdx.x] = t;
+    }
+  }
+}
+
+// z = dinv * r, partial r.z
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK) jacobi_dot_kernel(size_t n, const T* __restrict__ r, const T* __restrict__ dinv,
+                                                               T* __restrict__ z, double* __restrict__ partial) {
+  __shared__ double red[BLAS_BLOCK / 64];
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    const T zi = dinv[i] * r[i];
+    z[i] = zi;
+    acc += double(r[i]) * double(zi);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < BLAS_BLOCK / 64; ++k) t += red[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// p = z + beta p   (beta = s[S_BETA]); optionally zcopy = z in the same pass
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK) xpby_kernel(size_t n, const T* __restrict__ z, T* __restrict__ p,
+                                                         const double* __restrict__ s, T* __restrict__ zcopy) {
+  const T beta = T(s[S_BETA]);
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    const T zi = z[i];
+    p[i] = zi + beta * p[i];
+    if (zcopy) zcopy[i] = zi;
+  }
+}
+
+// y = a * x (* w if w) + b * y   with host scalars; b == 0 never reads y.
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK) axpby_kernel(size_t n, T a, const T* __restrict__ x, const T* __restrict__ w,
+                                                          T b, T* __restrict__ y) {
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    T v = a * x[i];
+    if (w) v *= w[i];
+    y[i] = (b == T(0)) ? v : v + b * y[i];
+  }
+}
+
+// Precision conversion with scaling: y = (To)(x * scale) ; or accumulate y += (To)x * scale.
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(BLAS_BLOCK) convert_kernel(size_t n, const Ti* __restrict__ x, To* __restrict__ y,
+                                                            double scale, const double* __restrict__ dscale,
+                                                            int inv_scale, int accumulate) {
+  double sc = dscale ? *dscale : scale;
+  if (inv_scale) sc = 1.0 / sc;
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    if (accumulate) y[i] = To(double(y[i]) + double(x[i]) * sc);
+    else y[i] = To(double(x[i]) * sc);
+  }
+}
+
+// Dense symmetric matrix-vector product y = mask * (A r): one wave per row (coarsest level).
+template <typename T>
+__global__ void __launch_bounds__(256) dense_mv_kernel(int n, const T* __restrict__ A, const T* __restrict__ r,
+                                                      T* __restrict__ y, const uint8_t* __restrict__ mask) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const T* a = A + size_t(row) * n;
+  T acc = T(0);
+  for (int j = lane; j < n; j += 64) acc += a[j] * r[j];
+  acc = wave_sum(acc);
+  if (lane == 0) y[row] = (mask && !mask[row]) ? T(0) : acc;
+}
+
+// --------------------------------------------------------------------------- launchers
+static inline int grid_for(size_t n) {
+  size_t g = (n + BLAS_BLOCK - 1) / BLAS_BLOCK;
+  if (g > BLAS_MAX_GRID) g = BLAS_MAX_GRID;
+  if (g < 1) g = 1;
+  return int(g);
+}
+int blas_grid(size_t n) { return grid_for(n); }
+
+template <typename T>
+int reduce_launch(int kind, size_t n, const T* x, const T* y, const T* w, double* partial, hipStream_t st) {
+  const int g = grid_for(n);
+  switch (kind) {
+    case RED_DOT: hipLaunchKernelGGL((reduce_kernel<T, RED_DOT>), dim3(g), dim3(BLAS_BLOCK), 0, st, n, x, y, w, partial); break;
+    case RED_DOT2: hipLaunchKernelGGL((reduce_kernel<T, RED_DOT2>), dim3(g), dim3(BLAS_BLOCK), 0, st, n, x, y, w, partial); break;
+    case RED_DOTDIFF: hipLaunchKernelGGL((reduce_kernel<T, RED_DOTDIFF>), dim3(g), dim3(BLAS_BLOCK), 0, st, n, x, y, w, partial); break;
+    default: hipLaunchKernelGGL((reduce_kernel<T, RED_NORM2>), dim3(g), dim3(BLAS_BLOCK), 0, st, n, x, y, w, partial); break;
+  }
+  return hip_status(hipGetLastError(), "reduce_kernel");
+}
+
+int finish_launch(const double* partial, int nparts, int count, double* s, int dst, int op, hipStream_t st) {
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, st, partial, nparts, count, s, dst, op);
+  return hip_status(hipGetLastError(), "finish_kernel");
+}
+
+template <typename T>
+int cg_update_launch(size_t n, T* x, const T* p, T* r, const T* q, const T* dinv, T* z, const double* s,
+                     int update_r, double* partial, hipStream_t st) {
+  const int g = grid_for(n);
+  if (dinv && update_r)
+    hipLaunchKernelGGL((cg_update_kernel<T, 1>), dim3(g), dim3(BLAS_BLOCK), 0, st, n, x, p, r, q, dinv, z, s, update_r, partial);
+  else
+    hipLaunchKernelGGL((cg_update_kernel<T, 0>), dim3(g), dim3(BLAS_BLOCK), 0, st, n, x, p, r, q, dinv, z, s, update_r, partial);
+  return hip_status(hipGetLastError(), "cg_update_kernel");
+}
+
+template <typename T>
+int jacobi_dot_launch(size_t n, const T* r, const T* dinv, T* z, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(jacobi_dot_kernel<T>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, r, dinv, z, partial);
+  return hip_status(hipGetLastError(), "jacobi_dot_kernel");
+}
+
+template <typename T>
+int xpby_launch(size_t n, const T* z, T* p, const double* s, T* zcopy, hipStream_t st) {
+  hipLaunchKernelGGL(xpby_kernel<T>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, z, p, s, zcopy);
+  return hip_status(hipGetLastError(), "xpby_kernel");
+}
+
+template <typename T>
+int axpby_launch(size_t n, T a, const T* x, const T* w, T b, T* y, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, a, x, w, b, y);
+  return hip_status(hipGetLastError(), "axpby_kernel");
+}
+
+template <typename Ti, typename To>
+int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dscale, int inv_scale, int accumulate,
+                   hipStream_t st) {
+  hipLaunchKernelGGL((convert_kernel<Ti, To>), dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, x, y, scale, dscale,
+                     inv_scale, accumulate);
+  return hip_status(hipGetLastError(), "convert_kernel");
+}
+
+template <typename T>
+int dense_mv_launch(int n, const T* A, const T* r, T* y, const uint8_t* mask, hipStream_t st) {
+  hipLaunchKernelGGL(dense_mv_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, A, r, y, mask);
+  return hip_status(hipGetLastError(), "dense_mv_kernel");
+}
+
+#define VH_INST(T)                                                                                           \
+  template int reduce_launch<T>(int, size_t, const T*, const T*, const T*, double*, hipStream_t);            \
+  template int cg_update_launch<T>(size_t, T*, const T*, T*, const T*, const T*, T*, const double*, int, double*, hipStream_t); \
+  template int jacobi_dot_launch<T>(size_t, const T*, const T*, T*, double*, hipStream_t);                    \
+  template int xpby_launch<T>(size_t, const T*, T*, const double*, T*, hipStream_t);                              \
+  template int axpby_launch<T>(size_t, T, const T*, const T*, T, T*, hipStream_t);                            \
+  template int dense_mv_launch<T>(int, const T*, const T*, T*, const uint8_t*, hipStream_t);
+VH_INST(double)
+VH_INST(float)
+#undef VH_INST
+template int convert_launch<double, float>(size_t, const double*, float*, double, const double*, int, int, hipStream_t);
+template int convert_launch<float, double>(size_t, const float*, double*, double, const double*, int, int, hipStream_t);
+template int convert_launch<double, double>(size_t, const double*, double*, double, const double*, int, int, hipStream_t);
+template int convert_launch<float, float>(size_t, const float*, float*, double, const double*, int, int, hipStream_t);
+
+}  // namespace vh

@@ -1,0 +1,399 @@
+// C-ABI entry points of libvoxelhip.so: argument checks, dtype dispatch, host-side setup.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "voxelhip.h"
+#include "common.h"
+#include "matvec.h"
+#include "grid_ops.h"
+#include "blas.h"
+#include "simp.h"
+
+namespace vh {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return fail(-1, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Modal algebra of the 8-node brick (binary lane l = x + 2y + 4z; native order of ke).
+static const int kNativeLane[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+static int modal_blocks(const double* ke, double tol, double* blocks, double* discarded) {
+  // full = T ke T^T / 64 with T[3p+c][3n+c] = (-1)^popcount(p & lane(n))
+  static double full[24][24];
+  double tmp[24][24];
+  auto h = [](int p, int l) { return (__builtin_popcount(p & l) & 1) ? -1.0 : 1.0; };
+  for (int i = 0; i < 24; ++i)      // tmp = T ke
+    for (int j = 0; j < 24; ++j) {
+      const int p = i / 3, c = i % 3;
+      double s = 0.0;
+      for (int n = 0; n < 8; ++n) s += h(p, kNativeLane[n]) * ke[(3 * n + c) * 24 + j];
+      tmp[i][j] = s;
+    }
+  for (int i = 0; i < 24; ++i)      // full = tmp T^T / 64
+    for (int j = 0; j < 24; ++j) {
+      const int p = j / 3, c = j % 3;
+      double s = 0.0;
+      for (int n = 0; n < 8; ++n) s += tmp[i][3 * n + c] * h(p, kNativeLane[n]);
+      full[i][j] = s / 64.0;
+    }
+  bool in_block[24][24] = {};
+  for (int g = 0; g < 8; ++g) {
+    const int idx[3] = {3 * (g ^ 1), 3 * (g ^ 2) + 1, 3 * (g ^ 4) + 2};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        blocks[9 * g + 3 * a + b] = full[idx[a]][idx[b]];
+        in_block[idx[a]][idx[b]] = true;
+      }
+  }
+  double out = 0.0, all = 0.0;
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) {
+      all += full[i][j] * full[i][j];
+      if (!in_block[i][j]) out += full[i][j] * full[i][j];
+    }
+  const double rel = all > 0 ? std::sqrt(out / all) : NAN;
+  if (discarded) *discarded = rel;
+  if (!std::isfinite(rel) || rel > tol)
+    return fail(-3, "Element is outside the declared parity-block model: %.3e", rel);
+  return 0;
+}
+
+static bool dims_ok(int nx, int ny, int nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
+static bool dtype_ok(int d) { return d == F64 || d == F32; }
+
+}  // namespace vh
+
+using namespace vh;
+
+#define VH_REQUIRE(cond, ...) \
+  do {                        \
+    if (!(cond)) return fail(-2, __VA_ARGS__); \
+  } while (0)
+
+template <typename T>
+static int matvec_t(int storage, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny, int nz,
+                    int mask_kind, const uint8_t* mask, const vh_epilogue* ep, hipStream_t st) {
+  MatvecParams<T> p{};
+  p.u = u;
+  p.E = static_cast<const T*>(E);
+  p.y = static_cast<T*>(y);
+  p.mask = mask;
+  p.nx = nx; p.ny = ny; p.nz = nz;
+  for (int i = 0; i < 72; ++i) p.coef[i] = T(blocks72[i]);
+  int mode = MODE_RAW;
+  if (ep) {
+    mode = ep->mode;
+    p.b = static_cast<const T*>(ep->b);
+    p.dinv = static_cast<const T*>(ep->dinv);
+    p.d = static_cast<T*>(ep->d);
+    p.partial = ep->partial;
+    p.omega = T(ep->omega);
+    p.cheb_a = T(ep->cheb_a);
+    p.cheb_c = T(ep->cheb_c);
+    p.lx = ep->lx;
+    p.fixed_zero = ep->fixed_zero;
+  }
+  return matvec_launch<T>(p, storage, mode, mask_kind, st);
+}
+
+template <typename T>
+static GridArgs<T> grid_args(int nx, int ny, int nz, int mask_kind, const uint8_t* mask, const double* ke576,
+                             const double* blocks72) {
+  GridArgs<T> g{};
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.mask_kind = mask_kind;
+  g.mask = mask;
+  if (ke576)
+    for (int i = 0; i < 24; ++i) g.kediag[i] = T(ke576[i * 24 + i]);
+  if (blocks72)
+    for (int i = 0; i < 72; ++i) g.coef[i] = T(blocks72[i]);
+  return g;
+}
+
+extern "C" {
+
+int vh_abi_version(void) { return VH_ABI_VERSION; }
+const char* vh_last_error(void) { return g_err; }
+
+int vh_modal_blocks(const double* ke576, double tolerance, double* blocks72, double* discarded) {
+  VH_REQUIRE(ke576 && blocks72, "vh_modal_blocks: null pointer");
+  return modal_blocks(ke576, tolerance, blocks72, discarded);
+}
+
+int vh_matvec_tiles(int nx, int ny, int nz, int lx, int* n_tiles, int* n_workgroups) {
+  VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec_tiles: bad grid %d x %d x %d", nx, ny, nz);
+  int ty, tz, cx, l, nt, grid;
+  matvec_geometry(nx, ny, nz, lx, &ty, &tz, &cx, &l, &nt, &grid);
+  if (n_tiles) *n_tiles = nt;
+  if (n_workgroups) *n_workgroups = grid;
+  return 0;
+}
+
+
+int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y, int nx,
+                    int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype), "vh_matvec: dtype %d not supported", dtype);
+  VH_REQUIRE(storage == dtype || (storage == BF16 && dtype == F32), "vh_matvec: storage %d with dtype %d", storage, dtype);
+  VH_REQUIRE(blocks72 && E && u && y, "vh_matvec: null pointer");
+  VH_REQUIRE(u != y, "vh_matvec: y must not alias u");
+  VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec: bad grid %d x %d x %d", nx, ny, nz);
+  VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2, "vh_matvec: bad mask kind %d", mask_kind);
+  VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_matvec: mask array missing");
+  if (ep) {
+    VH_REQUIRE(ep->mode >= 0 && ep->mode <= MODE_DOT, "vh_matvec: bad mode %d", ep->mode);
+    const int m = ep->mode;
+    VH_REQUIRE(!(m == MODE_RESID || m == MODE_JACOBI || m == MODE_CHEB) || ep->b, "vh_matvec: b missing");
+    VH_REQUIRE(!(m == MODE_JACOBI || m == MODE_CHEB) || ep->dinv, "vh_matvec: dinv missing");
+    VH_REQUIRE(m != MODE_CHEB || ep->d, "vh_matvec: d missing");
+    VH_REQUIRE(m != MODE_DOT || ep->partial, "vh_matvec: partial missing");
+    VH_REQUIRE(ep->b != y && ep->dinv != y, "vh_matvec: y must not alias an input");
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64) return matvec_t<double>(storage, blocks72, E, u, y, nx, ny, nz, mask_kind, mask, ep, st);
+  return matvec_t<float>(storage, blocks72, E, u, y, nx, ny, nz, mask_kind, mask, ep, st);
+}
+
+int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny,
+              int nz, int mask_kind, const uint8_t* mask, void* stream) {
+  return vh_matvec_fused(dtype, storage, blocks72, E, u, y, nx, ny, nz, mask_kind, mask, nullptr, stream);
+}
+
+int vh_dense_product(int dtype, const double* ke576, const void* E, const void* u, void* y, int nx, int ny, int nz,
+                     void* stream) {
+  VH_REQUIRE(dtype_ok(dtype), "vh_dense_product: dtype %d", dtype);
+  VH_REQUIRE(ke576 && E && u && y && u != y, "vh_dense_product: bad pointers");
+  VH_REQUIRE(dims_ok(nx, ny, nz), "vh_dense_product: bad grid");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // ke goes to the device through a small persistent buffer per dtype
+  static double* dk64 = nullptr;
+  static float* dk32 = nullptr;
+  if (dtype == F64) {
+    if (!dk64 && hipMalloc(&dk64, 576 * sizeof(double)) != hipSuccess) return fail(-1, "hipMalloc ke");
+    if (int r = hip_status(hipMemcpy(dk64, ke576, 576 * sizeof(double), hipMemcpyHostToDevice), "ke copy")) return r;
+    return dense_launch<double>(static_cast<const double*>(u), static_cast<const double*>(E), dk64,
+                                static_cast<double*>(y), nx, ny, nz, st);
+  }
+  float k32[576];
+  for (int i = 0; i < 576; ++i) k32[i] = float(ke576[i]);
+  if (!dk32 && hipMalloc(&dk32, 576 * sizeof(float)) != hipSuccess) return fail(-1, "hipMalloc ke");
+  if (int r = hip_status(hipMemcpy(dk32, k32, sizeof(k32), hipMemcpyHostToDevice), "ke copy")) return r;
+  return dense_launch<float>(static_cast<const float*>(u), static_cast<const float*>(E), dk32, static_cast<float*>(y),
+                             nx, ny, nz, st);
+}
+
+
+int vh_diagonal(int dtype, const double* ke576, const void* E, void* out, int nx, int ny, int nz, int mask_kind,
+                const uint8_t* mask, int invert, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && ke576 && E && out && dims_ok(nx, ny, nz), "vh_diagonal: bad arguments");
+  VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2 && (mask_kind != MASK_ARRAY || mask), "vh_diagonal: bad mask");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return diag_launch<double>(static_cast<const double*>(E), grid_args<double>(nx, ny, nz, mask_kind, mask, ke576, nullptr),
+                               static_cast<double*>(out), invert, st);
+  return diag_launch<float>(static_cast<const float*>(E), grid_args<float>(nx, ny, nz, mask_kind, mask, ke576, nullptr),
+                            static_cast<float*>(out), invert, st);
+}
+
+int vh_element_energy(int dtype, const double* blocks72, const void* u, void* ce, int nx, int ny, int nz, int mask_kind,
+                      const uint8_t* mask, const void* rho, void* dc, double penal, double dE, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && blocks72 && u && ce && dims_ok(nx, ny, nz), "vh_element_energy: bad arguments");
+  VH_REQUIRE(!dc || rho, "vh_element_energy: rho missing");
+  VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2 && (mask_kind != MASK_ARRAY || mask), "vh_element_energy: bad mask");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return energy_launch<double>(static_cast<const double*>(u), grid_args<double>(nx, ny, nz, mask_kind, mask, nullptr, blocks72),
+                                 static_cast<double*>(ce), static_cast<const double*>(rho), static_cast<double*>(dc), penal, dE, st);
+  return energy_launch<float>(static_cast<const float*>(u), grid_args<float>(nx, ny, nz, mask_kind, mask, nullptr, blocks72),
+                              static_cast<float*>(ce), static_cast<const float*>(rho), static_cast<float*>(dc), float(penal),
+                              float(dE), st);
+}
+
+int vh_restrict(int dtype, const void* rf, void* rc, int nx, int ny, int nz, int fmask_kind, const uint8_t* fmask,
+                int cmask_kind, const uint8_t* cmask, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && rf && rc && rf != rc && dims_ok(nx, ny, nz), "vh_restrict: bad arguments");
+  VH_REQUIRE(nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, "vh_restrict: fine grid %d x %d x %d not even", nx, ny, nz);
+  VH_REQUIRE((fmask_kind != MASK_ARRAY || fmask) && (cmask_kind != MASK_ARRAY || cmask), "vh_restrict: mask missing");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return restrict_launch<double>(static_cast<const double*>(rf), static_cast<double*>(rc),
+                                   grid_args<double>(nx, ny, nz, fmask_kind, fmask, nullptr, nullptr), cmask, cmask_kind, st);
+  return restrict_launch<float>(static_cast<const float*>(rf), static_cast<float*>(rc),
+                                grid_args<float>(nx, ny, nz, fmask_kind, fmask, nullptr, nullptr), cmask, cmask_kind, st);
+}
+
+int vh_prolong(int dtype, const void* ec, void* xf, int nx, int ny, int nz, int mask_kind, const uint8_t* mask,
+               int assign, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && ec && xf && ec != xf && dims_ok(nx, ny, nz), "vh_prolong: bad arguments");
+  VH_REQUIRE(nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, "vh_prolong: fine grid %d x %d x %d not even", nx, ny, nz);
+  VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_prolong: mask missing");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return prolong_launch<double>(static_cast<const double*>(ec), static_cast<double*>(xf),
+                                  grid_args<double>(nx, ny, nz, mask_kind, mask, nullptr, nullptr), assign, st);
+  return prolong_launch<float>(static_cast<const float*>(ec), static_cast<float*>(xf),
+                               grid_args<float>(nx, ny, nz, mask_kind, mask, nullptr, nullptr), assign, st);
+}
+
+int vh_coarsen_modulus(int dtype, const void* Ef, void* Ec, int nx, int ny, int nz, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && Ef && Ec && Ef != Ec && dims_ok(nx, ny, nz), "vh_coarsen_modulus: bad arguments");
+  VH_REQUIRE(nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, "vh_coarsen_modulus: grid not even");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return coarsen_modulus_launch<double>(static_cast<const double*>(Ef), static_cast<double*>(Ec), nx, ny, nz, st);
+  return coarsen_modulus_launch<float>(static_cast<const float*>(Ef), static_cast<float*>(Ec), nx, ny, nz, st);
+}
+
+int vh_dense_mv(int dtype, int n, const void* A, const void* r, void* y, const uint8_t* mask, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && n >= 1 && A && r && y && r != y, "vh_dense_mv: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return dense_mv_launch<double>(n, static_cast<const double*>(A), static_cast<const double*>(r), static_cast<double*>(y),
+                                   mask, st);
+  return dense_mv_launch<float>(n, static_cast<const float*>(A), static_cast<const float*>(r), static_cast<float*>(y), mask,
+                                st);
+}
+
+int vh_reduce_parts(size_t n) { return blas_grid(n); }
+
+int vh_reduce(int dtype, int kind, size_t n, const void* x, const void* y, const void* w, double* partial,
+              void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && x && partial && n >= 1, "vh_reduce: bad arguments");
+  VH_REQUIRE(kind >= 0 && kind <= 3, "vh_reduce: bad kind");
+  VH_REQUIRE(kind == RED_NORM2 || y, "vh_reduce: y missing");
+  VH_REQUIRE(!(kind == RED_DOT2 || kind == RED_DOTDIFF) || w, "vh_reduce: w missing");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return reduce_launch<double>(kind, n, static_cast<const double*>(x), static_cast<const double*>(y),
+                                 static_cast<const double*>(w), partial, st);
+  return reduce_launch<float>(kind, n, static_cast<const float*>(x), static_cast<const float*>(y),
+                              static_cast<const float*>(w), partial, st);
+}
+
+int vh_finish(const double* partial, int nparts, int count, double* scal, int dst, int op, void* stream) {
+  VH_REQUIRE(partial && scal && nparts >= 1 && count >= 1 && count <= 2 && dst >= 0, "vh_finish: bad arguments");
+  return finish_launch(partial, nparts, count, scal, dst, op, static_cast<hipStream_t>(stream));
+}
+
+int vh_cg_update(int dtype, size_t n, void* x, const void* p, void* r, const void* q, const void* dinv, void* z,
+                 const double* scal, int update_r, double* partial, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && x && p && scal && n >= 1, "vh_cg_update: bad arguments");
+  VH_REQUIRE(!update_r || (r && q), "vh_cg_update: r/q missing");
+  VH_REQUIRE(!(update_r && dinv) || (z && partial), "vh_cg_update: z/partial missing");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return cg_update_launch<double>(n, static_cast<double*>(x), static_cast<const double*>(p), static_cast<double*>(r),
+                                    static_cast<const double*>(q), static_cast<const double*>(dinv), static_cast<double*>(z),
+                                    scal, update_r, partial, st);
+  return cg_update_launch<float>(n, static_cast<float*>(x), static_cast<const float*>(p), static_cast<float*>(r),
+                                 static_cast<const float*>(q), static_cast<const float*>(dinv), static_cast<float*>(z), scal,
+                                 update_r, partial, st);
+}
+
+int vh_jacobi_dot(int dtype, size_t n, const void* r, const void* dinv, void* z, double* partial, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && r && dinv && z && partial && n >= 1, "vh_jacobi_dot: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return jacobi_dot_launch<double>(n, static_cast<const double*>(r), static_cast<const double*>(dinv),
+                                     static_cast<double*>(z), partial, st);
+  return jacobi_dot_launch<float>(n, static_cast<const float*>(r), static_cast<const float*>(dinv), static_cast<float*>(z),
+                                  partial, st);
+}
+
+int vh_xpby(int dtype, size_t n, const void* z, void* p, const double* scal, void* zcopy, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && z && p && scal && n >= 1, "vh_xpby: bad arguments");
+  VH_REQUIRE(zcopy != p && zcopy != z, "vh_xpby: zcopy aliases an operand");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return xpby_launch<double>(n, static_cast<const double*>(z), static_cast<double*>(p), scal, static_cast<double*>(zcopy), st);
+  return xpby_launch<float>(n, static_cast<const float*>(z), static_cast<float*>(p), scal, static_cast<float*>(zcopy), st);
+}
+
+int vh_axpby(int dtype, size_t n, double a, const void* x, const void* w, double b, void* y, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && x && y && n >= 1, "vh_axpby: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return axpby_launch<double>(n, a, static_cast<const double*>(x), static_cast<const double*>(w), b, static_cast<double*>(y), st);
+  return axpby_launch<float>(n, float(a), static_cast<const float*>(x), static_cast<const float*>(w), float(b),
+                             static_cast<float*>(y), st);
+}
+
+int vh_convert(int dtype_in, int dtype_out, size_t n, const void* x, void* y, double scale, const double* dscale,
+               int inv_scale, int accumulate, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype_in) && dtype_ok(dtype_out) && x && y && n >= 1, "vh_convert: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype_in == F64 && dtype_out == F32)
+    return convert_launch<double, float>(n, static_cast<const double*>(x), static_cast<float*>(y), scale, dscale, inv_scale, accumulate, st);
+  if (dtype_in == F32 && dtype_out == F64)
+    return convert_launch<float, double>(n, static_cast<const float*>(x), static_cast<double*>(y), scale, dscale, inv_scale, accumulate, st);
+  if (dtype_in == F64)
+    return convert_launch<double, double>(n, static_cast<const double*>(x), static_cast<double*>(y), scale, dscale, inv_scale, accumulate, st);
+  return convert_launch<float, float>(n, static_cast<const float*>(x), static_cast<float*>(y), scale, dscale, inv_scale, accumulate, st);
+}
+
+int vh_cone_filter(int dtype, const void* x, const void* pre_div, const void* post_div, void* y, int nx, int ny, int nz,
+                   double rmin, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && x && y && x != y && dims_ok(nx, ny, nz), "vh_cone_filter: bad arguments");
+  VH_REQUIRE(rmin > 0 && rmin < 3.0, "vh_cone_filter: radius %g outside (0, 3)", rmin);
+  const int R = int(std::ceil(rmin)) - 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto fill = [&](auto& c) {
+    c.nx = nx; c.ny = ny; c.nz = nz; c.n_off = 0;
+    for (int dx = -R; dx <= R; ++dx)
+      for (int dy = -R; dy <= R; ++dy)
+        for (int dz = -R; dz <= R; ++dz) {
+          const double w = rmin - std::sqrt(double(dx * dx + dy * dy + dz * dz));
+          if (w > 0) {
+            c.dx[c.n_off] = dx; c.dy[c.n_off] = dy; c.dz[c.n_off] = dz;
+            c.w[c.n_off] = w;
+            ++c.n_off;
+          }
+        }
+  };
+  if (dtype == F64) {
+    ConeArgs<double> c{};
+    fill(c);
+    return cone_launch<double>(static_cast<const double*>(x), static_cast<const double*>(pre_div),
+                               static_cast<const double*>(post_div), static_cast<double*>(y), c, st);
+  }
+  ConeArgs<float> c{};
+  fill(c);
+  return cone_launch<float>(static_cast<const float*>(x), static_cast<const float*>(pre_div),
+                            static_cast<const float*>(post_div), static_cast<float*>(y), c, st);
+}
+
+int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w, double lmid,
+                double move, void* xnew, double* partial, int nparts, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && x && dc && partial && n >= 1 && nparts >= 1, "vh_oc_probe: bad arguments");
+  VH_REQUIRE(lmid > 0, "vh_oc_probe: multiplier must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return oc_probe_launch<double>(n, static_cast<const double*>(x), static_cast<const double*>(dc),
+                                   static_cast<const double*>(dv), static_cast<const double*>(w), lmid, move,
+                                   static_cast<double*>(xnew), partial, nparts, st);
+  return oc_probe_launch<float>(n, static_cast<const float*>(x), static_cast<const float*>(dc),
+                                static_cast<const float*>(dv), static_cast<const float*>(w), float(lmid), float(move),
+                                static_cast<float*>(xnew), partial, nparts, st);
+}
+
+}  // extern "C"

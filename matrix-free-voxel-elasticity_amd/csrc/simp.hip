@@ -1,0 +1,81 @@
+// SIMP kernels around the solve: the cone density filter and the OC bisection probe.
+//
+// Filter (experiments/session_a/donor/src/gpu_fem/filter_r2.py:40-183): y_e = sum over the
+// in-grid neighbours e+d with |d| < r of (r - |d|) x_{e+d}; forward = H x / s, transpose =
+// H (g / s) (H is symmetric), s = H 1.  The offset list and weights are passed from the host.
+//
+// OC probe (experiments/session_h/run_w1b.py:46-62, 82-94): for a trial multiplier l,
+// x_new = clip(x sqrt(max(-dc/dv/l, 0)), max(x - move, 0), min(x + move, 1)) and the filtered
+// volume mean(H x_new / s).  Because H is symmetric, mean(H x_new / s) = sum_j x_new_j w_j / n
+// with w = H (1/s), so one probe is a single fused elementwise pass plus a deterministic
+// reduction instead of a filter application.
+#include "common.h"
+#include "simp.h"
+
+namespace vh {
+
+template <typename T>
+__global__ void __launch_bounds__(256) cone_kernel(const T* __restrict__ x, const T* __restrict__ pre_div,
+                                                  const T* __restrict__ post_div, T* __restrict__ y,
+                                                  const ConeArgs<T> c) {
+  const size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  const int nx = c.nx, ny = c.ny, nz = c.nz;
+  if (e >= size_t(nx) * ny * nz) return;
+  const int ez = int(e % nz), ey = int((e / nz) % ny), ex = int(e / (size_t(nz) * ny));
+  T acc = T(0);
+  for (int k = 0; k < c.n_off; ++k) {
+    const int jx = ex + c.dx[k], jy = ey + c.dy[k], jz = ez + c.dz[k];
+    if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
+    const size_t j = (size_t(jx) * ny + jy) * nz + jz;
+    T v = x[j];
+    if (pre_div) v = v / pre_div[j];
+    acc += c.w[k] * v;
+  }
+  if (post_div) acc = acc / post_div[e];
+  y[e] = acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) oc_probe_kernel(size_t n, const T* __restrict__ x, const T* __restrict__ dc,
+                                                      const T* __restrict__ dv, const T* __restrict__ w, T lmid,
+                                                      T move, T* __restrict__ xnew, double* __restrict__ partial) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const T xi = x[i];
+    const T ratio = -dc[i] / (dv ? dv[i] : T(1)) / lmid;
+    T v = xi * sqrt(ratio > T(0) ? ratio : T(0));
+    const T lo = xi - move > T(0) ? xi - move : T(0);
+    const T hi = xi + move < T(1) ? xi + move : T(1);
+    v = v < lo ? lo : (v > hi ? hi : v);
+    if (xnew) xnew[i] = v;
+    acc += double(v) * double(w ? w[i] : T(1));
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+template <typename T>
+int cone_launch(const T* x, const T* pre_div, const T* post_div, T* y, const ConeArgs<T>& c, hipStream_t st) {
+  const size_t ne = size_t(c.nx) * c.ny * c.nz;
+  hipLaunchKernelGGL(cone_kernel<T>, dim3(int((ne + 255) / 256)), dim3(256), 0, st, x, pre_div, post_div, y, c);
+  return hip_status(hipGetLastError(), "cone_kernel");
+}
+
+template <typename T>
+int oc_probe_launch(size_t n, const T* x, const T* dc, const T* dv, const T* w, T lmid, T move, T* xnew,
+                    double* partial, int nparts, hipStream_t st) {
+  hipLaunchKernelGGL(oc_probe_kernel<T>, dim3(nparts), dim3(256), 0, st, n, x, dc, dv, w, lmid, move, xnew, partial);
+  return hip_status(hipGetLastError(), "oc_probe_kernel");
+}
+
+template int cone_launch<double>(const double*, const double*, const double*, double*, const ConeArgs<double>&, hipStream_t);
+template int cone_launch<float>(const float*, const float*, const float*, float*, const ConeArgs<float>&, hipStream_t);
+template int oc_probe_launch<double>(size_t, const double*, const double*, const double*, const double*, double, double,
+                                     double*, double*, int, hipStream_t);
+template int oc_probe_launch<float>(size_t, const float*, const float*, const float*, const float*, float, float, float*,
+                                    double*, int, hipStream_t);
+
+}  // namespace vh

@@ -1,0 +1,379 @@
+"""Drop-in for the reference's src/voxel_mg.py on an MI355X: geometric multigrid-preconditioned
+flexible CG for voxel elasticity, with every vector operation in HIP kernels.
+
+Same names, signatures and return values as src/voxel_mg.py:27-244.  `xp` is `voxelhip.xp`
+(the reference passes numpy or cupy); vectors are torch tensors on the HIP device.
+
+  XYZ, edof_author(shape)                      host index helpers (unchanged meaning)
+  HipProduct(shape, ke, dtype)                 the product plugin: raw(u, E), diagonal(E) on the
+                                               GPU (the role of NumpyProduct / CustomProduct,
+                                               src/voxel_mg.py:41-58, session_d/run_gpu.py:42-65)
+  assemble_dense(shape, ke, E, mask)           dense constrained coarsest matrix (host setup,
+                                               as the reference does before np.linalg.cholesky)
+  prolong / restrict / coarsen_modulus / coarsen_mask   HIP transfer kernels
+  Level, Multigrid, fcg, hierarchy_shapes, spectral_omegas
+
+Extensions over the reference (all default to the reference's behaviour):
+  Multigrid(..., smoother="chebyshev", lambdas=..., lower_frac=1/30, cheb_variant="saad")
+      Chebyshev-Jacobi smoothing of degree pre/post (Saad Alg. 12.1; "donor" reproduces the
+      recurrence of donor_yang_gmg multigrid_v4.py:860-889).
+  The coarsest solve applies the explicit inverse of the coarsest operator (formed once on the
+  device from the Cholesky factor) with one dense matrix-vector kernel, where the reference runs
+  two triangular solves (src/voxel_mg.py:152-159).
+  fcg keeps alpha, beta and r.z on the device and synchronises with the host only when it tests
+  the true residual (every `check_every` iterations).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import voxelhip
+from voxelhip import _lib as L
+from voxelhip import ops
+from voxel_kernels import Modal
+
+XYZ = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+
+_TD = {np.float64: torch.float64, np.float32: torch.float32,
+       torch.float64: torch.float64, torch.float32: torch.float32}
+
+
+def _tdt(dtype):
+    if isinstance(dtype, torch.dtype):
+        return _TD[dtype]
+    return _TD[np.dtype(dtype).type]
+
+
+def edof_author(shape):
+    """(n_el, 24) author-layout DOFs in the native node order of ke (src/voxel_mg.py:32-38)."""
+    nx, ny, nz = (int(s) for s in shape)
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    base = (ex.ravel() * (ny + 1) + ey.ravel()) * (nz + 1) + ez.ravel()
+    off = (XYZ[:, 0] * (ny + 1) + XYZ[:, 1]) * (nz + 1) + XYZ[:, 2]
+    nodes = base[:, None] + off[None, :]
+    return (3 * nodes[:, :, None] + np.arange(3)).reshape(-1, 24)
+
+
+# ------------------------------------------------------------------------------- product
+class HipProduct:
+    """y = K(E) u with the parity-block HIP kernel.  `raw(u, E)` expects constrained DOFs already
+    zeroed (src/voxel_mg.py:42).  `storage="bf16"` stores u in bfloat16 (FP32 compute)."""
+
+    def __init__(self, shape, ke, dtype=np.float64, storage=None):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = _tdt(dtype)
+        self.ke = np.asarray(ke, dtype=np.float64).reshape(24, 24)
+        if self.dtype == torch.float32:     # the reference casts ke to the product dtype
+            self.ke = self.ke.astype(np.float32).astype(np.float64)
+        self.blocks = Modal(self.ke).blocks
+        self.ndof = 3 * ops.n_nodes(self.shape)
+        self.storage = L.BF16 if storage == "bf16" else None
+
+    def raw(self, u, E, out=None, mask=None, fixed_zero=False):
+        y = out if out is not None else torch.empty(self.ndof, dtype=self.dtype, device=u.device)
+        ops.matvec(self.blocks, E, u, y, self.shape, mask=mask, fixed_zero=fixed_zero, storage=self.storage)
+        return y
+
+    def diagonal(self, E, out=None, mask=None, invert=False):
+        d = out if out is not None else torch.empty(self.ndof, dtype=self.dtype, device=E.device)
+        return ops.diagonal(self.ke, E, d, self.shape, mask=mask, invert=invert)
+
+
+def assemble_dense(shape, ke, E, mask):
+    """Dense M K M + (I - M) in FP64 on the host (src/voxel_mg.py:61-70); setup only."""
+    import scipy.sparse as sp
+    E = E.detach().cpu().numpy() if isinstance(E, torch.Tensor) else np.asarray(E)
+    m = mask.detach().cpu().numpy() if isinstance(mask, torch.Tensor) else np.asarray(mask)
+    m = m.astype(np.float64)
+    edof = edof_author(shape)
+    n = 3 * ops.n_nodes(shape)
+    vals = (E[:, None, None] * np.asarray(ke, dtype=np.float64)[None]).ravel()
+    rows = np.repeat(edof, 24, axis=1).ravel()
+    cols = np.tile(edof, (1, 24)).ravel()
+    K = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).toarray()
+    return m[:, None] * K * m[None, :] + np.diag(1.0 - m)
+
+
+# ------------------------------------------------------------------------------- transfers
+def prolong(xp, uc, coarse_shape):
+    nx, ny, nz = (int(s) for s in coarse_shape)
+    fine = (2 * nx, 2 * ny, 2 * nz)
+    out = torch.empty(3 * ops.n_nodes(fine), dtype=uc.dtype, device=uc.device)
+    return ops.prolong(uc.contiguous(), out, fine, assign=True)
+
+
+def restrict(xp, rf, fine_shape):
+    nx, ny, nz = (int(s) for s in fine_shape)
+    out = torch.empty(3 * ops.n_nodes((nx // 2, ny // 2, nz // 2)), dtype=rf.dtype, device=rf.device)
+    return ops.restrict(rf.contiguous(), out, fine_shape)
+
+
+def coarsen_modulus(xp, E, shape):
+    nx, ny, nz = (int(s) for s in shape)
+    out = torch.empty((nx // 2) * (ny // 2) * (nz // 2), dtype=E.dtype, device=E.device)
+    return ops.coarsen_modulus(E.contiguous(), out, shape)
+
+
+def coarsen_mask(xp, mask, shape):
+    nx, ny, nz = (int(s) for s in shape)
+    return mask.reshape(nx + 1, ny + 1, nz + 1, 3)[::2, ::2, ::2, :].contiguous().reshape(-1)
+
+
+def hierarchy_shapes(shape, max_coarse_x=16):
+    """Halve every direction until x <= max_coarse_x (all extents stay integers) (src/voxel_mg.py:220-225)."""
+    shapes = [tuple(int(s) for s in shape)]
+    while shapes[-1][0] > max_coarse_x and all(s % 2 == 0 for s in shapes[-1]):
+        shapes.append(tuple(s // 2 for s in shapes[-1]))
+    return shapes
+
+
+# ------------------------------------------------------------------------------- hierarchy
+class Level:
+    """Constrained operator A u = M K M u + (I - M) u (src/voxel_mg.py:120-130)."""
+
+    def __init__(self, xp, shape, product, E, mask, dtype):
+        self.xp, self.shape, self.product = xp, tuple(int(s) for s in shape), product
+        self.dtype = _tdt(dtype)
+        dev = voxelhip.device()
+        self.E = voxelhip.to_device(E, self.dtype)
+        self.maskobj = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape, dev)
+        self.mask = self.maskobj.values(self.shape, self.dtype, dev)
+        self.fixed = 1 - self.mask
+        self.n = 3 * ops.n_nodes(self.shape)
+        self.fused = hasattr(product, "blocks")
+        if self.fused:
+            self.inv_diag = product.diagonal(self.E, mask=self.maskobj, invert=True)
+        else:
+            d = product.diagonal(self.E)
+            self.inv_diag = (1 / torch.where(self.mask != 0, d, torch.ones_like(d))).to(self.dtype)
+
+    def apply(self, u, out=None):
+        y = out if out is not None else torch.empty(self.n, dtype=self.dtype, device=u.device)
+        if self.fused:
+            return ops.matvec(self.product.blocks, self.E, u, y, self.shape, mask=self.maskobj,
+                              storage=self.product.storage)
+        t = ops.axpby(1.0, u, 0.0, torch.empty_like(u), w=self.mask)
+        y.copy_(self.product.raw(t, self.E))
+        ops.axpby(1.0, y, 0.0, y, w=self.mask)
+        return ops.axpby(1.0, u, 1.0, y, w=self.fixed)
+
+    # fused passes used by the V-cycle and the Krylov loops
+    def matvec(self, u, y, **kw):
+        if not self.fused:
+            raise TypeError("fused passes need a HipProduct")
+        return ops.matvec(self.product.blocks, self.E, u, y, self.shape, mask=self.maskobj,
+                          storage=self.product.storage, **kw)
+
+
+def _cheb_params(lam, lower_frac):
+    lmin = lower_frac * lam
+    return 0.5 * (lam + lmin), 0.5 * (lam - lmin)
+
+
+class Multigrid:
+    """V-cycle preconditioner (src/voxel_mg.py:133-178).  `products` is a list of per-level
+    product objects (finest first) exposing raw(u, E) and diagonal(E); `coarse_factor` is the
+    lower Cholesky factor of the coarsest constrained operator (host or device)."""
+
+    def __init__(self, xp, shapes, products, E_fine, mask_fine, coarse_factor, dtype,
+                 pre=4, post=4, omegas=None, smoother="jacobi", lambdas=None,
+                 lower_frac=1.0 / 30.0, cheb_variant="saad"):
+        self.xp, self.pre, self.post = xp, pre, post
+        self.dtype = _tdt(dtype)
+        dev = voxelhip.device()
+        self.levels = []
+        E = voxelhip.to_device(E_fine, self.dtype)
+        mask = mask_fine if isinstance(mask_fine, ops.Mask) else ops.Mask.from_values(mask_fine, shapes[0], dev)
+        for i, (shape, prod) in enumerate(zip(shapes, products)):
+            if i:
+                E = coarsen_modulus(xp, E, shapes[i - 1])
+                mask = mask.coarsen(shapes[i - 1])
+            self.levels.append(Level(xp, shape, prod, E, mask, self.dtype))
+        n_smooth = len(self.levels) - 1
+        self.omegas = list(omegas) if omegas is not None else [0.6] * n_smooth
+        assert len(self.omegas) == n_smooth
+        self.smoother = smoother
+        self.lambdas = list(lambdas) if lambdas is not None else None
+        self.lower_frac = lower_frac
+        if cheb_variant not in ("saad", "donor"):
+            raise ValueError(cheb_variant)
+        self.cheb_variant = cheb_variant
+        # coarsest: explicit inverse A^-1 = L^-T L^-1 of the constrained operator, in FP64
+        Lf = coarse_factor if isinstance(coarse_factor, torch.Tensor) else torch.from_numpy(np.asarray(coarse_factor))
+        Lf = Lf.to(dev, torch.float64)
+        self.coarse_inverse = torch.cholesky_inverse(Lf).to(self.dtype).contiguous()
+        cl = self.levels[-1]
+        self._coarse_mask = ops.Mask(L.MASK_ARRAY, (cl.mask != 0).to(torch.uint8).contiguous())
+        # work vectors per level: x, x2 (ping-pong), res, d (Chebyshev), rhs (levels > 0)
+        self._work = []
+        for lv in self.levels:
+            z = lambda: torch.zeros(lv.n, dtype=self.dtype, device=dev)  # noqa: E731
+            self._work.append({"x": z(), "x2": z(), "res": z(), "d": z(), "rhs": z()})
+
+    # ---------------------------------------------------------------- smoothing
+    def _smooth(self, level, x, r, sweeps, first):
+        """Returns the buffer holding the smoothed x (x or its ping-pong partner)."""
+        Lv, w = self.levels[level], self._work[level]
+        other = w["x2"] if x is w["x"] else w["x"]
+        if self.smoother == "chebyshev":
+            theta, delta = _cheb_params(self.lambdas[level], self.lower_frac)
+            d = w["d"]
+            if first:        # x0 = 0: z0 = D^-1 r, d0 = z0 / theta, x1 = d0
+                ops.axpby(1.0 / theta, r, 0.0, d, w=Lv.inv_diag)
+                ops.axpby(1.0 / theta, r, 0.0, x, w=Lv.inv_diag)
+            else:
+                Lv.matvec(x, other, mode=L.MODE_CHEB, b=r, dinv=Lv.inv_diag, d=d,
+                          cheb_a=1.0 / theta, cheb_c=0.0)
+                x, other = other, x
+            alpha = 2.0 / theta if self.cheb_variant == "saad" else 1.0 / theta
+            for k in range(1, sweeps):
+                c = 0.25 * delta * delta * alpha
+                alpha = 1.0 / (theta - c)
+                cd = c * theta if (self.cheb_variant == "donor" and k == 1) else c
+                Lv.matvec(x, other, mode=L.MODE_CHEB, b=r, dinv=Lv.inv_diag, d=d, cheb_a=alpha, cheb_c=cd)
+                x, other = other, x
+            return x
+        om = self.omegas[level]
+        if first:
+            ops.axpby(om, r, 0.0, x, w=Lv.inv_diag)
+            sweeps -= 1
+        for _ in range(sweeps):
+            Lv.matvec(x, other, mode=L.MODE_JACOBI, b=r, dinv=Lv.inv_diag, omega=om)
+            x, other = other, x
+        return x
+
+    def coarse_solve(self, r, out=None):
+        y = out if out is not None else torch.empty_like(r)
+        return ops.dense_mv(self.coarse_inverse, r, y, mask=self._coarse_mask)
+
+    def vcycle(self, r, level=0):
+        Lv, w = self.levels[level], self._work[level]
+        if level == len(self.levels) - 1:
+            return self.coarse_solve(r, w["x"])
+        x = self._smooth(level, w["x"], r, self.pre, True)
+        Lv.matvec(x, w["res"], mode=L.MODE_RESID, b=r, fixed_zero=True)      # (r - A x) M
+        nxt = self.levels[level + 1]
+        rc = self._work[level + 1]["rhs"]
+        ops.restrict(w["res"], rc, Lv.shape, cmask=nxt.maskobj)
+        ec = self.vcycle(rc, level + 1)
+        ops.prolong(ec, x, Lv.shape, mask=Lv.maskobj)                          # x += P ec M
+        return self._smooth(level, x, r, self.post, False)
+
+    def __call__(self, r, out=None):
+        w0 = self._work[0]
+        if r.dtype != self.dtype:
+            ops.convert(r, w0["rhs"])
+            rin = w0["rhs"]
+        else:
+            rin = r
+        x = self.vcycle(rin)
+        y = out if out is not None else torch.empty(r.numel(), dtype=r.dtype, device=r.device)
+        if y.dtype != self.dtype:
+            ops.convert(x, y)
+        else:
+            y.copy_(x)
+        return y
+
+
+# ------------------------------------------------------------------------------- Krylov
+class _Reducer:
+    """Device scalars (FP64) and partial-sum buffers for one vector length."""
+
+    def __init__(self, n, dev, extra_parts=0):
+        self.n = n
+        self.parts = ops.reduce_parts(n)
+        self.partial = torch.zeros(2 * max(self.parts, extra_parts), dtype=torch.float64, device=dev)
+        self.scal = torch.zeros(16, dtype=torch.float64, device=dev)
+
+    def dot(self, x, y, dst, op=L.OP_NONE):
+        ops.reduce(L.RED_DOT, x, y, partial=self.partial)
+        ops.finish(self.partial, self.parts, 1, self.scal, dst, op)
+
+    def norm2(self, x, dst):
+        ops.reduce(L.RED_NORM2, x, partial=self.partial)
+        ops.finish(self.partial, self.parts, 1, self.scal, dst)
+
+
+def fcg(xp, apply_A, b, precondition, tol, maxiter, check_every=5, fixed_iterations=None):
+    """Flexible PCG (Polak-Ribiere beta), FP64 outer; stops on the TRUE relative residual
+    (src/voxel_mg.py:181-217).  Returns (x, iterations, converged, true_relative_residual,
+    outer_matvecs).  `apply_A(p)` / `precondition(r)` return device tensors; when apply_A is a
+    Level's bound `apply`, q = A p and p.q come from one fused pass."""
+    dev = b.device
+    n = b.numel()
+    level = getattr(apply_A, "__self__", None)
+    fused = isinstance(level, Level) and level.fused and apply_A.__func__ is Level.apply
+    red = _Reducer(n, dev, ops.matvec_tiles(level.shape)[0] if fused else 0)
+    S_NORM, S_NB = 8, 9
+    red.norm2(b, S_NB)
+    nb = float(red.scal[S_NB].sqrt())
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = precondition(r)
+    p = z.clone()
+    q = torch.empty_like(b)
+    true = torch.empty_like(b)
+    red.dot(r, z, L.S_RZ)
+    matvecs = 0
+    limit = fixed_iterations if fixed_iterations is not None else maxiter
+    it = 0
+    for it in range(1, limit + 1):
+        if fused:
+            nt = ops.matvec_tiles(level.shape)[0]
+            level.matvec(p, q, mode=L.MODE_DOT, partial=red.partial)
+            ops.finish(red.partial, nt, 1, red.scal, L.S_PQ, L.OP_ALPHA)
+        else:
+            q.copy_(apply_A(p))
+            red.dot(p, q, L.S_PQ, L.OP_ALPHA)
+        matvecs += 1
+        ops.cg_update(x, p, red.scal, r=r, q=q)            # x += a p ; r -= a q
+        if fixed_iterations is None and it % check_every == 0:
+            if fused:
+                level.matvec(x, true, mode=L.MODE_RESID, b=b)
+            else:
+                true.copy_(b - apply_A(x))
+            matvecs += 1
+            red.norm2(true, S_NORM)
+            rel = float(red.scal[S_NORM].sqrt()) / nb
+            if rel <= tol:
+                return x, it, True, rel, matvecs
+            r, true = true, r                              # residual replacement
+        z_new = precondition(r)
+        ops.reduce(L.RED_DOTDIFF, r, z_new, z, partial=red.partial)   # r.z_new, r.(z_new - z)
+        ops.finish(red.partial, red.parts, 2, red.scal, 5, L.OP_BETA_PR)
+        ops.xpby(z_new, p, red.scal)                       # p = z_new + beta p
+        z = z_new
+    if fused:
+        level.matvec(x, true, mode=L.MODE_RESID, b=b)
+    else:
+        true.copy_(b - apply_A(x))
+    matvecs += 1
+    red.norm2(true, S_NORM)
+    rel = float(red.scal[S_NORM].sqrt()) / nb
+    return x, it, bool(rel <= tol), rel, matvecs
+
+
+def spectral_omegas(xp, levels, iterations=30, seed=20260923):
+    """omega_l = 4 / (3 lambda_l) for every non-coarsest level, lambda_l from a seeded power
+    iteration on D^-1 A restricted to free DOFs (src/voxel_mg.py:228-244).  Returns (omegas, lambdas)."""
+    omegas, lambdas = [], []
+    for Lv in levels[:-1]:
+        rng = np.random.default_rng(seed)
+        v = voxelhip.to_device(rng.standard_normal(Lv.n), torch.float64).to(Lv.dtype)
+        v = ops.axpby(1.0, v, 0.0, v, w=Lv.mask)
+        red = _Reducer(Lv.n, v.device)
+        red.norm2(v, 8)
+        ops.axpby(1.0 / float(red.scal[8].sqrt()), v, 0.0, v)
+        w = torch.empty_like(v)
+        lam = 0.0
+        for _ in range(iterations):
+            Lv.matvec(v, w, fixed_zero=True)              # A v M = M K M v
+            ops.axpby(1.0, w, 0.0, w, w=Lv.inv_diag)
+            red.norm2(w, 8)
+            lam = float(red.scal[8].sqrt())
+            ops.axpby(1.0 / lam, w, 0.0, v)
+        lambdas.append(lam)
+        omegas.append(4.0 / (3.0 * lam))
+    return omegas, lambdas

@@ -1,0 +1,96 @@
+"""ctypes binding of libvoxelhip.so (the C-ABI declared in include/voxelhip.h).
+
+The library is built in-tree by matrix-free-voxel-elasticity_amd/build.py.  Loading fails
+loudly when it is missing: there is no CPU fallback for any device operation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libvoxelhip.so"
+
+F64, F32, BF16 = 0, 1, 2
+MASK_NONE, MASK_ARRAY, MASK_X0 = 0, 1, 2
+MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT = 0, 1, 2, 3, 4
+RED_DOT, RED_DOT2, RED_DOTDIFF, RED_NORM2 = 0, 1, 2, 3
+OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ = 0, 1, 2, 3, 4
+S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Epilogue(ctypes.Structure):
+    _fields_ = [("mode", _i), ("fixed_zero", _i), ("b", _vp), ("dinv", _vp), ("d", _vp),
+                ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i)]
+
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "vh_abi_version": [],
+    "vh_last_error": [],
+    "vh_modal_blocks": [_vp, _d, _vp, _dp],
+    "vh_matvec": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp],
+    "vh_matvec_fused": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Epilogue), _vp],
+    "vh_matvec_tiles": [_i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "vh_dense_product": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp],
+    "vh_diagonal": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp],
+    "vh_element_energy": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _d, _d, _vp],
+    "vh_restrict": [_i, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
+    "vh_prolong": [_i, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp],
+    "vh_coarsen_modulus": [_i, _vp, _vp, _i, _i, _i, _vp],
+    "vh_dense_mv": [_i, _i, _vp, _vp, _vp, _vp, _vp],
+    "vh_reduce_parts": [_sz],
+    "vh_reduce": [_i, _i, _sz, _vp, _vp, _vp, _vp, _vp],
+    "vh_finish": [_vp, _i, _i, _vp, _i, _i, _vp],
+    "vh_cg_update": [_i, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
+    "vh_jacobi_dot": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
+    "vh_xpby": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
+    "vh_axpby": [_i, _sz, _d, _vp, _vp, _d, _vp, _vp],
+    "vh_convert": [_i, _i, _sz, _vp, _vp, _d, _vp, _i, _i, _vp],
+    "vh_cone_filter": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp],
+    "vh_oc_probe": [_i, _sz, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _i, _vp],
+}
+_RESTYPES = {"vh_last_error": ctypes.c_char_p}
+
+_LIB = None
+
+
+class VoxelHipError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libvoxelhip.so once; raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    p = Path(path or os.environ.get("VOXELHIP_LIB", LIB_PATH))
+    if not p.exists():
+        raise VoxelHipError(f"{p} is not built: run python matrix-free-voxel-elasticity_amd/build.py")
+    lib = ctypes.CDLL(str(p))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, _i)
+    _LIB = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = _LIB.vh_last_error().decode() if _LIB is not None else ""
+        if status == -3:
+            raise ValueError(msg)
+        raise VoxelHipError(f"{what}: status {status}: {msg}")
+    return status
+
+
+def call(name, *args):
+    lib = load()
+    return check(getattr(lib, name)(*args), name)

@@ -1,0 +1,260 @@
+"""Tensor-level wrappers over the C-ABI: argument checks, stream plumbing, dtype dispatch.
+
+Every function takes torch tensors on the current HIP device and launches asynchronously on
+torch's current stream.  There is no CPU path: a tensor that is not on the GPU is an error.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+_DT = {torch.float64: L.F64, torch.float32: L.F32}
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(t, name):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor on the GPU, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: tensor is on {t.device}; voxelhip runs on the GPU only")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    return t
+
+
+def dtype_code(t):
+    try:
+        return _DT[t.dtype]
+    except KeyError:
+        raise TypeError(f"unsupported dtype {t.dtype}") from None
+
+
+def n_nodes(shape):
+    nx, ny, nz = (int(s) for s in shape)
+    return (nx + 1) * (ny + 1) * (nz + 1)
+
+
+def n_elements(shape):
+    nx, ny, nz = (int(s) for s in shape)
+    return nx * ny * nz
+
+
+# --------------------------------------------------------------------------- host setup
+def modal_blocks(ke, tolerance=2e-14):
+    """(blocks[8,3,3], discarded_relative) of C = T ke T^T / 64; ValueError outside the model."""
+    ke = np.ascontiguousarray(np.asarray(ke, dtype=np.float64).reshape(24, 24))
+    blocks = np.zeros(72)
+    disc = ctypes.c_double(0.0)
+    lib = L.load()
+    st = lib.vh_modal_blocks(ke.ctypes.data_as(ctypes.c_void_p), float(tolerance),
+                             blocks.ctypes.data_as(ctypes.c_void_p), ctypes.byref(disc))
+    if st == -3:
+        raise ValueError(lib.vh_last_error().decode())
+    L.check(st, "vh_modal_blocks")
+    return blocks.reshape(8, 3, 3), float(disc.value)
+
+
+class Mask:
+    """Per-DOF constraint mask on the device: kind NONE, X0 (every DOF of node plane x = 0 fixed;
+    no array) or ARRAY (uint8, 1 = free)."""
+
+    def __init__(self, kind, array=None):
+        self.kind, self.array = kind, array
+
+    @staticmethod
+    def none():
+        return Mask(L.MASK_NONE)
+
+    @staticmethod
+    def from_values(mask, shape, device=None):
+        """Classify a 0/1 mask (host or device); the x = 0 clamp needs no array."""
+        m = mask.detach().cpu().numpy() if isinstance(mask, torch.Tensor) else np.asarray(mask)
+        m = (m.reshape(-1) != 0)
+        nx, ny, nz = (int(s) for s in shape)
+        if m.size != 3 * n_nodes(shape):
+            raise ValueError(f"mask has {m.size} entries, grid {shape} has {3 * n_nodes(shape)} DOFs")
+        g = m.reshape(nx + 1, (ny + 1) * (nz + 1) * 3)
+        if m.all():
+            return Mask(L.MASK_NONE)
+        if (not g[0].any()) and g[1:].all():
+            return Mask(L.MASK_X0)
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        return Mask(L.MASK_ARRAY, torch.from_numpy(m.astype(np.uint8)).to(dev))
+
+    def values(self, shape, dtype=torch.float64, device=None):
+        """The mask as a 0/1 float tensor (for callers that want the reference's float masks)."""
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        if self.kind == L.MASK_ARRAY:
+            return self.array.to(dtype)
+        v = torch.ones(3 * n_nodes(shape), dtype=dtype, device=dev)
+        if self.kind == L.MASK_X0:
+            nx, ny, nz = (int(s) for s in shape)
+            v[: 3 * (ny + 1) * (nz + 1)] = 0
+        return v
+
+    def coarsen(self, shape):
+        """Injection onto the 2x coarser grid (src/voxel_mg.py:114-116)."""
+        if self.kind != L.MASK_ARRAY:
+            return Mask(self.kind)
+        nx, ny, nz = (int(s) for s in shape)
+        a = self.array.view(nx + 1, ny + 1, nz + 1, 3)[::2, ::2, ::2, :].contiguous().view(-1)
+        return Mask(L.MASK_ARRAY, a)
+
+    @property
+    def ptr(self):
+        return _p(self.array) if self.kind == L.MASK_ARRAY else None
+
+
+# --------------------------------------------------------------------------- operator
+def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None, d=None,
+           partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None):
+    """One pass of the parity product with a fused epilogue (see include/voxelhip.h)."""
+    _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
+    nx, ny, nz = (int(s) for s in shape)
+    dt = dtype_code(y)
+    st = dt if storage is None else storage
+    if E.dtype != y.dtype:
+        raise TypeError("E and y must share the compute dtype")
+    if st == L.BF16:
+        if u.dtype not in (torch.bfloat16, torch.int16, torch.uint16):
+            raise TypeError("bf16 storage expects a bfloat16 u")
+    elif u.dtype != y.dtype:
+        raise TypeError("u and y must share dtype")
+    nd = 3 * n_nodes(shape)
+    if u.numel() != nd or y.numel() != nd or E.numel() != nx * ny * nz:
+        raise ValueError(f"sizes do not match grid {shape}")
+    m = mask or Mask.none()
+    hb = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
+    ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), _p(d), _p(partial),
+                    float(omega), float(cheb_a), float(cheb_c), int(lx))
+    L.call("vh_matvec_fused", dt, st, hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
+           nx, ny, nz, m.kind, m.ptr, ctypes.byref(ep), _stream())
+    return y
+
+
+def matvec_tiles(shape, lx=0):
+    nt, nw = ctypes.c_int(0), ctypes.c_int(0)
+    L.call("vh_matvec_tiles", *[int(s) for s in shape], int(lx), ctypes.byref(nt), ctypes.byref(nw))
+    return nt.value, nw.value
+
+
+def dense_product(ke, E, u, y, shape):
+    _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
+    k = np.ascontiguousarray(np.asarray(ke, dtype=np.float64).reshape(576))
+    L.call("vh_dense_product", dtype_code(y), k.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
+           *[int(s) for s in shape], _stream())
+    return y
+
+
+def diagonal(ke, E, out, shape, mask=None, invert=False):
+    _dev(E, "E"); _dev(out, "out")
+    k = np.ascontiguousarray(np.asarray(ke, dtype=np.float64).reshape(576))
+    m = mask or Mask.none()
+    L.call("vh_diagonal", dtype_code(out), k.ctypes.data_as(ctypes.c_void_p), _p(E), _p(out),
+           *[int(s) for s in shape], m.kind, m.ptr, int(bool(invert)), _stream())
+    return out
+
+
+def element_energy(blocks, u, ce, shape, mask=None, rho=None, dc=None, penal=3.0, dE=1.0):
+    _dev(u, "u"); _dev(ce, "ce")
+    hb = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
+    m = mask or Mask.none()
+    L.call("vh_element_energy", dtype_code(ce), hb.ctypes.data_as(ctypes.c_void_p), _p(u), _p(ce),
+           *[int(s) for s in shape], m.kind, m.ptr, _p(rho), _p(dc), float(penal), float(dE), _stream())
+    return ce
+
+
+def restrict(rf, rc, fine_shape, fmask=None, cmask=None):
+    _dev(rf, "rf"); _dev(rc, "rc")
+    fm, cm = fmask or Mask.none(), cmask or Mask.none()
+    L.call("vh_restrict", dtype_code(rc), _p(rf), _p(rc), *[int(s) for s in fine_shape],
+           fm.kind, fm.ptr, cm.kind, cm.ptr, _stream())
+    return rc
+
+
+def prolong(ec, xf, fine_shape, mask=None, assign=False):
+    _dev(ec, "ec"); _dev(xf, "xf")
+    m = mask or Mask.none()
+    L.call("vh_prolong", dtype_code(xf), _p(ec), _p(xf), *[int(s) for s in fine_shape], m.kind, m.ptr,
+           int(bool(assign)), _stream())
+    return xf
+
+
+def coarsen_modulus(Ef, Ec, fine_shape):
+    _dev(Ef, "Ef"); _dev(Ec, "Ec")
+    L.call("vh_coarsen_modulus", dtype_code(Ec), _p(Ef), _p(Ec), *[int(s) for s in fine_shape], _stream())
+    return Ec
+
+
+def dense_mv(A, r, y, mask=None):
+    _dev(A, "A"); _dev(r, "r"); _dev(y, "y")
+    m = mask.array if (mask is not None and mask.kind == L.MASK_ARRAY) else None
+    if mask is not None and mask.kind == L.MASK_X0:
+        raise ValueError("dense_mv takes an explicit mask array")
+    L.call("vh_dense_mv", dtype_code(y), int(r.numel()), _p(A), _p(r), _p(y), _p(m), _stream())
+    return y
+
+
+# --------------------------------------------------------------------------- BLAS-1
+def reduce_parts(n):
+    return int(L.load().vh_reduce_parts(int(n)))
+
+
+def reduce(kind, x, y=None, w=None, partial=None):
+    _dev(x, "x")
+    L.call("vh_reduce", dtype_code(x), kind, int(x.numel()), _p(x), _p(y), _p(w), _p(partial), _stream())
+    return partial
+
+
+def finish(partial, nparts, count, scal, dst, op=L.OP_NONE):
+    L.call("vh_finish", _p(partial), int(nparts), int(count), _p(scal), int(dst), int(op), _stream())
+
+
+def cg_update(x, p, scal, r=None, q=None, dinv=None, z=None, partial=None):
+    L.call("vh_cg_update", dtype_code(x), int(x.numel()), _p(x), _p(p), _p(r), _p(q), _p(dinv), _p(z),
+           _p(scal), int(r is not None), _p(partial), _stream())
+
+
+def jacobi_dot(r, dinv, z, partial):
+    L.call("vh_jacobi_dot", dtype_code(r), int(r.numel()), _p(r), _p(dinv), _p(z), _p(partial), _stream())
+
+
+def xpby(z, p, scal, zcopy=None):
+    L.call("vh_xpby", dtype_code(p), int(p.numel()), _p(z), _p(p), _p(scal), _p(zcopy), _stream())
+
+
+def axpby(a, x, b, y, w=None):
+    _dev(x, "x"); _dev(y, "y")
+    L.call("vh_axpby", dtype_code(y), int(y.numel()), float(a), _p(x), _p(w), float(b), _p(y), _stream())
+    return y
+
+
+def convert(x, y, scale=1.0, dscale=None, inv_scale=False, accumulate=False):
+    _dev(x, "x"); _dev(y, "y")
+    L.call("vh_convert", dtype_code(x), dtype_code(y), int(x.numel()), _p(x), _p(y), float(scale),
+           _p(dscale), int(bool(inv_scale)), int(bool(accumulate)), _stream())
+    return y
+
+
+# --------------------------------------------------------------------------- SIMP
+def cone_filter(x, y, shape, rmin, pre_div=None, post_div=None):
+    _dev(x, "x"); _dev(y, "y")
+    L.call("vh_cone_filter", dtype_code(y), _p(x), _p(pre_div), _p(post_div), _p(y),
+           *[int(s) for s in shape], float(rmin), _stream())
+    return y
+
+
+def oc_probe(x, dc, dv, w, lmid, move, xnew, partial, nparts):
+    L.call("vh_oc_probe", dtype_code(x), int(x.numel()), _p(x), _p(dc), _p(dv), _p(w), float(lmid),
+           float(move), _p(xnew), _p(partial), int(nparts), _stream())

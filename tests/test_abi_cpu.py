@@ -1,0 +1,91 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol include/voxelhip.h declares,
+and its host-only entry point (parity-block factorisation) agrees with the oracle."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "voxelhip.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(vh_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from voxelhip import _lib
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES)
+    assert lib.vh_abi_version() == 1
+
+
+def test_modal_blocks_match_the_oracle():
+    from oracle import hexref as hx
+    from voxelhip import ops
+    for lengths in ([1, 1, 1], [1, 2, 0.5]):
+        ke = hx.quadrature(lengths, 0.3)
+        b, disc = ops.modal_blocks(ke)
+        ref = hx.Modal(ke)
+        assert np.abs(b - ref.blocks).max() < 1e-15
+        assert disc == pytest.approx(ref.discarded_relative, abs=1e-16)
+
+
+def test_modal_blocks_reject_a_non_brick():
+    from voxelhip import ops
+    a = np.random.default_rng(1).standard_normal((24, 24))
+    with pytest.raises(ValueError, match="parity-block"):
+        ops.modal_blocks(a @ a.T)
+
+
+def test_product_quadrature_matches_the_oracle(golden):
+    from oracle import hexref as hx
+    import voxel_kernels as vk
+    for lengths in ([1, 1, 1], [1, 2, 0.5], [0.25, 0.25, 0.25]):
+        assert np.abs(vk.quadrature(lengths, 0.3) - hx.quadrature(lengths, 0.3)).max() < 1e-15
+    assert np.abs(2 * vk.quadrature([1, 1, 1], 0.3) - np.load(golden / "ke_unit_3d.npy")).max() < 1e-15
+
+
+def test_make_source_returns_the_hip_kernel_source():
+    import voxel_kernels as vk
+    src = vk.make_source("double")
+    assert src is vk.PROTOTYPE_SOURCE and "parity_xmarch" in src
+    assert vk.make_source("float") is src
+    with pytest.raises(ValueError):
+        vk.make_source("half")
+
+
+def test_device_entry_points_fail_loudly_without_a_gpu():
+    import torch
+    import voxel_kernels as vk
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError):
+        vk.emulate("modal8", (2, 2, 2), vk.quadrature([1, 1, 1], 0.3), np.ones(8), np.zeros(81))
+
+
+def test_host_index_helpers_match_the_oracle():
+    from oracle import hexref as hx
+    import voxel_kernels as vk
+    import voxel_mg as mg
+    for shape in ((3, 2, 4), (1, 1, 1), (5, 3, 2)):
+        assert np.array_equal(vk.element_nodes(shape), hx.element_nodes(shape))
+        assert np.array_equal(mg.edof_author(shape), hx.edof_author(shape))
+        assert mg.hierarchy_shapes((128, 64, 64)) == [(128, 64, 64), (64, 32, 32), (32, 16, 16), (16, 8, 8)]
+
+
+def test_assemble_dense_matches_the_oracle():
+    from oracle import hexref as hx
+    import voxel_mg as mg
+    rng = np.random.default_rng(0)
+    shape = (3, 2, 2)
+    ke = hx.quadrature([1, 1, 1], 0.3)
+    E = rng.random(12)
+    mask = (rng.random(3 * hx.n_nodes(shape)) > 0.2).astype(float)
+    assert np.abs(mg.assemble_dense(shape, ke, E, mask) - hx.assemble_dense(shape, ke, E, mask)).max() < 1e-15

@@ -1,0 +1,221 @@
+"""GPU parity of the operator kernels against the oracle, through the C-ABI.
+
+Tolerances: FP64 relative L2 <= 1e-12 (the bar of the north-star contract; the reference's own
+gate is 1e-11, experiments/session_a/protocol.json:127-133); FP32 <= 1e-5 (same gate).
+Index/mask work (fixed DOFs) is checked exactly.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import hexref as hx
+from oracle import mgref as mg_ref
+
+pytestmark = pytest.mark.gpu
+
+KE = hx.quadrature([1, 1, 1], 0.3)
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return float(np.linalg.norm(np.asarray(a, dtype=np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def case(shape, seed=7):
+    rng = np.random.default_rng(seed)
+    E = 1e-6 + (1 - 1e-6) * rng.random(int(np.prod(shape))) ** 3
+    u = rng.standard_normal(3 * hx.n_nodes(shape))
+    return E, u
+
+
+SHAPES = [(3, 2, 4), (1, 1, 1), (2, 3, 5), (8, 4, 4), (17, 33, 40), (40, 20, 70), (5, 16, 32), (6, 17, 33),
+          (9, 48, 96)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_parity_product_fp64_matches_the_oracle(shape):
+    import voxel_kernels as vk
+    print("case", shape, flush=True)
+    E, u = case(shape)
+    ref = hx.NumpyProduct(shape, KE).raw(u, E)
+    y = vk.emulate("modal8", shape, KE, E, u, np.float64)
+    assert rel(y, ref) < 1e-12
+    if int(np.prod(shape)) < 2000:
+        assert rel(y, hx.assembled_reference(shape, KE, E, u)) < 1e-12
+
+
+@pytest.mark.parametrize("lx", [1, 2, 3, 7, 100])
+def test_parity_product_is_independent_of_the_x_chunking(lx):
+    from voxelhip import ops
+    shape = (13, 18, 35)
+    E, u = case(shape, 3)
+    ref = hx.NumpyProduct(shape, KE).raw(u, E)
+    y = torch.empty(u.size, dtype=torch.float64, device="cuda")
+    ops.matvec(hx.Modal(KE).blocks, dev(E), dev(u), y, shape, lx=lx)
+    assert rel(y, ref) < 1e-12
+
+
+def test_parity_product_is_bitwise_deterministic():
+    from voxelhip import ops
+    shape = (24, 20, 36)
+    E, u = case(shape, 5)
+    Ed, ud = dev(E), dev(u)
+    b = hx.Modal(KE).blocks
+    y1, y2 = torch.empty_like(ud), torch.empty_like(ud)
+    ops.matvec(b, Ed, ud, y1, shape)
+    ops.matvec(b, Ed, ud, y2, shape)
+    assert torch.equal(y1, y2)
+
+
+@pytest.mark.parametrize("shape", [(3, 2, 4), (17, 33, 40)])
+def test_parity_product_fp32(shape):
+    import voxel_kernels as vk
+    E, u = case(shape)
+    ref = hx.NumpyProduct(shape, KE).raw(u, E)
+    y = vk.emulate("modal8", shape, KE, E, u, np.float32)
+    assert y.dtype == np.float32 and rel(y, ref) < 1e-5
+
+
+def test_bf16_storage_product():
+    from voxelhip import _lib as L, ops
+    shape = (10, 12, 34)
+    E, u = case(shape)
+    ub = dev(u, torch.float32).to(torch.bfloat16)
+    ref = hx.NumpyProduct(shape, KE).raw(ub.float().cpu().numpy().astype(np.float64), E)
+    y = torch.empty(u.size, dtype=torch.float32, device="cuda")
+    ops.matvec(hx.Modal(KE).blocks, dev(E, torch.float32), ub, y, shape, storage=L.BF16)
+    assert rel(y, ref) < 1e-5          # exact bf16 inputs, FP32 arithmetic
+
+
+@pytest.mark.parametrize("shape", [(3, 2, 4), (17, 33, 40)])
+def test_dense_control_product(shape):
+    import voxel_kernels as vk
+    E, u = case(shape)
+    ref = hx.NumpyProduct(shape, KE).raw(u, E)
+    assert rel(vk.emulate("dense8", shape, KE, E, u, np.float64), ref) < 1e-12
+    assert rel(vk.assembled_reference(shape, KE, E, u), ref) < 1e-12
+
+
+def _masks(shape, rng):
+    nn = hx.n_nodes(shape)
+    x0 = np.ones((shape[0] + 1, nn // (shape[0] + 1), 3)); x0[0] = 0
+    return {"x0": x0.ravel(), "array": (rng.random(3 * nn) > 0.15).astype(float)}
+
+
+@pytest.mark.parametrize("kind", ["x0", "array"])
+@pytest.mark.parametrize("shape", [(6, 5, 7), (18, 17, 34)])
+def test_constrained_operator_and_epilogues(kind, shape):
+    from voxelhip import _lib as L, ops
+    rng = np.random.default_rng(11)
+    E, u = case(shape, 4)
+    m = _masks(shape, rng)[kind]
+    prod = hx.NumpyProduct(shape, KE)
+    lvl = mg_ref.Level(shape, prod, E, m, np.float64)
+    Au = lvl.apply(u)
+    b = rng.standard_normal(u.size)
+    dinv = lvl.inv_diag
+    d_old = rng.standard_normal(u.size)
+    mask = ops.Mask.from_values(m, shape)
+    assert mask.kind == (L.MASK_X0 if kind == "x0" else L.MASK_ARRAY)
+    blocks = hx.Modal(KE).blocks
+    Ed, ud, bd, dd = dev(E), dev(u), dev(b), dev(dinv)
+    y = torch.empty_like(ud)
+    run = lambda **kw: ops.matvec(blocks, Ed, ud, y, shape, mask=mask, **kw)  # noqa: E731
+    run()
+    assert rel(y, Au) < 1e-12
+    fixed = m == 0
+    assert np.array_equal(y.cpu().numpy()[fixed], u[fixed])
+    run(fixed_zero=True)
+    assert rel(y, Au * m) < 1e-12 and np.all(y.cpu().numpy()[fixed] == 0)
+    run(mode=L.MODE_RESID, b=bd, fixed_zero=True)
+    assert rel(y, (b - Au) * m) < 1e-12
+    run(mode=L.MODE_RESID, b=bd)
+    assert rel(y, b - Au) < 1e-12
+    run(mode=L.MODE_JACOBI, b=bd, dinv=dd, omega=0.43)
+    assert rel(y, u + 0.43 * dinv * (b - Au)) < 1e-12
+    d = dev(d_old)
+    run(mode=L.MODE_CHEB, b=bd, dinv=dd, d=d, cheb_a=0.7, cheb_c=0.2)
+    dn = 0.7 * (dinv * (b - Au) + 0.2 * d_old)
+    assert rel(d, dn) < 1e-12 and rel(y, u + dn) < 1e-12
+    nt, _ = ops.matvec_tiles(shape)
+    part = torch.zeros(nt, dtype=torch.float64, device="cuda")
+    run(mode=L.MODE_DOT, partial=part)
+    assert float(part.sum()) == pytest.approx(float(u @ Au), rel=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["none", "x0", "array"])
+def test_diagonal_and_inverse_diagonal(kind):
+    from voxelhip import ops
+    shape = (7, 6, 9)
+    rng = np.random.default_rng(2)
+    E, _ = case(shape)
+    prod = hx.NumpyProduct(shape, KE)
+    d_ref = prod.diagonal(E)
+    out = torch.empty(d_ref.size, dtype=torch.float64, device="cuda")
+    ops.diagonal(KE, dev(E), out, shape)
+    assert rel(out, d_ref) < 1e-14
+    if kind != "none":
+        m = _masks(shape, rng)[kind]
+        inv_ref = mg_ref.Level(shape, prod, E, m, np.float64).inv_diag
+        ops.diagonal(KE, dev(E), out, shape, mask=ops.Mask.from_values(m, shape), invert=True)
+        assert rel(out, inv_ref) < 1e-14
+
+
+def test_element_energy_and_sensitivity():
+    from oracle import simpref as sr
+    from voxelhip import ops
+    shape = (9, 6, 10)
+    rng = np.random.default_rng(3)
+    rho = rng.uniform(0.01, 1, int(np.prod(shape)))
+    u = rng.standard_normal(3 * hx.n_nodes(shape))
+    ke = 2 * KE
+    ce_ref = hx.element_energy(shape, ke, u)
+    ce = torch.empty(rho.size, dtype=torch.float64, device="cuda")
+    dc = torch.empty_like(ce)
+    ops.element_energy(hx.Modal(ke).blocks, dev(u), ce, shape, rho=dev(rho), dc=dc, penal=3.0, dE=1 - 1e-9)
+    assert rel(ce, ce_ref) < 1e-12
+    assert rel(dc, sr.sensitivity(rho, ce_ref)) < 1e-12
+
+
+@pytest.mark.parametrize("fine", [(4, 2, 2), (16, 8, 8), (34, 18, 66)])
+def test_transfers_match_the_oracle(fine):
+    import voxel_mg as mg
+    rng = np.random.default_rng(4)
+    coarse = tuple(s // 2 for s in fine)
+    uc = rng.standard_normal(3 * hx.n_nodes(coarse))
+    rf = rng.standard_normal(3 * hx.n_nodes(fine))
+    assert rel(mg.prolong(None, dev(uc), coarse), mg_ref.prolong(uc, coarse)) < 1e-15
+    assert rel(mg.restrict(None, dev(rf), fine), mg_ref.restrict(rf, fine)) < 1e-15
+    E = rng.random(int(np.prod(fine)))
+    assert rel(mg.coarsen_modulus(None, dev(E), fine), mg_ref.coarsen_modulus(E, fine)) < 1e-15
+
+
+def test_full_size_product_properties():
+    """256 x 128 x 128 (12.8 M DOFs): the parity kernel against the independent dense-ke kernel,
+    linearity, and rigid translations / rotations in the null space."""
+    from voxelhip import ops
+    shape = (256, 128, 128)
+    print("case", shape, flush=True)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    ne, nd = int(np.prod(shape)), 3 * hx.n_nodes(shape)
+    E = 1e-9 + torch.rand(ne, generator=g, device="cuda", dtype=torch.float64) ** 3
+    u = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64)
+    v = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64)
+    blocks = hx.Modal(KE).blocks
+    y, yd, yv, yl = (torch.empty_like(u) for _ in range(4))
+    ops.matvec(blocks, E, u, y, shape)
+    ops.dense_product(KE, E, u, yd, shape)
+    assert float((y - yd).norm() / yd.norm()) < 1e-12
+    ops.matvec(blocks, E, v, yv, shape)
+    ops.matvec(blocks, E, (2.5 * u - v).contiguous(), yl, shape)
+    assert float((yl - (2.5 * y - yv)).norm() / yl.norm()) < 1e-12
+    nx, ny, nz = shape
+    X = torch.stack(torch.meshgrid(*[torch.arange(n + 1, device="cuda", dtype=torch.float64) for n in shape],
+                                   indexing="ij"), -1).reshape(-1, 3)
+    rot = torch.stack([-X[:, 1], X[:, 0], torch.zeros_like(X[:, 0])], 1).reshape(-1).contiguous()
+    ops.matvec(blocks, E, rot, yl, shape)
+    assert float(yl.abs().max()) < 1e-12 * float(rot.abs().max())
