@@ -25,7 +25,8 @@ extern "C" {
 
 enum { VH_F64 = 0, VH_F32 = 1, VH_BF16 = 2 };
 enum { VH_MASK_NONE = 0, VH_MASK_ARRAY = 1, VH_MASK_X0 = 2 };
-enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3, VH_MODE_DOT = 4 };
+enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3, VH_MODE_DOT = 4,
+       VH_MODE_CHEB3 = 5 };
 enum { VH_RED_DOT = 0, VH_RED_DOT2 = 1, VH_RED_DOTDIFF = 2, VH_RED_NORM2 = 3 };
 enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4 };
 /* device scalar slots used by vh_finish / vh_cg_update / vh_xpby */
@@ -54,7 +55,12 @@ int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, con
 /* Fused epilogues over the same single pass (A as in vh_matvec; with a mask, fixed_zero makes
  * the output 0 at fixed DOFs, i.e. M K M u for RAW/DOT and (b - A u) M for RESID):  RESID y = b - A u;  JACOBI y = u + omega dinv (b - A u);
  * CHEB d = cheb_a (dinv (b - A u) + cheb_c d), y = u + d (d not read when cheb_c == 0);
- * DOT y = A u and partial[t] = sum of u . A u over tile t (FP64, length vh_matvec_tiles()).
+ * DOT y = A u and partial[t] = sum of u . A u over tile t (FP64, length vh_matvec_tiles());
+ * CHEB3 y = u + cheb_a (dinv (b - A u) + cheb_c (u - y)), y holding the previous iterate on
+ * entry (not read when cheb_c == 0): the three-term form of CHEB without a direction array.
+ * JACOBI/CHEB/CHEB3 with dinv == NULL compute D^-1 = 1 / (kdiag[c] * sum of the incident E)
+ * at free DOFs and 1 at fixed ones (kdiag: the diagonal of ke per component, which an
+ * axis-aligned brick has equal at its 8 corners).
  * Replaces: the smoothing sweep and residual of Multigrid.vcycle src/voxel_mg.py:166-174,
  * the Chebyshev step of donor_yang_gmg multigrid_v4.py:860-889, the matvec + p.q of fcg
  * src/voxel_mg.py:197-199 and of donor pcg simp_r2.py:258-262. */
@@ -66,13 +72,19 @@ typedef struct vh_epilogue {
   void* d;
   double* partial;
   double omega, cheb_a, cheb_c;
-  int lx; /* x planes per workgroup chunk, 0 = automatic */
+  int lx;   /* x planes per workgroup chunk, 0 = automatic */
+  int tile; /* (y, z) tile variant: 0 = automatic, 1 = 16 x 32 nodes, 2 = 8 x 32 nodes */
+  double kdiag[3];
+  int flags; /* VH_PREV_ZERO: CHEB3's previous iterate is zero (y is then not read) */
 } vh_epilogue;
+#define VH_PREV_ZERO 1
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
                     int nx, int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep,
                     void* stream);
-/* number of tiles (DOT partials) and workgroups of the product for a grid */
-int vh_matvec_tiles(int nx, int ny, int nz, int lx, int* n_tiles, int* n_workgroups);
+/* Number of tiles (= DOT partials) and workgroups the product launches for these arguments
+ * (blocks72 host pointer or NULL for the voxel-cube pattern; lx/tile as in vh_epilogue). */
+int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int ny, int nz, int lx, int tile,
+                    int* n_tiles, int* n_workgroups);
 
 /* Node-owned product with the dense 24x24 ke (host pointer): the dense8 / node control.
  * Replaces: dense8 code/hex_modal_cuda/cuda.py:47-79 and node_matvec_fp64

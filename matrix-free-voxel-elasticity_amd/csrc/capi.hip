@@ -79,6 +79,32 @@ static int modal_blocks(const double* ke, double tol, double* blocks, double* di
   return 0;
 }
 
+// The parity blocks of a voxel cube (any edge length, isotropic) follow one pattern with 8
+// distinct values: B0 = (p-q) I + q 11^T;  B1 = [0 0 0; 0 a b; 0 b a], B2 = [a 0 b; 0 0 0;
+// b 0 a], B4 = [a b 0; b a 0; 0 0 0];  B3 = [c c 0; c c 0; 0 0 d], B5 = [c 0 c; 0 d 0; c 0 c],
+// B6 = [d 0 0; 0 c c; 0 c c];  B7 = (e-f) I + f 11^T.  Returns true and the values
+// [p-q, q, a, b, c, d, e-f, f] when `blocks` matches it to 1e-12 of its largest entry.
+bool cube_pattern(const double* B, double* k8) {
+  double mx = 0.0;
+  for (int i = 0; i < 72; ++i) mx = std::fmax(mx, std::fabs(B[i]));
+  if (!(mx > 0.0)) return false;
+  const double p = B[0], q = B[1], a = B[9 + 4], b = B[9 + 5], c = B[27 + 0], d = B[27 + 8];
+  const double e = B[63 + 0], f = B[63 + 1];
+  const double want[72] = {
+      p, q, q, q, p, q, q, q, p,      // B0
+      0, 0, 0, 0, a, b, 0, b, a,      // B1
+      a, 0, b, 0, 0, 0, b, 0, a,      // B2
+      c, c, 0, c, c, 0, 0, 0, d,      // B3
+      a, b, 0, b, a, 0, 0, 0, 0,      // B4
+      c, 0, c, 0, d, 0, c, 0, c,      // B5
+      d, 0, 0, 0, c, c, 0, c, c,      // B6
+      e, f, f, f, e, f, f, f, e};     // B7
+  for (int i = 0; i < 72; ++i)
+    if (std::fabs(B[i] - want[i]) > 1e-12 * mx) return false;
+  k8[0] = p - q; k8[1] = q; k8[2] = a; k8[3] = b; k8[4] = c; k8[5] = d; k8[6] = e - f; k8[7] = f;
+  return true;
+}
+
 static bool dims_ok(int nx, int ny, int nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
 static bool dtype_ok(int d) { return d == F64 || d == F32; }
 
@@ -100,7 +126,13 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
   p.y = static_cast<T*>(y);
   p.mask = mask;
   p.nx = nx; p.ny = ny; p.nz = nz;
-  for (int i = 0; i < 72; ++i) p.coef[i] = T(blocks72[i]);
+  double k8[8];
+  if (cube_pattern(blocks72, k8)) {
+    p.cube = 1;
+    for (int i = 0; i < 8; ++i) p.coef[i] = T(k8[i]);
+  } else {
+    for (int i = 0; i < 72; ++i) p.coef[i] = T(blocks72[i]);
+  }
   int mode = MODE_RAW;
   if (ep) {
     mode = ep->mode;
@@ -112,6 +144,9 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
     p.cheb_a = T(ep->cheb_a);
     p.cheb_c = T(ep->cheb_c);
     p.lx = ep->lx;
+    p.tile = ep->tile;
+    for (int c = 0; c < 3; ++c) p.kd[c] = T(ep->kdiag[c]);
+    p.flags = ep->flags;
     p.fixed_zero = ep->fixed_zero;
   }
   return matvec_launch<T>(p, storage, mode, mask_kind, st);
@@ -141,12 +176,15 @@ int vh_modal_blocks(const double* ke576, double tolerance, double* blocks72, dou
   return modal_blocks(ke576, tolerance, blocks72, discarded);
 }
 
-int vh_matvec_tiles(int nx, int ny, int nz, int lx, int* n_tiles, int* n_workgroups) {
+int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int ny, int nz, int lx, int tile,
+                    int* n_tiles, int* n_workgroups) {
   VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec_tiles: bad grid %d x %d x %d", nx, ny, nz);
-  int ty, tz, cx, l, nt, grid;
-  matvec_geometry(nx, ny, nz, lx, &ty, &tz, &cx, &l, &nt, &grid);
-  if (n_tiles) *n_tiles = nt;
-  if (n_workgroups) *n_workgroups = grid;
+  VH_REQUIRE(dtype_ok(dtype), "vh_matvec_tiles: dtype %d", dtype);
+  double k8[8];
+  const bool cube = blocks72 == nullptr || cube_pattern(blocks72, k8);
+  const MatvecPlan pl = matvec_plan(dtype, storage, cube, nx, ny, nz, lx, tile);
+  if (n_tiles) *n_tiles = pl.n_tiles;
+  if (n_workgroups) *n_workgroups = pl.grid;
   return 0;
 }
 
@@ -161,13 +199,15 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
   VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2, "vh_matvec: bad mask kind %d", mask_kind);
   VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_matvec: mask array missing");
   if (ep) {
-    VH_REQUIRE(ep->mode >= 0 && ep->mode <= MODE_DOT, "vh_matvec: bad mode %d", ep->mode);
+    VH_REQUIRE(ep->mode >= 0 && ep->mode <= MODE_CHEB3, "vh_matvec: bad mode %d", ep->mode);
     const int m = ep->mode;
-    VH_REQUIRE(!(m == MODE_RESID || m == MODE_JACOBI || m == MODE_CHEB) || ep->b, "vh_matvec: b missing");
-    VH_REQUIRE(!(m == MODE_JACOBI || m == MODE_CHEB) || ep->dinv, "vh_matvec: dinv missing");
+    const bool smoother = m == MODE_JACOBI || m == MODE_CHEB || m == MODE_CHEB3;
+    VH_REQUIRE(!(m == MODE_RESID || smoother) || ep->b, "vh_matvec: b missing");
     VH_REQUIRE(m != MODE_CHEB || ep->d, "vh_matvec: d missing");
     VH_REQUIRE(m != MODE_DOT || ep->partial, "vh_matvec: partial missing");
-    VH_REQUIRE(ep->b != y && ep->dinv != y, "vh_matvec: y must not alias an input");
+    VH_REQUIRE(ep->b != y && (ep->dinv == nullptr || ep->dinv != y), "vh_matvec: y must not alias an input");
+    if (smoother && ep->dinv == nullptr)   // D^-1 from the moduli and ke's diagonal
+      VH_REQUIRE(ep->kdiag[0] > 0 && ep->kdiag[1] > 0 && ep->kdiag[2] > 0, "vh_matvec: kdiag missing");
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == F64) return matvec_t<double>(storage, blocks72, E, u, y, nx, ny, nz, mask_kind, mask, ep, st);

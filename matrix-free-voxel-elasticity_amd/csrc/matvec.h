@@ -5,16 +5,19 @@
 namespace vh {
 
 // Owned node tile per workgroup in (y, z); element columns computed = (TY+1) x (TZ+1).
-constexpr int MV_TY = 16;
-constexpr int MV_TZ = 32;
-constexpr int MV_EY = MV_TY + 1;
-constexpr int MV_EZ = MV_TZ + 1;
-constexpr int MV_NY = MV_TY + 2;                  // node rows staged per plane
-constexpr int MV_NZ = MV_TZ + 2;                  // node columns staged per plane
-constexpr int MV_NCOL = MV_EY * MV_EZ;            // 561 element columns
-constexpr int MV_BLOCK = ((MV_NCOL + 63) / 64) * 64;   // 576 threads = 9 waves
-constexpr int MV_PLANE = MV_NY * MV_NZ;           // 612 nodes
-constexpr int MV_LOADS = (MV_NY * 3 * MV_NZ + MV_BLOCK - 1) / MV_BLOCK;
+template <int TY_, int TZ_>
+struct Tile {
+  static constexpr int TY = TY_, TZ = TZ_;
+  static constexpr int EY = TY + 1, EZ = TZ + 1;          // element columns (with the low halo)
+  static constexpr int NY = TY + 2, NZ = TZ + 2;          // staged node rows / columns per plane
+  static constexpr int NCOL = EY * EZ;
+  static constexpr int BLOCK = ((NCOL + 63) / 64) * 64;
+  static constexpr int PLANE = NY * NZ;
+  static constexpr int LOADS = (NY * 3 * NZ + BLOCK - 1) / BLOCK;
+};
+constexpr int MV_PREV_ZERO = 1;
+// tile variants: 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads)
+constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2;
 
 // Epilogue modes (A = constrained operator M K M + I - M when a mask is given, else K).
 enum Mode : int {
@@ -23,6 +26,8 @@ enum Mode : int {
   MODE_JACOBI = 2,  // y = u + omega * dinv * (b - A u)
   MODE_CHEB = 3,    // d = cheb_a * (dinv (b - A u) + cheb_c * d);  y = u + d
   MODE_DOT = 4,     // y = A u;  partial[tile] = sum over owned DOFs of u . A u
+  MODE_CHEB3 = 5,   // three-term Chebyshev: y (holding x_prev on entry; not read if cheb_c == 0)
+                    //   <- u + cheb_a * (dinv (b - A u) + cheb_c * (u - y))
 };
 
 template <typename T>
@@ -37,14 +42,25 @@ struct MatvecParams {
   double* partial;        // per-tile FP64 partial sums (DOT), length >= n_tiles
   int nx, ny, nz;
   int lx;                 // x chunk length (0: automatic)
+  int tile;               // tile variant (MV_TILE_*)
+  int cube;               // coef holds the 8 cube-pattern values instead of 72 block entries
+  int flags;              // MV_PREV_ZERO: CHEB3's previous iterate is zero (y not read)
   int fixed_zero;         // with a mask: the output is 0 at fixed DOFs (M K M u, (b - A u) M)
   int tiles_y, tiles_z, chunks_x, n_tiles;
   T omega, cheb_a, cheb_c;
-  T coef[72];             // parity blocks C[g][i][j]
+  T kd[3];                // diagonal of ke per component (same at every corner of a brick): with
+                          // dinv == nullptr, D^-1 = 1 / (kd[c] * sum of the incident moduli)
+  T coef[72];             // parity blocks C[g][i][j], or [p-q, q, a, b, c, d, e-f, f] (cube)
 };
 
-void matvec_geometry(int nx, int ny, int nz, int lx_hint, int* tiles_y, int* tiles_z,
-                     int* chunks_x, int* lx, int* n_tiles, int* grid);
+// Launch plan of the product for a grid: tile variant, x chunking and grid size, chosen from
+// the CU count and the resident workgroups per CU of the kernel (hint values > 0 override).
+struct MatvecPlan {
+  int tile, lx, tiles_y, tiles_z, chunks_x, n_tiles, grid;
+  double cost;
+};
+MatvecPlan matvec_plan(int dtype, int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint);
+bool cube_pattern(const double* blocks72, double* k8);
 
 template <typename T>
 int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipStream_t st);

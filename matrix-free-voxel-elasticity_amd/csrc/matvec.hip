@@ -32,11 +32,86 @@
 
 namespace vh {
 
-template <typename T, typename S, int MODE, int MASK>
-__global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
-  __shared__ T su[3][MV_PLANE];          // staged node plane (masked input), SoA
-  __shared__ T sc[4][3][MV_PLANE];       // corner contributions, indexed by node
-  __shared__ double red[MV_BLOCK / 64];
+// Block product of one element: f = E C v on the modal coefficients v[mode][comp].
+// General bricks: the 8 blocks of 3x3 (72 coefficients).  Voxel cubes (any edge length):
+// the blocks have the fixed pattern checked on the host (capi.hip, cube_pattern) with 8
+// distinct values, p, q: B0 = (p-q) I + q 11^T;  a, b: B1, B2, B4 (one zero row/column each,
+// the rigid translations);  c, d: B3, B5, B6 (a rank-one 2x2 part and one diagonal entry);
+// e, f: B7 = (e-f) I + f 11^T.  E is folded into the 8 values once per element.
+template <typename T, bool CUBE>
+__device__ __forceinline__ void element_blocks(const T (&v)[8][3], T Ee, const MatvecParams<T>& a, T (&f)[8][3]) {
+  if constexpr (CUBE) {
+    const T* k = a.coef;              // [p-q, q, a, b, c, d, e-f, f]
+    const T Epq = Ee * k[0], Eq = Ee * k[1], Ea = Ee * k[2], Eb = Ee * k[3];
+    const T Ec = Ee * k[4], Ed = Ee * k[5], Eef = Ee * k[6], Ef = Ee * k[7];
+    {  // g = 0: (v[1][0], v[2][1], v[4][2])
+      const T s = (v[1][0] + v[2][1]) + v[4][2];
+      const T qs = Eq * s;
+      f[1][0] = Epq * v[1][0] + qs;
+      f[2][1] = Epq * v[2][1] + qs;
+      f[4][2] = Epq * v[4][2] + qs;
+    }
+    {  // g = 7: (v[6][0], v[5][1], v[3][2])
+      const T s = (v[6][0] + v[5][1]) + v[3][2];
+      const T fs = Ef * s;
+      f[6][0] = Eef * v[6][0] + fs;
+      f[5][1] = Eef * v[5][1] + fs;
+      f[3][2] = Eef * v[3][2] + fs;
+    }
+    // g = 1: (v[0][0] unused, v[3][1], v[5][2])
+    f[0][0] = T(0);
+    f[3][1] = Ea * v[3][1] + Eb * v[5][2];
+    f[5][2] = Eb * v[3][1] + Ea * v[5][2];
+    // g = 2: (v[3][0], v[0][1] unused, v[6][2])
+    f[3][0] = Ea * v[3][0] + Eb * v[6][2];
+    f[0][1] = T(0);
+    f[6][2] = Eb * v[3][0] + Ea * v[6][2];
+    // g = 4: (v[5][0], v[6][1], v[0][2] unused)
+    f[5][0] = Ea * v[5][0] + Eb * v[6][1];
+    f[6][1] = Eb * v[5][0] + Ea * v[6][1];
+    f[0][2] = T(0);
+    {  // g = 3: (v[2][0], v[1][1], v[7][2])
+      const T t = Ec * (v[2][0] + v[1][1]);
+      f[2][0] = t;
+      f[1][1] = t;
+      f[7][2] = Ed * v[7][2];
+    }
+    {  // g = 5: (v[4][0], v[7][1], v[1][2])
+      const T t = Ec * (v[4][0] + v[1][2]);
+      f[4][0] = t;
+      f[1][2] = t;
+      f[7][1] = Ed * v[7][1];
+    }
+    {  // g = 6: (v[7][0], v[4][1], v[2][2])
+      const T t = Ec * (v[4][1] + v[2][2]);
+      f[4][1] = t;
+      f[2][2] = t;
+      f[7][0] = Ed * v[7][0];
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const T a0 = Ee * v[g ^ 1][0], a1 = Ee * v[g ^ 2][1], a2 = Ee * v[g ^ 4][2];
+      f[g ^ 1][0] = a.coef[9 * g + 0] * a0 + a.coef[9 * g + 1] * a1 + a.coef[9 * g + 2] * a2;
+      f[g ^ 2][1] = a.coef[9 * g + 3] * a0 + a.coef[9 * g + 4] * a1 + a.coef[9 * g + 5] * a2;
+      f[g ^ 4][2] = a.coef[9 * g + 6] * a0 + a.coef[9 * g + 7] * a1 + a.coef[9 * g + 8] * a2;
+    }
+  }
+}
+
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE>
+__global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
+  constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
+  constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
+  __shared__ T su[2][3][PLANE];       // staged node planes ix & 1 (input), SoA
+  __shared__ T sc[4][3][PLANE];       // corner contributions, indexed by node
+  __shared__ T se[PLANE];             // per element column: E(ex-1) + E(ex), for D^-1 on the fly
+  __shared__ double red[BLOCK / 64];
+  constexpr bool NEEDS_DINV = MODE == MODE_JACOBI || MODE == MODE_CHEB || MODE == MODE_CHEB3;
+  // FP64 smoothing passes recompute the left face from the still-staged plane instead of
+  // carrying it in 24 registers, which keeps them within the 168 VGPRs of 3 waves per SIMD
+  constexpr bool RECOMPUTE_FL = sizeof(T) == 8 && NEEDS_DINV;
+  const bool dinv_from_E = NEEDS_DINV && a.dinv == nullptr;
 
   // ---- XCD-aware tile decode: hardware assigns block b to XCD b % 8; give XCD k the
   // contiguous logical range [k*per, (k+1)*per).
@@ -50,15 +125,15 @@ __global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ny1 = ny + 1, nz1 = nz + 1;
-  const int oy0 = ty * MV_TY, oy1 = (ty == a.tiles_y - 1) ? ny1 : oy0 + MV_TY;
-  const int oz0 = tz * MV_TZ, oz1 = (tz == a.tiles_z - 1) ? nz1 : oz0 + MV_TZ;
+  const int oy0 = ty * TL::TY, oy1 = (ty == a.tiles_y - 1) ? ny1 : oy0 + TL::TY;
+  const int oz0 = tz * TL::TZ, oz1 = (tz == a.tiles_z - 1) ? nz1 : oz0 + TL::TZ;
   const int ox0 = cx * a.lx, ox1 = (cx == a.chunks_x - 1) ? nx + 1 : ox0 + a.lx;
   const int ybase = oy0 - 1, zbase = oz0 - 1;      // local node j <-> global base + j
 
   const int tid = threadIdx.x;
-  const int ly = tid / MV_EZ, lz = tid - (tid / MV_EZ) * MV_EZ;
+  const int ly = tid / EZ, lz = tid - (tid / EZ) * EZ;
   const int ey = ybase + ly, ez = zbase + lz;
-  const bool col_in_tile = tid < MV_NCOL;
+  const bool col_in_tile = tid < TL::NCOL;
   const bool col_ok = col_in_tile && ey >= 0 && ey < ny && ez >= 0 && ez < nz;
   const int iy_own = oy0 + ly, iz_own = oz0 + lz;  // node (ly+1, lz+1) in local coordinates
   const bool node_own = col_in_tile && iy_own < oy1 && iz_own < oz1;
@@ -66,58 +141,96 @@ __global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
   // zero the contribution slots once: slots without a writer column must read as zero
   {
     T* flat = &sc[0][0][0];
-    for (int i = tid; i < 12 * MV_PLANE; i += MV_BLOCK) flat[i] = T(0);
+    for (int i = tid; i < 12 * PLANE; i += BLOCK) flat[i] = T(0);
+    if (dinv_from_E)
+      for (int i = tid; i < PLANE; i += BLOCK) se[i] = T(0);
   }
 
   const S* __restrict__ u = reinterpret_cast<const S*>(a.u);
+  const T* __restrict__ E = a.E;
 
-  // ---- flat, coalesced plane loader: NY rows of 3*NZ contiguous values per plane.
-  T pre[MV_LOADS];
+  // ---- loader geometry (plane independent): this thread's LOADS slots of the flat plane
+  // image (NY rows of 3*NZ contiguous values), their in-plane DOF offsets and validity.
+  int ld_off[LOADS], ld_lds[LOADS];
+  unsigned ld_okbits = 0;
+#pragma unroll
+  for (int j = 0; j < LOADS; ++j) {
+    const int k = tid + j * BLOCK;
+    const int row = k / ROWLEN;
+    const int col = k - row * ROWLEN;
+    const int iy = ybase + row;
+    const int iz = zbase + col / 3;
+    const bool ok = k < NLOAD && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
+    ld_okbits |= unsigned(ok) << j;
+    ld_off[j] = ok ? 3 * (iy * nz1 + zbase) + col : 0;
+    ld_lds[j] = (col - 3 * (col / 3)) * PLANE + row * NZ + col / 3;
+  }
+  const int e_off = col_ok ? ey * nz + ez : 0;
+  const size_t pstride = size_t(3) * ny1 * nz1;     // DOFs per node plane
+  const size_t estride = size_t(ny) * nz;           // elements per element plane
+
+  // ---- branch-free prefetch of node plane ix and of this column's modulus in element plane
+  // ix-1: out-of-range slots load a clamped in-bounds address, and the zeroing of those slots
+  // is deferred to commit(), so no instruction consumes a loaded value before the next step
+  // (the batch stays in flight through this step's compute).
+  T pre[LOADS];
+  uint8_t pre_m[LOADS];
+  unsigned pre_keep = 0;
+  T pre_E;
+  bool pre_E_ok = false;
   auto prefetch = [&](int ix) {
+    // (the x = 0 clamp is applied to whole faces in face(), so the staged plane keeps the
+    //  unmasked input the epilogue needs at fixed DOFs)
+    const bool plane_ok = ix >= 0 && ix <= nx;
+    const size_t base = (ix >= 0 && ix <= nx) ? size_t(ix) * pstride : 0;
+    unsigned keep = 0;
 #pragma unroll
-    for (int j = 0; j < MV_LOADS; ++j) {
-      const int k = tid + j * MV_BLOCK;
-      T v = T(0);
-      if (k < MV_NY * 3 * MV_NZ) {
-        const int row = k / (3 * MV_NZ);
-        const int col = k - row * (3 * MV_NZ);
-        const int iy = ybase + row;
-        const int iz = zbase + col / 3;
-        if (ix >= 0 && ix <= nx && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz) {
-          const size_t dof = 3 * ((size_t(ix) * ny1 + iy) * nz1 + zbase) + col;
-          bool free_dof = true;
-          if (MASK == MASK_X0) free_dof = ix > 0;
-          if (MASK == MASK_ARRAY) free_dof = a.mask[dof] != 0;
-          if (free_dof) v = load_as<S, T>(u, dof);
-        }
-      }
-      pre[j] = v;
+    for (int j = 0; j < LOADS; ++j) {
+      const size_t dof = base + ld_off[j];
+      pre[j] = load_as<S, T>(u, dof);
+      if (MASK == MASK_ARRAY) pre_m[j] = a.mask[dof];
+      keep |= unsigned(plane_ok && ((ld_okbits >> j) & 1u)) << j;
+    }
+    pre_keep = keep;
+    const int ex = ix - 1;
+    pre_E_ok = col_ok && ex >= 0 && ex < nx;
+    pre_E = E[(pre_E_ok ? size_t(ex) * estride : 0) + e_off];
+  };
+  auto commit = [&](int ix) {
+    T* flat = &su[ix & 1][0][0];
+#pragma unroll
+    for (int j = 0; j < LOADS; ++j) {
+      bool keep = (pre_keep >> j) & 1u;
+      if (MASK == MASK_ARRAY) keep = keep && pre_m[j] != 0;
+      if (tid + j * BLOCK < NLOAD) flat[ld_lds[j]] = keep ? pre[j] : T(0);
     }
   };
-  auto commit = [&]() {
-#pragma unroll
-    for (int j = 0; j < MV_LOADS; ++j) {
-      const int k = tid + j * MV_BLOCK;
-      if (k < MV_NY * 3 * MV_NZ) {
-        const int row = k / (3 * MV_NZ);
-        const int col = k - row * (3 * MV_NZ);
-        const int jz = col / 3, c = col - 3 * (col / 3);
-        su[c][row * MV_NZ + jz] = pre[j];
-      }
-    }
-  };
-  // 2x2 (y,z) transform of this column's face: F[q], q = py + 2 pz.
-  auto face = [&](T (&F)[4][3]) {
-    const int i00 = ly * MV_NZ + lz;
+  // 2x2 (y,z) transform of this column's face of node plane `plane`: F[q], q = py + 2 pz.
+  auto face = [&](T (&F)[4][3], int plane) {
+    const int i00 = ly * NZ + lz;
+    const int pb = plane & 1;
+    T w[3][4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T u00 = su[c][i00], u01 = su[c][i00 + 1];
-      const T u10 = su[c][i00 + MV_NZ], u11 = su[c][i00 + MV_NZ + 1];
-      const T s0 = u00 + u01, d0 = u00 - u01, s1 = u10 + u11, d1 = u10 - u11;
+      w[c][0] = su[pb][c][i00];
+      w[c][1] = su[pb][c][i00 + 1];
+      w[c][2] = su[pb][c][i00 + NZ];
+      w[c][3] = su[pb][c][i00 + NZ + 1];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T s0 = w[c][0] + w[c][1], d0 = w[c][0] - w[c][1];
+      const T s1 = w[c][2] + w[c][3], d1 = w[c][2] - w[c][3];
       F[0][c] = s0 + s1;
       F[1][c] = s0 - s1;
       F[2][c] = d0 + d1;
       F[3][c] = d0 - d1;
+    }
+    if (MASK == MASK_X0 && plane == 0) {     // every DOF of node plane 0 is fixed: M u = 0 there
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F[q][c] = T(0);
     }
   };
 
@@ -128,25 +241,52 @@ __global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
     for (int c = 0; c < 3; ++c) Gc[q][c] = T(0);
 
   double dot_acc = 0.0;
-  const T* __restrict__ E = a.E;
+  T Eprev = T(0);                  // modulus of this column in element plane ex - 1
+
+  // ---- epilogue operands of this thread's owned node in node plane ex, issued at the start of
+  // step ex (they land during the element compute): b, the previous iterate / d, dinv; the
+  // input u at the node is read from the staged plane (still in LDS: planes are double-buffered)
+  constexpr bool NEED_B = MODE == MODE_RESID || NEEDS_DINV;
+  constexpr bool NEED_U = MODE != MODE_RAW || MASK != MASK_NONE;
+  T ep_b[3], ep_p[3], ep_di[3], ep_u[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = T(0);
+  const int jn_own = (ly + 1) * NZ + (lz + 1);
+  auto fetch_epilogue = [&](int ix) {
+    const bool use = node_own && ix >= ox0 && ix < ox1;
+    const size_t node = use ? (size_t(ix) * ny1 + iy_own) * nz1 + iz_own : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (NEED_U && MASK == MASK_ARRAY) ep_u[c] = load_as<S, T>(u, 3 * node + c);   // staged plane is masked
+      if (NEED_B) ep_b[c] = a.b[3 * node + c];
+      if (MODE == MODE_CHEB) ep_p[c] = a.d[3 * node + c];
+      if (MODE == MODE_CHEB3) ep_p[c] = (a.cheb_c == T(0) || (a.flags & MV_PREV_ZERO)) ? T(0) : a.y[3 * node + c];
+      if (NEEDS_DINV && !dinv_from_E) ep_di[c] = a.dinv[3 * node + c];
+    }
+  };
 
   prefetch(ox0 - 1);
   __syncthreads();                 // sc zeroed
-  commit();
+  commit(ox0 - 1);
   __syncthreads();
-  prefetch(ox0);
-  face(Fl);
+  prefetch(ox0);                   // plane ox0 and the modulus of element ox0 - 1
+  face(Fl, ox0 - 1);
+  __syncthreads();                 // face() reads of plane ox0-1 before the first commit
 
+  // Per step: commit plane ex+1 | barrier | prefetch ex+2 | face + element ex | contributions
+  // | barrier | gather.  The second barrier also orders this step's face() reads before the
+  // next step's commit, and the gather before the next step's contribution writes.
   for (int ex = ox0 - 1; ex < ox1; ++ex) {
-    __syncthreads();               // all face() reads of the previous plane are done
-    commit();                      // node plane ex + 1
+    commit(ex + 1);
+    const T Ee = pre_E_ok ? pre_E : T(0);
     __syncthreads();
     prefetch(ex + 2);
-    face(Fr);
+    fetch_epilogue(ex);
+    if (RECOMPUTE_FL) face(Fl, ex);
+    face(Fr, ex + 1);
 
     T Gl[4][3], Gr[4][3];
-    if (col_ok && ex >= 0 && ex < nx) {
-      const T Ee = E[(size_t(ex) * ny + ey) * nz + ez];
+    {
       T v[8][3];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -156,13 +296,7 @@ __global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
           v[2 * q + 1][c] = Fl[q][c] - Fr[q][c];
         }
       T f[8][3];
-#pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        const T a0 = Ee * v[g ^ 1][0], a1 = Ee * v[g ^ 2][1], a2 = Ee * v[g ^ 4][2];
-        f[g ^ 1][0] = a.coef[9 * g + 0] * a0 + a.coef[9 * g + 1] * a1 + a.coef[9 * g + 2] * a2;
-        f[g ^ 2][1] = a.coef[9 * g + 3] * a0 + a.coef[9 * g + 4] * a1 + a.coef[9 * g + 5] * a2;
-        f[g ^ 4][2] = a.coef[9 * g + 6] * a0 + a.coef[9 * g + 7] * a1 + a.coef[9 * g + 8] * a2;
-      }
+      element_blocks<T, CUBE>(v, Ee, a, f);   // Ee == 0 outside the grid: f == 0
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -170,64 +304,79 @@ __global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
           Gl[q][c] = f[2 * q][c] + f[2 * q + 1][c];
           Gr[q][c] = f[2 * q][c] - f[2 * q + 1][c];
         }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) Gl[q][c] = Gr[q][c] = T(0);
     }
 
-    if (ex >= ox0) {
-      // node plane ex is complete: carried right half of element ex-1 + left half of ex
-      if (col_in_tile) {
+    const bool finalize = ex >= ox0;     // node plane ex complete: carried right half + left half
+    if (finalize && col_in_tile) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T g0 = Gc[0][c] + Gl[0][c], g1 = Gc[1][c] + Gl[1][c];
-          const T g2 = Gc[2][c] + Gl[2][c], g3 = Gc[3][c] + Gl[3][c];
-          const T A0 = g0 + g1, A1 = g0 - g1, B0 = g2 + g3, B1 = g2 - g3;
-          const int i00 = ly * MV_NZ + lz;
-          sc[0][c][i00] = A0 + B0;                       // corner (b=0, c=0)
-          sc[1][c][i00 + 1] = A0 - B0;                   // corner (0, 1)
-          sc[2][c][i00 + MV_NZ] = A1 + B1;               // corner (1, 0)
-          sc[3][c][i00 + MV_NZ + 1] = A1 - B1;           // corner (1, 1)
-        }
+      for (int c = 0; c < 3; ++c) {
+        const T g0 = Gc[0][c] + Gl[0][c], g1 = Gc[1][c] + Gl[1][c];
+        const T g2 = Gc[2][c] + Gl[2][c], g3 = Gc[3][c] + Gl[3][c];
+        const T A0 = g0 + g1, A1 = g0 - g1, B0 = g2 + g3, B1 = g2 - g3;
+        const int i00 = ly * NZ + lz;
+        sc[0][c][i00] = A0 + B0;                       // corner (b=0, c=0)
+        sc[1][c][i00 + 1] = A0 - B0;                   // corner (0, 1)
+        sc[2][c][i00 + NZ] = A1 + B1;                  // corner (1, 0)
+        sc[3][c][i00 + NZ + 1] = A1 - B1;              // corner (1, 1)
       }
-      __syncthreads();
-      if (node_own) {
-        const int jn = (ly + 1) * MV_NZ + (lz + 1);
-        const size_t node = (size_t(ex) * ny1 + iy_own) * nz1 + iz_own;
+      if (dinv_from_E) se[ly * NZ + lz] = Eprev + Ee;
+    }
+    // unmasked input at the owned node of plane ex, read before the barrier that lets the next
+    // step overwrite this plane buffer
+    if (NEED_U && MASK != MASK_ARRAY && finalize && node_own) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T ku = ((sc[0][c][jn] + sc[1][c][jn]) + sc[2][c][jn]) + sc[3][c][jn];
-          const size_t dof = 3 * node + c;
-          bool free_dof = true;
-          if (MASK == MASK_X0) free_dof = ex > 0;
-          if (MASK == MASK_ARRAY) free_dof = a.mask[dof] != 0;
-          // unmasked input at the owned DOF (the staged plane already moved on to ex + 1);
-          // the re-read hits the cache, the plane was fetched two steps earlier
-          T uo = T(0);
-          if (MODE != MODE_RAW || MASK != MASK_NONE) uo = load_as<S, T>(u, dof);
-          // constrained operator (M K M + I - M) u, or the raw K u without a mask
-          const T au = (MASK == MASK_NONE || free_dof) ? ku : (a.fixed_zero ? T(0) : uo);
-          T out;
-          if (MODE == MODE_RAW) {
-            out = au;
-          } else if (MODE == MODE_RESID) {
-            out = a.b[dof] - au;
-          } else if (MODE == MODE_JACOBI) {
-            out = uo + a.omega * a.dinv[dof] * (a.b[dof] - au);
+      for (int c = 0; c < 3; ++c) ep_u[c] = su[ex & 1][c][jn_own];
+    }
+    __syncthreads();
+    if (finalize && node_own) {
+      const int jn = jn_own;
+      const size_t node = (size_t(ex) * ny1 + iy_own) * nz1 + iz_own;
+      T part[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[c][k] = sc[k][c][jn];
+      T esum = T(0);
+      if (dinv_from_E)   // incident columns (jy-1|jy, jz-1|jz) of node (jy, jz) = (ly+1, lz+1)
+        esum = ((se[jn - NZ - 1] + se[jn - 1]) + se[jn - NZ]) + se[jn];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T ku = ((part[c][0] + part[c][1]) + part[c][2]) + part[c][3];
+        const size_t dof = 3 * node + c;
+        bool free_dof = true;
+        if (MASK == MASK_X0) free_dof = ex > 0;
+        if (MASK == MASK_ARRAY) free_dof = a.mask[dof] != 0;
+        const T uo = ep_u[c];           // unmasked input at the owned DOF
+        // constrained operator (M K M + I - M) u, or the raw K u without a mask
+        const T au = (MASK == MASK_NONE || free_dof) ? ku : (a.fixed_zero ? T(0) : uo);
+        T out;
+        if (MODE == MODE_RAW) {
+          out = au;
+        } else if (MODE == MODE_RESID) {
+          out = ep_b[c] - au;
+        } else if (NEEDS_DINV) {
+          T dinv;
+          if (dinv_from_E) {
+            const T kdc = c == 0 ? a.kd[0] : (c == 1 ? a.kd[1] : a.kd[2]);
+            dinv = free_dof ? T(1) / (kdc * esum) : T(1);
+          }
+          else dinv = ep_di[c];
+          const T z = dinv * (ep_b[c] - au);
+          if (MODE == MODE_JACOBI) {
+            out = uo + a.omega * z;
           } else if (MODE == MODE_CHEB) {
-            const T z = a.dinv[dof] * (a.b[dof] - au);
-            const T d = a.cheb_a * (a.cheb_c == T(0) ? z : z + a.cheb_c * a.d[dof]);
+            const T d = a.cheb_a * (a.cheb_c == T(0) ? z : z + a.cheb_c * ep_p[c]);
             a.d[dof] = d;
             out = uo + d;
-          } else {  // MODE_DOT: y = A u and sum u . A u
-            out = au;
-            dot_acc += double(uo) * double(au);
+          } else {
+            out = uo + a.cheb_a * (a.cheb_c == T(0) ? z : z + a.cheb_c * (uo - ep_p[c]));
           }
-          if (MASK != MASK_NONE && a.fixed_zero && !free_dof) out = T(0);
-          a.y[dof] = out;
+        } else {  // MODE_DOT: y = A u and sum u . A u
+          out = au;
+          dot_acc += double(uo) * double(au);
         }
+        if (MASK != MASK_NONE && a.fixed_zero && !free_dof) out = T(0);
+        a.y[dof] = out;
       }
     }
 #pragma unroll
@@ -235,20 +384,123 @@ __global__ void __launch_bounds__(MV_BLOCK) parity_xmarch(MatvecParams<T> a) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         Gc[q][c] = Gr[q][c];
-        Fl[q][c] = Fr[q][c];
+        if (!RECOMPUTE_FL) Fl[q][c] = Fr[q][c];
       }
+    Eprev = Ee;
   }
 
   if (MODE == MODE_DOT) {
-    const double s = block_sum<double, MV_BLOCK>(dot_acc, red);
+    const double s = block_sum<double, BLOCK>(dot_acc, red);
     if (tid == 0) a.partial[logical] = s;
   }
 }
 
 // ------------------------------------------------------------------------------ launcher
+using Tile16 = Tile<16, 32>;
+using Tile8 = Tile<8, 32>;
+
+static int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+// Resident workgroups per CU of one kernel instantiation (cached).
+template <typename T, typename S, typename TL, bool CUBE>
+static int resident_per_cu() {
+  static int occ = 0;
+  if (occ == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch<T, S, MODE_RAW, MASK_NONE, TL, CUBE>,
+                                                    TL::BLOCK, 0) != hipSuccess || n <= 0)
+      n = 1;
+    occ = n;
+  }
+  return occ;
+}
+
+// x chunking: with P tiles per plane, S = CUs * resident workgroups per CU and c chunks, the
+// sweep takes about ceil(P c / S) rounds of (nx / c + 1) plane steps (the +1 is the halo
+// element plane recomputed at each chunk start); pick the c that minimises it.
+static void plan_chunks(int nx, int per_plane, int slots, int lx_hint, int* lx, int* chunks) {
+  int l = lx_hint;
+  if (l <= 0) {
+    double best = 1e300;
+    l = nx;
+    for (int c = 1; c <= nx; ++c) {
+      const int lc = (nx + c - 1) / c;
+      const int cc = (nx + lc - 1) / lc;
+      const double rounds = double((long(per_plane) * cc + slots - 1) / slots);
+      const double cost = rounds * (lc + 1);
+      if (cost < best - 1e-9) { best = cost; l = lc; }
+    }
+  }
+  if (l > nx) l = nx;
+  if (l < 1) l = 1;
+  *lx = l;
+  *chunks = (nx + l - 1) / l;
+}
+
+template <typename TL>
+static void plan_tile(int nx, int ny, int nz, int lx_hint, int occ, MatvecPlan* pl) {
+  const int ty = (ny + TL::TY - 1) / TL::TY;
+  const int tz = (nz + TL::TZ - 1) / TL::TZ;
+  int lx = 0, cx = 0;
+  plan_chunks(nx, ty * tz, device_cus() * occ, lx_hint, &lx, &cx);
+  pl->tiles_y = ty;
+  pl->tiles_z = tz;
+  pl->chunks_x = cx;
+  pl->lx = lx;
+  pl->n_tiles = ty * tz * cx;
+  pl->grid = ((pl->n_tiles + 7) / 8) * 8;
+  // relative cost: rounds x steps x element columns x resident workgroups
+  const long slots = long(device_cus()) * occ;
+  const long rounds = (pl->n_tiles + slots - 1) / slots;
+  pl->cost = double(rounds) * (lx + 1) * TL::NCOL * occ;
+}
+
+template <typename T, typename S, bool CUBE>
+static MatvecPlan plan_typed(int nx, int ny, int nz, int lx_hint, int tile_hint) {
+  MatvecPlan a{}, b{};
+  plan_tile<Tile16>(nx, ny, nz, lx_hint, resident_per_cu<T, S, Tile16, CUBE>(), &a);
+  a.tile = MV_TILE_16x32;
+  plan_tile<Tile8>(nx, ny, nz, lx_hint, resident_per_cu<T, S, Tile8, CUBE>(), &b);
+  b.tile = MV_TILE_8x32;
+  if (tile_hint == MV_TILE_16x32) return a;
+  if (tile_hint == MV_TILE_8x32) return b;
+  return (b.cost < a.cost) ? b : a;
+}
+
+MatvecPlan matvec_plan(int dtype, int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint) {
+  if (dtype == F64)
+    return cube ? plan_typed<double, double, true>(nx, ny, nz, lx_hint, tile_hint)
+                : plan_typed<double, double, false>(nx, ny, nz, lx_hint, tile_hint);
+  if (storage == BF16)
+    return cube ? plan_typed<float, uint16_t, true>(nx, ny, nz, lx_hint, tile_hint)
+                : plan_typed<float, uint16_t, false>(nx, ny, nz, lx_hint, tile_hint);
+  return cube ? plan_typed<float, float, true>(nx, ny, nz, lx_hint, tile_hint)
+              : plan_typed<float, float, false>(nx, ny, nz, lx_hint, tile_hint);
+}
+
 template <typename T, typename S, int MODE, int MASK>
 static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK>), dim3(grid), dim3(MV_BLOCK), 0, st, p);
+  if (p.cube) {
+    if (p.tile == MV_TILE_8x32)
+      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
+    else
+      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, true>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
+  } else {
+    if (p.tile == MV_TILE_8x32)
+      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, false>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
+    else
+      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, false>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
+  }
 }
 
 template <typename T, typename S, int MODE>
@@ -267,44 +519,26 @@ static void launch_mode(const MatvecParams<T>& p, int mode, int mask_kind, int g
     case MODE_RESID: launch_mask<T, S, MODE_RESID>(p, mask_kind, grid, st); break;
     case MODE_JACOBI: launch_mask<T, S, MODE_JACOBI>(p, mask_kind, grid, st); break;
     case MODE_CHEB: launch_mask<T, S, MODE_CHEB>(p, mask_kind, grid, st); break;
+    case MODE_CHEB3: launch_mask<T, S, MODE_CHEB3>(p, mask_kind, grid, st); break;
     default: launch_mask<T, S, MODE_DOT>(p, mask_kind, grid, st); break;
   }
 }
 
-void matvec_geometry(int nx, int ny, int nz, int lx_hint, int* tiles_y, int* tiles_z,
-                     int* chunks_x, int* lx, int* n_tiles, int* grid) {
-  const int ty = (ny + MV_TY - 1) / MV_TY;
-  const int tz = (nz + MV_TZ - 1) / MV_TZ;
-  int l = lx_hint;
-  if (l <= 0) {
-    // aim for >= 2 resident workgroups per CU on 256 CUs, with chunks of at least 8 planes
-    const int want = 1024;
-    const int per_plane = ty * tz;
-    int chunks = (want + per_plane - 1) / per_plane;
-    l = (nx + chunks - 1) / chunks;
-    if (l < 8) l = 8;
-  }
-  if (l > nx) l = nx;
-  if (l < 1) l = 1;
-  const int cx = (nx + l - 1) / l;
-  *tiles_y = ty;
-  *tiles_z = tz;
-  *chunks_x = cx;
-  *lx = l;
-  *n_tiles = ty * tz * cx;
-  *grid = ((*n_tiles + 7) / 8) * 8;
-}
-
 template <typename T>
 int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipStream_t st) {
-  int grid = 0;
-  matvec_geometry(p.nx, p.ny, p.nz, p.lx, &p.tiles_y, &p.tiles_z, &p.chunks_x, &p.lx,
-                  &p.n_tiles, &grid);
+  const int dtype = sizeof(T) == 8 ? F64 : F32;
+  const MatvecPlan pl = matvec_plan(dtype, storage, p.cube != 0, p.nx, p.ny, p.nz, p.lx, p.tile);
+  p.tile = pl.tile;
+  p.lx = pl.lx;
+  p.tiles_y = pl.tiles_y;
+  p.tiles_z = pl.tiles_z;
+  p.chunks_x = pl.chunks_x;
+  p.n_tiles = pl.n_tiles;
   if (storage == BF16) {
-    if constexpr (sizeof(T) == 4) launch_mode<T, uint16_t>(p, mode, mask_kind, grid, st);
+    if constexpr (sizeof(T) == 4) launch_mode<T, uint16_t>(p, mode, mask_kind, pl.grid, st);
     else return fail(-2, "bf16 storage requires fp32 compute");
   } else {
-    launch_mode<T, T>(p, mode, mask_kind, grid, st);
+    launch_mode<T, T>(p, mode, mask_kind, pl.grid, st);
   }
   return hip_status(hipGetLastError(), "parity_xmarch launch");
 }

@@ -65,9 +65,12 @@ class HipProduct:
         self.shape = tuple(int(s) for s in shape)
         self.dtype = _tdt(dtype)
         self.ke = np.asarray(ke, dtype=np.float64).reshape(24, 24)
-        if self.dtype == torch.float32:     # the reference casts ke to the product dtype
-            self.ke = self.ke.astype(np.float32).astype(np.float64)
+        # blocks from the FP64 ke, rounded to the product dtype inside the library
         self.blocks = Modal(self.ke).blocks
+        d = np.diag(self.ke).reshape(8, 3)
+        # per-component diagonal, when equal at every corner (axis-aligned bricks): D^-1 can then
+        # be formed from the moduli inside the smoothing pass
+        self.kdiag = d[0].copy() if np.allclose(d, d[0], rtol=1e-13, atol=0) else None
         self.ndof = 3 * ops.n_nodes(self.shape)
         self.storage = L.BF16 if storage == "bf16" else None
 
@@ -160,6 +163,11 @@ class Level:
         return ops.axpby(1.0, u, 1.0, y, w=self.fixed)
 
     # fused passes used by the V-cycle and the Krylov loops
+    def tiles(self):
+        """Number of per-tile partial sums a DOT-mode pass of this level writes."""
+        return ops.matvec_tiles(self.shape, dtype=self.dtype, blocks=self.product.blocks,
+                                storage=self.product.storage)[0]
+
     def matvec(self, u, y, **kw):
         if not self.fused:
             raise TypeError("fused passes need a HipProduct")
@@ -200,6 +208,7 @@ class Multigrid:
         if cheb_variant not in ("saad", "donor"):
             raise ValueError(cheb_variant)
         self.cheb_variant = cheb_variant
+        self.dinv_on_the_fly = True
         # coarsest: explicit inverse A^-1 = L^-T L^-1 of the constrained operator, in FP64
         Lf = coarse_factor if isinstance(coarse_factor, torch.Tensor) else torch.from_numpy(np.asarray(coarse_factor))
         Lf = Lf.to(dev, torch.float64)
@@ -213,26 +222,36 @@ class Multigrid:
             self._work.append({"x": z(), "x2": z(), "res": z(), "d": z(), "rhs": z()})
 
     # ---------------------------------------------------------------- smoothing
+    def _dinv_kw(self, Lv):
+        """D^-1 formed from the moduli inside the pass when ke allows it, else the stored array."""
+        kd = getattr(Lv.product, "kdiag", None)
+        if self.dinv_on_the_fly and kd is not None:
+            return {"kdiag": kd}
+        return {"dinv": Lv.inv_diag}
+
     def _smooth(self, level, x, r, sweeps, first):
         """Returns the buffer holding the smoothed x (x or its ping-pong partner)."""
         Lv, w = self.levels[level], self._work[level]
         other = w["x2"] if x is w["x"] else w["x"]
+        dk = self._dinv_kw(Lv)
         if self.smoother == "chebyshev":
+            # three-term form x_{k+1} = x_k + a_k (z_k + c_k (x_k - x_{k-1})), d_k = x_{k+1} - x_k;
+            # the pass overwrites x_{k-1} (held by the ping-pong partner) with x_{k+1}
             theta, delta = _cheb_params(self.lambdas[level], self.lower_frac)
-            d = w["d"]
-            if first:        # x0 = 0: z0 = D^-1 r, d0 = z0 / theta, x1 = d0
-                ops.axpby(1.0 / theta, r, 0.0, d, w=Lv.inv_diag)
+            prev_zero = False
+            if first:        # x0 = 0: x1 = D^-1 r / theta, and x_{k-1} = 0 at the next step
                 ops.axpby(1.0 / theta, r, 0.0, x, w=Lv.inv_diag)
+                prev_zero = True
             else:
-                Lv.matvec(x, other, mode=L.MODE_CHEB, b=r, dinv=Lv.inv_diag, d=d,
-                          cheb_a=1.0 / theta, cheb_c=0.0)
+                Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
                 x, other = other, x
             alpha = 2.0 / theta if self.cheb_variant == "saad" else 1.0 / theta
             for k in range(1, sweeps):
                 c = 0.25 * delta * delta * alpha
                 alpha = 1.0 / (theta - c)
                 cd = c * theta if (self.cheb_variant == "donor" and k == 1) else c
-                Lv.matvec(x, other, mode=L.MODE_CHEB, b=r, dinv=Lv.inv_diag, d=d, cheb_a=alpha, cheb_c=cd)
+                Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=alpha, cheb_c=cd, prev_zero=prev_zero, **dk)
+                prev_zero = False
                 x, other = other, x
             return x
         om = self.omegas[level]
@@ -240,7 +259,7 @@ class Multigrid:
             ops.axpby(om, r, 0.0, x, w=Lv.inv_diag)
             sweeps -= 1
         for _ in range(sweeps):
-            Lv.matvec(x, other, mode=L.MODE_JACOBI, b=r, dinv=Lv.inv_diag, omega=om)
+            Lv.matvec(x, other, mode=L.MODE_JACOBI, b=r, omega=om, **dk)
             x, other = other, x
         return x
 
@@ -305,7 +324,7 @@ def fcg(xp, apply_A, b, precondition, tol, maxiter, check_every=5, fixed_iterati
     n = b.numel()
     level = getattr(apply_A, "__self__", None)
     fused = isinstance(level, Level) and level.fused and apply_A.__func__ is Level.apply
-    red = _Reducer(n, dev, ops.matvec_tiles(level.shape)[0] if fused else 0)
+    red = _Reducer(n, dev, level.tiles() if fused else 0)
     S_NORM, S_NB = 8, 9
     red.norm2(b, S_NB)
     nb = float(red.scal[S_NB].sqrt())
@@ -321,7 +340,7 @@ def fcg(xp, apply_A, b, precondition, tol, maxiter, check_every=5, fixed_iterati
     it = 0
     for it in range(1, limit + 1):
         if fused:
-            nt = ops.matvec_tiles(level.shape)[0]
+            nt = level.tiles()
             level.matvec(p, q, mode=L.MODE_DOT, partial=red.partial)
             ops.finish(red.partial, nt, 1, red.scal, L.S_PQ, L.OP_ALPHA)
         else:

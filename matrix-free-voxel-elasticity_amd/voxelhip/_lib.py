@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libvoxelhip.so"
 
 F64, F32, BF16 = 0, 1, 2
 MASK_NONE, MASK_ARRAY, MASK_X0 = 0, 1, 2
-MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT = 0, 1, 2, 3, 4
+MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT, MODE_CHEB3 = 0, 1, 2, 3, 4, 5
 RED_DOT, RED_DOT2, RED_DOTDIFF, RED_NORM2 = 0, 1, 2, 3
 OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ = 0, 1, 2, 3, 4
 S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
@@ -27,7 +27,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 class Epilogue(ctypes.Structure):
     _fields_ = [("mode", _i), ("fixed_zero", _i), ("b", _vp), ("dinv", _vp), ("d", _vp),
-                ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i)]
+                ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i), ("tile", _i),
+                ("kdiag", _d * 3), ("flags", _i)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)
@@ -37,7 +38,7 @@ SIGNATURES = {
     "vh_modal_blocks": [_vp, _d, _vp, _dp],
     "vh_matvec": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp],
     "vh_matvec_fused": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Epilogue), _vp],
-    "vh_matvec_tiles": [_i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "vh_matvec_tiles": [_i, _i, _vp, _i, _i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "vh_dense_product": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp],
     "vh_diagonal": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp],
     "vh_element_energy": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _d, _d, _vp],

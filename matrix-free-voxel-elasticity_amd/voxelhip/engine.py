@@ -120,7 +120,7 @@ class GMGSolver:
         self.b, self.x, self.r, self.p, self.q = z(), z(), z(), z(), z()
         self.za, self.zb = z(), z()
         self.parts = ops.reduce_parts(n)
-        self.tiles = ops.matvec_tiles(self.shape)[0]
+        self.tiles = self.fine.tiles()
         self.partial = torch.zeros(2 * max(self.parts, self.tiles), dtype=torch.float64, device=self.dev)
         self.scal = torch.zeros(16, dtype=torch.float64, device=self.dev)
 

@@ -118,7 +118,8 @@ class Mask:
 
 # --------------------------------------------------------------------------- operator
 def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None, d=None,
-           partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None):
+           partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None, tile=0,
+           kdiag=None, prev_zero=False):
     """One pass of the parity product with a fused epilogue (see include/voxelhip.h)."""
     _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
     nx, ny, nz = (int(s) for s in shape)
@@ -136,16 +137,24 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
         raise ValueError(f"sizes do not match grid {shape}")
     m = mask or Mask.none()
     hb = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
+    kd = (ctypes.c_double * 3)(*([0.0] * 3 if kdiag is None else [float(v) for v in kdiag]))
+    if mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3) and dinv is None and kdiag is None:
+        raise ValueError("smoothing passes need dinv or kdiag")
     ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), _p(d), _p(partial),
-                    float(omega), float(cheb_a), float(cheb_c), int(lx))
+                    float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, int(bool(prev_zero)))
     L.call("vh_matvec_fused", dt, st, hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
            nx, ny, nz, m.kind, m.ptr, ctypes.byref(ep), _stream())
     return y
 
 
-def matvec_tiles(shape, lx=0):
+def matvec_tiles(shape, lx=0, tile=0, dtype=torch.float64, blocks=None, storage=None):
+    """(n_tiles, n_workgroups) of the product launch: n_tiles is the DOT partial count."""
     nt, nw = ctypes.c_int(0), ctypes.c_int(0)
-    L.call("vh_matvec_tiles", *[int(s) for s in shape], int(lx), ctypes.byref(nt), ctypes.byref(nw))
+    dt = _DT[dtype]
+    hb = None if blocks is None else np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
+    L.call("vh_matvec_tiles", dt, dt if storage is None else storage,
+           None if hb is None else hb.ctypes.data_as(ctypes.c_void_p), *[int(s) for s in shape], int(lx),
+           int(tile), ctypes.byref(nt), ctypes.byref(nw))
     return nt.value, nw.value
 
 

@@ -49,7 +49,7 @@ def pcg(matvec, b, M_inv, x0=None, tol=1e-5, maxiter=20000, mask=None, check_eve
     """Jacobi-PCG whose stopping test is the true residual ||b - A x|| / ||b||."""
     dev, n, dt = b.device, b.numel(), b.dtype
     lvl = _level_of(matvec)
-    tiles = ops.matvec_tiles(lvl.shape)[0] if lvl is not None else 0
+    tiles = lvl.tiles() if lvl is not None else 0
     cx = _Ctx(n, dev, tiles)
     x = torch.zeros_like(b) if x0 is None else x0.to(dt).clone()
     if mask is not None:

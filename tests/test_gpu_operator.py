@@ -141,7 +141,7 @@ def test_constrained_operator_and_epilogues(kind, shape):
     run(mode=L.MODE_CHEB, b=bd, dinv=dd, d=d, cheb_a=0.7, cheb_c=0.2)
     dn = 0.7 * (dinv * (b - Au) + 0.2 * d_old)
     assert rel(d, dn) < 1e-12 and rel(y, u + dn) < 1e-12
-    nt, _ = ops.matvec_tiles(shape)
+    nt, _ = ops.matvec_tiles(shape, blocks=blocks)
     part = torch.zeros(nt, dtype=torch.float64, device="cuda")
     run(mode=L.MODE_DOT, partial=part)
     assert float(part.sum()) == pytest.approx(float(u @ Au), rel=1e-12)
@@ -219,3 +219,74 @@ def test_full_size_product_properties():
     rot = torch.stack([-X[:, 1], X[:, 0], torch.zeros_like(X[:, 0])], 1).reshape(-1).contiguous()
     ops.matvec(blocks, E, rot, yl, shape)
     assert float(yl.abs().max()) < 1e-12 * float(rot.abs().max())
+
+
+def test_product_and_fused_passes_are_reproducible_under_repetition():
+    """Regression for LDS ordering: many launches of every epilogue must give bitwise-identical
+    output (a missing barrier between the staged-plane reads and the next plane's writes shows
+    up as run-to-run differences)."""
+    from voxelhip import _lib as L, ops
+    shape = (64, 48, 96)
+    E, u = case(shape, 8)
+    m = np.ones((shape[0] + 1, (shape[1] + 1) * (shape[2] + 1) * 3)); m[0] = 0
+    mask = ops.Mask.from_values(m.ravel(), shape)
+    blocks = hx.Modal(KE).blocks
+    ref = hx.NumpyProduct(shape, KE).raw(u, E)
+    for dt in (torch.float64, torch.float32):
+        Ed, ud = dev(E, dt), dev(u, dt)
+        first = torch.empty_like(ud)
+        ops.matvec(blocks, Ed, ud, first, shape)
+        assert rel(first, ref) < (1e-12 if dt == torch.float64 else 1e-5)
+        y = torch.empty_like(ud)
+        for _ in range(30):
+            ops.matvec(blocks, Ed, ud, y, shape)
+            assert torch.equal(y, first)
+        ops.matvec(blocks, Ed, ud, first, shape, mask=mask, mode=L.MODE_RESID, b=ud)
+        for _ in range(10):
+            ops.matvec(blocks, Ed, ud, y, shape, mask=mask, mode=L.MODE_RESID, b=ud)
+            assert torch.equal(y, first)
+
+
+@pytest.mark.parametrize("kind", ["x0", "array"])
+def test_three_term_chebyshev_and_diagonal_from_moduli(kind):
+    from voxelhip import _lib as L, ops
+    shape = (18, 17, 34)
+    rng = np.random.default_rng(12)
+    E, u = case(shape, 6)
+    m = _masks(shape, rng)[kind]
+    prod = hx.NumpyProduct(shape, KE)
+    lvl = mg_ref.Level(shape, prod, E, m, np.float64)
+    Au = lvl.apply(u)
+    b = rng.standard_normal(u.size)
+    prev = rng.standard_normal(u.size)
+    dinv = lvl.inv_diag
+    kd = np.diag(KE)[:3]
+    mask = ops.Mask.from_values(m, shape)
+    blocks = hx.Modal(KE).blocks
+    Ed, ud, bd = dev(E), dev(u), dev(b)
+    for dk in ({"dinv": dev(dinv)}, {"kdiag": kd}):
+        y = dev(prev)
+        ops.matvec(blocks, Ed, ud, y, shape, mask=mask, mode=L.MODE_CHEB3, b=bd, cheb_a=0.7, cheb_c=0.2, **dk)
+        assert rel(y, u + 0.7 * (dinv * (b - Au) + 0.2 * (u - prev))) < 1e-12
+        y = dev(prev)
+        ops.matvec(blocks, Ed, ud, y, shape, mask=mask, mode=L.MODE_CHEB3, b=bd, cheb_a=0.7, cheb_c=0.2,
+                   prev_zero=True, **dk)
+        assert rel(y, u + 0.7 * (dinv * (b - Au) + 0.2 * u)) < 1e-12
+        y = torch.full_like(ud, float("nan"))
+        ops.matvec(blocks, Ed, ud, y, shape, mask=mask, mode=L.MODE_CHEB3, b=bd, cheb_a=0.7, cheb_c=0.0, **dk)
+        assert rel(y, u + 0.7 * dinv * (b - Au)) < 1e-12
+        ops.matvec(blocks, Ed, ud, y, shape, mask=mask, mode=L.MODE_JACOBI, b=bd, omega=0.43, **dk)
+        assert rel(y, u + 0.43 * dinv * (b - Au)) < 1e-12
+
+
+def test_fp32_product_takes_the_cube_path_and_matches_fp64_blocks():
+    """The FP32 product keeps the voxel-cube fast path (blocks from the FP64 element matrix)."""
+    import voxel_mg as mg
+    shape = (20, 18, 40)
+    E, u = case(shape, 9)
+    p32 = mg.HipProduct(shape, KE, np.float32)
+    from voxelhip import ops
+    assert ops.matvec_tiles(shape, dtype=torch.float32, blocks=p32.blocks) == \
+        ops.matvec_tiles(shape, dtype=torch.float32, blocks=None)
+    y = p32.raw(dev(u, torch.float32), dev(E, torch.float32))
+    assert rel(y, hx.NumpyProduct(shape, KE).raw(u, E)) < 1e-5
