@@ -58,6 +58,7 @@ int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, con
  * DOT y = A u and partial[t] = sum of u . A u over tile t (FP64, length vh_matvec_tiles());
  * CHEB3 y = u + cheb_a (dinv (b - A u) + cheb_c (u - y)), y holding the previous iterate on
  * entry (not read when cheb_c == 0): the three-term form of CHEB without a direction array.
+ * (Load-time combination and extra outputs of DOT / RESID: see the w, sdev, u_out, y32 fields.)
  * JACOBI/CHEB/CHEB3 with dinv == NULL compute D^-1 = 1 / (kdiag[c] * sum of the incident E)
  * at free DOFs and 1 at fixed ones (kdiag: the diagonal of ke per component, which an
  * axis-aligned brick has equal at its 8 corners).
@@ -76,6 +77,14 @@ typedef struct vh_epilogue {
   int tile; /* (y, z) tile variant: 0 = automatic, 1 = 16 x 32 nodes, 2 = 8 x 32 nodes */
   double kdiag[3];
   int flags; /* VH_PREV_ZERO: CHEB3's previous iterate is zero (y is then not read) */
+  /* DOT / RESID only: the pass reads u + s w with s = sdev[sidx] (device FP64 scalar) and writes
+   * that combined input to u_out at every DOF (voxel-cube blocks);  RESID also writes a float32
+   * copy of y to y32 and, with partial != NULL, per-tile sums of y^2. */
+  const void* w;
+  const double* sdev;
+  int sidx;
+  void* u_out;
+  float* y32;
 } vh_epilogue;
 #define VH_PREV_ZERO 1
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
@@ -134,6 +143,12 @@ int vh_finish(const double* partial, int nparts, int count, double* scal, int ds
  * (alpha = scal[VH_S_ALPHA]).  Replaces: the update lines of donor pcg simp_r2.py:266-284. */
 int vh_cg_update(int dtype, size_t n, void* x, const void* p, void* r, const void* q, const void* dinv, void* z,
                  const double* scal, int update_r, double* partial, void* stream);
+/* znew = (float64) zin with FP64 partials [r.znew | r.(znew - zold)] (layout of VH_RED_DOTDIFF,
+ * vh_reduce_parts(n) each): the V-cycle output turned into the next preconditioned vector of a
+ * flexible CG step together with both Polak-Ribiere dots.
+ * Replaces: rz_new = r @ z_new and beta = r @ (z_new - z) / rz of fcg src/voxel_mg.py:209-211. */
+int vh_cvt_dotdiff(int dtype_in, size_t n, const void* zin, const double* r, const double* zold, double* znew,
+                   double* partial, void* stream);
 /* z = dinv r and partial r.z.  Replaces: z = M_inv * r; rz = r.z of simp_r2.py:239-242,282-283. */
 int vh_jacobi_dot(int dtype, size_t n, const void* r, const void* dinv, void* z, double* partial, void* stream);
 /* p = z + beta p (beta = scal[VH_S_BETA]); if zcopy != NULL also zcopy = z in the same pass.

@@ -178,6 +178,37 @@ __global__ void __launch_bounds__(BLAS_BLOCK) convert_kernel(size_t n, const Ti*
   }
 }
 
+// z_new = (double) z_in; partials of r.z_new and r.(z_new - z_old): the preconditioned vector
+// of a flexible CG iteration taken from the V-cycle output (any precision) with both
+// Polak-Ribiere dots in the same pass.
+template <typename Ti>
+__global__ void __launch_bounds__(BLAS_BLOCK) cvt_dotdiff_kernel(size_t n, const Ti* __restrict__ zin,
+                                                                const double* __restrict__ r,
+                                                                const double* __restrict__ zold,
+                                                                double* __restrict__ znew,
+                                                                double* __restrict__ partial) {
+  __shared__ double red[2][BLAS_BLOCK / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    const double z = double(zin[i]);
+    const double ri = r[i];
+    znew[i] = z;
+    s0 += ri * z;
+    s1 += ri * (z - zold[i]);
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wv] = s0; red[1][wv] = s1; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < BLAS_BLOCK / 64; ++k) { a += red[0][k]; b += red[1][k]; }
+    partial[blockIdx.x] = a;
+    partial[gridDim.x + blockIdx.x] = b;
+  }
+}
+
 // Dense symmetric matrix-vector product y = mask * (A r): one wave per row (coarsest level).
 template <typename T>
 __global__ void __launch_bounds__(256) dense_mv_kernel(int n, const T* __restrict__ A, const T* __restrict__ r,
@@ -254,6 +285,15 @@ int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dsc
                      inv_scale, accumulate);
   return hip_status(hipGetLastError(), "convert_kernel");
 }
+
+template <typename Ti>
+int cvt_dotdiff_launch(size_t n, const Ti* zin, const double* r, const double* zold, double* znew, double* partial,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(cvt_dotdiff_kernel<Ti>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, zin, r, zold, znew, partial);
+  return hip_status(hipGetLastError(), "cvt_dotdiff_kernel");
+}
+template int cvt_dotdiff_launch<double>(size_t, const double*, const double*, const double*, double*, double*, hipStream_t);
+template int cvt_dotdiff_launch<float>(size_t, const float*, const double*, const double*, double*, double*, hipStream_t);
 
 template <typename T>
 int dense_mv_launch(int n, const T* A, const T* r, T* y, const uint8_t* mask, hipStream_t st) {

@@ -127,6 +127,8 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
   p.mask = mask;
   p.nx = nx; p.ny = ny; p.nz = nz;
   double k8[8];
+  if (ep && ep->w && !cube_pattern(blocks72, k8))
+    return fail(-2, "vh_matvec: the load-time combination is implemented for voxel-cube blocks");
   if (cube_pattern(blocks72, k8)) {
     p.cube = 1;
     for (int i = 0; i < 8; ++i) p.coef[i] = T(k8[i]);
@@ -147,6 +149,11 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
     p.tile = ep->tile;
     for (int c = 0; c < 3; ++c) p.kd[c] = T(ep->kdiag[c]);
     p.flags = ep->flags;
+    p.w = ep->w;
+    p.sdev = ep->sdev;
+    p.sidx = ep->sidx;
+    p.u_out = static_cast<T*>(ep->u_out);
+    p.y32 = static_cast<float*>(ep->y32);
     p.fixed_zero = ep->fixed_zero;
   }
   return matvec_launch<T>(p, storage, mode, mask_kind, st);
@@ -205,6 +212,10 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
     VH_REQUIRE(!(m == MODE_RESID || smoother) || ep->b, "vh_matvec: b missing");
     VH_REQUIRE(m != MODE_CHEB || ep->d, "vh_matvec: d missing");
     VH_REQUIRE(m != MODE_DOT || ep->partial, "vh_matvec: partial missing");
+    VH_REQUIRE(!ep->w || ((m == MODE_DOT || m == MODE_RESID) && ep->sdev && ep->u_out && ep->u_out != u &&
+                          ep->u_out != y && ep->w != y && ep->u_out != ep->w),
+               "vh_matvec: the load-time combination needs a DOT/RESID pass, sdev and a distinct u_out");
+    VH_REQUIRE(!ep->y32 || m == MODE_RESID, "vh_matvec: y32 is an output of the RESID pass");
     VH_REQUIRE(ep->b != y && (ep->dinv == nullptr || ep->dinv != y), "vh_matvec: y must not alias an input");
     if (smoother && ep->dinv == nullptr)   // D^-1 from the moduli and ke's diagonal
       VH_REQUIRE(ep->kdiag[0] > 0 && ep->kdiag[1] > 0 && ep->kdiag[2] > 0, "vh_matvec: kdiag missing");
@@ -348,6 +359,16 @@ int vh_cg_update(int dtype, size_t n, void* x, const void* p, void* r, const voi
   return cg_update_launch<float>(n, static_cast<float*>(x), static_cast<const float*>(p), static_cast<float*>(r),
                                  static_cast<const float*>(q), static_cast<const float*>(dinv), static_cast<float*>(z), scal,
                                  update_r, partial, st);
+}
+
+int vh_cvt_dotdiff(int dtype_in, size_t n, const void* zin, const double* r, const double* zold, double* znew,
+                   double* partial, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype_in) && zin && r && zold && znew && partial && n >= 1, "vh_cvt_dotdiff: bad arguments");
+  VH_REQUIRE(static_cast<const void*>(znew) != zin && znew != zold && znew != r, "vh_cvt_dotdiff: znew aliases an input");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype_in == F64)
+    return cvt_dotdiff_launch<double>(n, static_cast<const double*>(zin), r, zold, znew, partial, st);
+  return cvt_dotdiff_launch<float>(n, static_cast<const float*>(zin), r, zold, znew, partial, st);
 }
 
 int vh_jacobi_dot(int dtype, size_t n, const void* r, const void* dinv, void* z, double* partial, void* stream) {

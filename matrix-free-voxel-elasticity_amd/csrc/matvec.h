@@ -39,7 +39,14 @@ struct MatvecParams {
   const T* dinv;          // inverse diagonal (JACOBI, CHEB)
   T* d;                   // Chebyshev direction, updated in place (CHEB)
   const uint8_t* mask;    // per-DOF free flags (MASK_ARRAY)
-  double* partial;        // per-tile FP64 partial sums (DOT), length >= n_tiles
+  double* partial;        // per-tile FP64 partial sums (DOT: u.Au; RESID: |y|^2), length >= n_tiles
+  // load-time combination u <- u + s w (s = sdev[sidx], a device scalar); the combined input
+  // is written to u_out at the owned nodes (DOT and RESID passes of the Krylov loops)
+  const void* w;
+  const double* sdev;
+  int sidx;
+  T* u_out;
+  float* y32;             // RESID: FP32 copy of the output (the V-cycle right-hand side)
   int nx, ny, nz;
   int lx;                 // x chunk length (0: automatic)
   int tile;               // tile variant (MV_TILE_*)

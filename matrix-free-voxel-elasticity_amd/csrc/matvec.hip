@@ -99,7 +99,7 @@ __device__ __forceinline__ void element_blocks(const T (&v)[8][3], T Ee, const M
   }
 }
 
-template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE>
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE = false>
 __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
   constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
@@ -147,7 +147,9 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   }
 
   const S* __restrict__ u = reinterpret_cast<const S*>(a.u);
+  const S* __restrict__ wv = reinterpret_cast<const S*>(a.w);
   const T* __restrict__ E = a.E;
+  const T sw = COMBINE ? T(a.sdev[a.sidx]) : T(0);
 
   // ---- loader geometry (plane independent): this thread's LOADS slots of the flat plane
   // image (NY rows of 3*NZ contiguous values), their in-plane DOF offsets and validity.
@@ -174,6 +176,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   // is deferred to commit(), so no instruction consumes a loaded value before the next step
   // (the batch stays in flight through this step's compute).
   T pre[LOADS];
+  T pre_w[COMBINE ? LOADS : 1];
   uint8_t pre_m[LOADS];
   unsigned pre_keep = 0;
   T pre_E;
@@ -188,6 +191,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     for (int j = 0; j < LOADS; ++j) {
       const size_t dof = base + ld_off[j];
       pre[j] = load_as<S, T>(u, dof);
+      if (COMBINE) pre_w[j] = load_as<S, T>(wv, dof);
       if (MASK == MASK_ARRAY) pre_m[j] = a.mask[dof];
       keep |= unsigned(plane_ok && ((ld_okbits >> j) & 1u)) << j;
     }
@@ -202,7 +206,8 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     for (int j = 0; j < LOADS; ++j) {
       bool keep = (pre_keep >> j) & 1u;
       if (MASK == MASK_ARRAY) keep = keep && pre_m[j] != 0;
-      if (tid + j * BLOCK < NLOAD) flat[ld_lds[j]] = keep ? pre[j] : T(0);
+      const T v = COMBINE ? pre[j] + sw * pre_w[j] : pre[j];
+      if (tid + j * BLOCK < NLOAD) flat[ld_lds[j]] = keep ? v : T(0);
     }
   };
   // 2x2 (y,z) transform of this column's face of node plane `plane`: F[q], q = py + 2 pz.
@@ -247,7 +252,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   // step ex (they land during the element compute): b, the previous iterate / d, dinv; the
   // input u at the node is read from the staged plane (still in LDS: planes are double-buffered)
   constexpr bool NEED_B = MODE == MODE_RESID || NEEDS_DINV;
-  constexpr bool NEED_U = MODE != MODE_RAW || MASK != MASK_NONE;
+  constexpr bool NEED_U = MODE != MODE_RAW || MASK != MASK_NONE || COMBINE;
   T ep_b[3], ep_p[3], ep_di[3], ep_u[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = T(0);
@@ -257,7 +262,10 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     const size_t node = use ? (size_t(ix) * ny1 + iy_own) * nz1 + iz_own : 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      if (NEED_U && MASK == MASK_ARRAY) ep_u[c] = load_as<S, T>(u, 3 * node + c);   // staged plane is masked
+      if (NEED_U && MASK == MASK_ARRAY) {       // the staged plane is masked: fetch the input
+        ep_u[c] = load_as<S, T>(u, 3 * node + c);
+        if (COMBINE) ep_u[c] += sw * load_as<S, T>(wv, 3 * node + c);
+      }
       if (NEED_B) ep_b[c] = a.b[3 * node + c];
       if (MODE == MODE_CHEB) ep_p[c] = a.d[3 * node + c];
       if (MODE == MODE_CHEB3) ep_p[c] = (a.cheb_c == T(0) || (a.flags & MV_PREV_ZERO)) ? T(0) : a.y[3 * node + c];
@@ -354,6 +362,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
           out = au;
         } else if (MODE == MODE_RESID) {
           out = ep_b[c] - au;
+          if (a.partial) dot_acc += double(out) * double(out);
         } else if (NEEDS_DINV) {
           T dinv;
           if (dinv_from_E) {
@@ -377,6 +386,8 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
         }
         if (MASK != MASK_NONE && a.fixed_zero && !free_dof) out = T(0);
         a.y[dof] = out;
+        if (COMBINE) a.u_out[dof] = uo;
+        if (MODE == MODE_RESID && a.y32) a.y32[dof] = float(out);
       }
     }
 #pragma unroll
@@ -389,7 +400,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     Eprev = Ee;
   }
 
-  if (MODE == MODE_DOT) {
+  if (MODE == MODE_DOT || (MODE == MODE_RESID && a.partial != nullptr)) {
     const double s = block_sum<double, BLOCK>(dot_acc, red);
     if (tid == 0) a.partial[logical] = s;
   }
@@ -490,6 +501,15 @@ MatvecPlan matvec_plan(int dtype, int storage, bool cube, int nx, int ny, int nz
 
 template <typename T, typename S, int MODE, int MASK>
 static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
+  if constexpr (MODE == MODE_DOT || MODE == MODE_RESID) {
+    if (p.w) {     // load-time combination (voxel-cube blocks only; checked by the caller)
+      if (p.tile == MV_TILE_8x32)
+        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true, true>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
+      else
+        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, true, true>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
+      return;
+    }
+  }
   if (p.cube) {
     if (p.tile == MV_TILE_8x32)
       hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);

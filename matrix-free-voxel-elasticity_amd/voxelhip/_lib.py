@@ -28,7 +28,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class Epilogue(ctypes.Structure):
     _fields_ = [("mode", _i), ("fixed_zero", _i), ("b", _vp), ("dinv", _vp), ("d", _vp),
                 ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i), ("tile", _i),
-                ("kdiag", _d * 3), ("flags", _i)]
+                ("kdiag", _d * 3), ("flags", _i), ("w", _vp), ("sdev", _vp), ("sidx", _i),
+                ("u_out", _vp), ("y32", _vp)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)
@@ -51,6 +52,7 @@ SIGNATURES = {
     "vh_finish": [_vp, _i, _i, _vp, _i, _i, _vp],
     "vh_cg_update": [_i, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "vh_jacobi_dot": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
+    "vh_cvt_dotdiff": [_i, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "vh_xpby": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
     "vh_axpby": [_i, _sz, _d, _vp, _vp, _d, _vp, _vp],
     "vh_convert": [_i, _i, _sz, _vp, _vp, _d, _vp, _i, _i, _vp],

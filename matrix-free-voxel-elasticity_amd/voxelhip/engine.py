@@ -13,8 +13,6 @@ precision:
   "fp64"          FP64 everywhere (the reference's plain FP64 GMG-FCG path)
   "mixed"         FP64 outer flexible CG and true residual, FP32 V-cycle (matvec, smoothing,
                   transfers, coarsest solve in FP32), FP64 dot accumulation
-  "mixed-bf16"    as "mixed" with the fine-level FP32 products reading u stored in bfloat16
-                  (storage ablation; the smoother iterate is rounded to bf16 before each product)
 """
 from __future__ import annotations
 
@@ -39,7 +37,7 @@ class GMGSolver:
             shapes = shapes[:n_levels]
         self.shapes = shapes
         self.smoother, self.degree, self.pre, self.post = smoother, degree, pre, post
-        if precision not in ("fp64", "mixed", "mixed-bf16"):
+        if precision not in ("fp64", "mixed"):
             raise ValueError(precision)
         self.precision = precision
         self.pdtype = torch.float64 if precision == "fp64" else torch.float32
@@ -59,9 +57,7 @@ class GMGSolver:
         """(Re)build every level for a new modulus field (device or host array)."""
         E64 = E.to(self.dev, torch.float64).contiguous() if isinstance(E, torch.Tensor) else \
             torch.from_numpy(np.ascontiguousarray(E, dtype=np.float64)).to(self.dev)
-        prods = [mg.HipProduct(s, self.ke * 2 ** i, self.pdtype,
-                               storage="bf16" if (i == 0 and self.precision == "mixed-bf16") else None)
-                 for i, s in enumerate(self.shapes)]
+        prods = [mg.HipProduct(s, self.ke * 2 ** i, self.pdtype) for i, s in enumerate(self.shapes)]
         self.fine = mg.Level(None, self.shape, mg.HipProduct(self.shape, self.ke, torch.float64), E64,
                              self.mask, torch.float64)
         # coarsest operator: rediscretised (ke * 2^L on the averaged modulus), or Galerkin
@@ -117,92 +113,87 @@ class GMGSolver:
     def _alloc(self):
         n = self.fine.n
         z = lambda: torch.zeros(n, dtype=torch.float64, device=self.dev)  # noqa: E731
-        self.b, self.x, self.r, self.p, self.q = z(), z(), z(), z(), z()
-        self.za, self.zb = z(), z()
+        self.b, self.r, self.q = z(), z(), z()
+        self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
         self.parts = ops.reduce_parts(n)
         self.tiles = self.fine.tiles()
         self.partial = torch.zeros(2 * max(self.parts, self.tiles), dtype=torch.float64, device=self.dev)
         self.scal = torch.zeros(16, dtype=torch.float64, device=self.dev)
+        # right-hand side of the finest V-cycle level: the FP32 copy of r written by the
+        # residual pass (mixed), or r itself (FP64)
+        self.rhs0 = self.M._work[0]["rhs"] if self.pdtype != torch.float64 else self.r
 
     # ------------------------------------------------------------------ one iteration
-    def _iteration(self, z_new, z_old, check):
-        """Launch one flexible-CG iteration (no host synchronisation)."""
-        f = self.fine
-        f.matvec(self.p, self.q, mode=L.MODE_DOT, partial=self.partial)          # q = A p, p.q
-        ops.finish(self.partial, self.tiles, 1, self.scal, L.S_PQ, L.OP_ALPHA)
-        if check:
-            ops.cg_update(self.x, self.p, self.scal)                              # x += a p
-            f.matvec(self.x, self.r, mode=L.MODE_RESID, b=self.b)                # r = b - A x
-            ops.reduce(L.RED_NORM2, self.r, partial=self.partial)
-            ops.finish(self.partial, self.parts, 1, self.scal, 8)
-        else:
-            ops.cg_update(self.x, self.p, self.scal, r=self.r, q=self.q)          # + r -= a q
-        self.M(self.r, out=z_new)                                                 # z = M r
-        ops.reduce(L.RED_DOTDIFF, self.r, z_new, z_old, partial=self.partial)
-        ops.finish(self.partial, self.parts, 2, self.scal, 5, L.OP_BETA_PR)
-        ops.xpby(z_new, self.p, self.scal)                                        # p = z + beta p
+    def _iteration(self, i):
+        """Launch flexible-CG iteration with buffer parity i (no host synchronisation).
 
-    def _capture(self, check):
-        graphs = []
+        DOT pass:   p_i = z_i + beta p_o (read on the fly), q = A p_i, p_i.q     -> alpha
+        RESID pass: x_i = x_o + alpha p_i (on the fly), r = b - A x_i, |r|^2, r -> fp32
+        V-cycle on r, then z_o = its output with r.z_o and r.(z_o - z_i)          -> beta, r.z
+        """
+        o = 1 - i
+        f, L_ = self.fine, L
+        f.matvec(self.zs[i], self.q, mode=L_.MODE_DOT, partial=self.partial,
+                 w=self.ps[o], sdev=self.scal, sidx=L_.S_BETA, u_out=self.ps[i])
+        ops.finish(self.partial, self.tiles, 1, self.scal, L_.S_PQ, L_.OP_ALPHA)
+        f.matvec(self.xs[o], self.r, mode=L_.MODE_RESID, b=self.b, partial=self.partial,
+                 w=self.ps[i], sdev=self.scal, sidx=L_.S_ALPHA, u_out=self.xs[i],
+                 y32=self.rhs0 if self.pdtype != torch.float64 else None)
+        ops.finish(self.partial, self.tiles, 1, self.scal, 8)
+        zl = self.M.vcycle(self.rhs0)
+        ops.cvt_dotdiff(zl, self.r, self.zs[i], self.zs[o], self.partial)
+        ops.finish(self.partial, self.parts, 2, self.scal, 5, L_.OP_BETA_PR)
+
+    def _capture(self):
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):          # warm-up on the capture stream
-            self._iteration(self.za, self.zb, check)
+        with torch.cuda.stream(s):          # warm-up on a side stream (state is reset afterwards)
+            self._iteration(0)
+            self._iteration(1)
         torch.cuda.current_stream().wait_stream(s)
-        for zn, zo in ((self.za, self.zb), (self.zb, self.za)):
+        graphs = []
+        for i in (0, 1):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._iteration(zn, zo, check)
+                self._iteration(i)
             graphs.append(g)
         return graphs
 
     # ------------------------------------------------------------------ solve
-    def solve(self, b, tol=1e-8, maxiter=200, check_every=1, x0=None):
-        """Flexible CG to ||b - A x|| / ||b|| <= tol.  Returns (x, iterations, converged, rel, matvecs).
-        The returned x is a view of the solver's buffer (copy it to keep it across solves)."""
-        f = self.fine
+    def solve(self, b, tol=1e-8, maxiter=200):
+        """Flexible CG (the true residual is formed and tested every iteration, as the reference
+        harness runs fcg with check_every=1) to ||b - A x|| / ||b|| <= tol from x = 0.
+        Returns (x, iterations, converged, rel, matvecs); x is a view of the solver's buffer."""
+        if self.graph and self._graphs is None:
+            self._graphs = self._capture()
         self.b.copy_(b)
         ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
         ops.finish(self.partial, self.parts, 1, self.scal, 9)
         nb = float(self.scal[9].sqrt())
-        matvecs = 0
-        if x0 is None:
-            self.x.zero_()
-            self.r.copy_(self.b)
-        else:
-            self.x.copy_(x0)
-            f.matvec(self.x, self.r, mode=L.MODE_RESID, b=self.b)
-            matvecs += 1
-        self.M(self.r, out=self.zb)
-        self.p.copy_(self.zb)
-        ops.reduce(L.RED_DOT, self.r, self.zb, partial=self.partial)
+        # x = 0, r = b, z = M r, p = z (beta = 0 against a zero previous direction), rz = r.z
+        self.xs[1].zero_()
+        self.ps[1].zero_()
+        self.r.copy_(self.b)
+        if self.pdtype != torch.float64:
+            ops.convert(self.r, self.rhs0)
+        zl = self.M.vcycle(self.rhs0)
+        ops.convert(zl, self.zs[0])
+        ops.reduce(L.RED_DOT, self.r, self.zs[0], partial=self.partial)
         ops.finish(self.partial, self.parts, 1, self.scal, L.S_RZ)
-        if self.graph and self._graphs is None and check_every == 1:
-            # capture clobbers x / r / p through its warm-up: snapshot and restore them
-            saved = [t.clone() for t in (self.x, self.r, self.p, self.zb, self.scal)]
-            self._graphs = self._capture(True)
-            for t, s in zip((self.x, self.r, self.p, self.zb, self.scal), saved):
-                t.copy_(s)
+        self.scal[L.S_BETA] = 0.0
+        matvecs = 0
         it, rel = 0, float("nan")
-        zpair = [(self.za, self.zb), (self.zb, self.za)]
         for it in range(1, maxiter + 1):
-            check = it % check_every == 0
-            k = (it - 1) % 2
-            if self._graphs is not None and check:
-                self._graphs[k].replay()
+            i = (it - 1) % 2
+            if self._graphs is not None:
+                self._graphs[i].replay()
             else:
-                self._iteration(*zpair[k], check)
-            matvecs += 2 if check else 1
-            if check:
-                rel = float(self.scal[8].sqrt()) / nb
-                if rel <= tol:
-                    return self.x, it, True, rel, matvecs
-        f.matvec(self.x, self.r, mode=L.MODE_RESID, b=self.b)
-        matvecs += 1
-        ops.reduce(L.RED_NORM2, self.r, partial=self.partial)
-        ops.finish(self.partial, self.parts, 1, self.scal, 8)
-        rel = float(self.scal[8].sqrt()) / nb
-        return self.x, it, bool(rel <= tol), rel, matvecs
+                self._iteration(i)
+            matvecs += 2
+            rel = float(self.scal[8].sqrt()) / nb
+            if rel <= tol:
+                return self.xs[i], it, True, rel, matvecs
+        return self.xs[(it - 1) % 2], it, bool(rel <= tol), rel, matvecs
 
 
 def _prolongation_matrix(coarse):

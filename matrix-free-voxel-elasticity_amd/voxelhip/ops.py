@@ -119,7 +119,7 @@ class Mask:
 # --------------------------------------------------------------------------- operator
 def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None, d=None,
            partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None, tile=0,
-           kdiag=None, prev_zero=False):
+           kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None):
     """One pass of the parity product with a fused epilogue (see include/voxelhip.h)."""
     _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
     nx, ny, nz = (int(s) for s in shape)
@@ -141,7 +141,8 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
     if mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3) and dinv is None and kdiag is None:
         raise ValueError("smoothing passes need dinv or kdiag")
     ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), _p(d), _p(partial),
-                    float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, int(bool(prev_zero)))
+                    float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, int(bool(prev_zero)),
+                    _p(w), _p(sdev), int(sidx), _p(u_out), _p(y32))
     L.call("vh_matvec_fused", dt, st, hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
            nx, ny, nz, m.kind, m.ptr, ctypes.byref(ep), _stream())
     return y
@@ -237,6 +238,12 @@ def cg_update(x, p, scal, r=None, q=None, dinv=None, z=None, partial=None):
 
 def jacobi_dot(r, dinv, z, partial):
     L.call("vh_jacobi_dot", dtype_code(r), int(r.numel()), _p(r), _p(dinv), _p(z), _p(partial), _stream())
+
+
+def cvt_dotdiff(zin, r, zold, znew, partial):
+    _dev(zin, "zin"); _dev(r, "r"); _dev(zold, "zold"); _dev(znew, "znew")
+    L.call("vh_cvt_dotdiff", dtype_code(zin), int(r.numel()), _p(zin), _p(r), _p(zold), _p(znew), _p(partial),
+           _stream())
 
 
 def xpby(z, p, scal, zcopy=None):

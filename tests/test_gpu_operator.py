@@ -290,3 +290,49 @@ def test_fp32_product_takes_the_cube_path_and_matches_fp64_blocks():
         ops.matvec_tiles(shape, dtype=torch.float32, blocks=None)
     y = p32.raw(dev(u, torch.float32), dev(E, torch.float32))
     assert rel(y, hx.NumpyProduct(shape, KE).raw(u, E)) < 1e-5
+
+
+def test_fused_krylov_passes():
+    """DOT pass with p = z + beta p_old formed on load, RESID pass with x = x_old + alpha p formed
+    on load (+ FP32 copy and |r|^2 partials), and the convert + Polak-Ribiere-dots kernel."""
+    from voxelhip import _lib as L, ops
+    shape = (20, 18, 36)
+    rng = np.random.default_rng(21)
+    E, z = case(shape, 10)
+    m = _masks(shape, rng)["x0"]
+    lvl = mg_ref.Level(shape, hx.NumpyProduct(shape, KE), E, m, np.float64)
+    p_old, x_old, b = (rng.standard_normal(z.size) for _ in range(3))
+    beta, alpha = 0.37, -1.25
+    scal = torch.zeros(16, dtype=torch.float64, device="cuda")
+    scal[L.S_BETA], scal[L.S_ALPHA] = beta, alpha
+    mask = ops.Mask.from_values(m, shape)
+    blocks = hx.Modal(KE).blocks
+    Ed = dev(E)
+    nt, _ = ops.matvec_tiles(shape, blocks=blocks)
+    part = torch.zeros(nt, dtype=torch.float64, device="cuda")
+    q, pn = torch.empty_like(Ed.new_empty(z.size)), torch.empty(z.size, dtype=torch.float64, device="cuda")
+    ops.matvec(blocks, Ed, dev(z), q, shape, mask=mask, mode=L.MODE_DOT, partial=part, w=dev(p_old), sdev=scal,
+               sidx=L.S_BETA, u_out=pn)
+    p_ref = z + beta * p_old
+    Ap = lvl.apply(p_ref)
+    assert rel(pn, p_ref) < 1e-15 and rel(q, Ap) < 1e-12
+    assert float(part.sum()) == pytest.approx(float(p_ref @ Ap), rel=1e-12)
+    r, xn = torch.empty_like(pn), torch.empty_like(pn)
+    r32 = torch.empty(z.size, dtype=torch.float32, device="cuda")
+    ops.matvec(blocks, Ed, dev(x_old), r, shape, mask=mask, mode=L.MODE_RESID, b=dev(b), partial=part,
+               w=pn, sdev=scal, sidx=L.S_ALPHA, u_out=xn, y32=r32)
+    x_ref = x_old + alpha * p_ref
+    r_ref = b - lvl.apply(x_ref)
+    assert rel(xn, x_ref) < 1e-15 and rel(r, r_ref) < 1e-12
+    assert float(part.sum()) == pytest.approx(float(r_ref @ r_ref), rel=1e-12)
+    assert rel(r32, r_ref) < 1e-7
+    zin = rng.standard_normal(z.size).astype(np.float32)
+    zold = rng.standard_normal(z.size)
+    znew = torch.empty_like(pn)
+    nparts = ops.reduce_parts(z.size)
+    p2 = torch.zeros(2 * nparts, dtype=torch.float64, device="cuda")
+    ops.cvt_dotdiff(dev(zin, torch.float32), r, dev(zold), znew, p2)
+    zd = zin.astype(np.float64)
+    assert torch.equal(znew, dev(zd))
+    assert float(p2[:nparts].sum()) == pytest.approx(float(r_ref @ zd), rel=1e-12)
+    assert float(p2[nparts:].sum()) == pytest.approx(float(r_ref @ (zd - zold)), rel=1e-12)
