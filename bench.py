@@ -32,6 +32,16 @@ sys.path.insert(0, str(ROOT / "matrix-free-voxel-elasticity_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12           # MI355X HBM3E bytes/s
+# HBM bytes per launch of the FP64 product at 256 x 128 x 128, from FETCH_SIZE / WRITE_SIZE
+# counter runs of `bench.py --only matvec`, each scaled by the same counter read on a plain copy
+TRAFFIC_FILE = ROOT / "profiles" / "parity_xmarch_fp64_256x128x128_traffic.json"
+
+
+def measured_traffic():
+    try:
+        return int(json.loads(TRAFFIC_FILE.read_text())["traffic_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def problem(shape, seed, rmin=1.5):
@@ -135,6 +145,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--only", choices=["matvec"], default=None,
+                    help="matvec: only the dominant kernel (FP64 product at 256x128x128) and a plain "
+                         "copy of known size, for counter collection")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -144,6 +157,16 @@ def main():
     import voxel_kernels as vk
     from voxelhip.engine import GMGSolver
     torch.cuda.set_device(0)
+    if args.only == "matvec":
+        dom = matvec_timing((256, 128, 128), torch.float64)
+        nd = 3 * 257 * 129 * 129
+        src = torch.ones(nd, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):          # plain copy of 2 x 102.6 MB: the counter calibration reference
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"only": "matvec", "matvec": dom, "copy_bytes_each_way": nd * 8}), flush=True)
+        return
     ke = 2 * vk.quadrature([1, 1, 1], 0.3)          # the donor's KE_UNIT_3D (ke_unit_3d.npy)
     shape = (256, 128, 128)
     rho, E, force, mask, ops = problem(shape, seed=3)
@@ -197,7 +220,7 @@ def main():
         "solves": out,
         "matvec": mv,
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": dom["frac_hbm"], "traffic": None,
+                     "frac": dom["frac_hbm"], "traffic": measured_traffic(),
                      "kernel": "parity_xmarch<double> (FP64 product, 256x128x128)",
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "avg_launch_us": dom["us"]},
         "cpu_baseline": cpu,
