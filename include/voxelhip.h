@@ -126,9 +126,14 @@ int vh_prolong(int dtype, const void* ec, void* xf, int nx, int ny, int nz, int 
 /* Ec = 2x2x2 mean of Ef.  Replaces: coarsen_modulus src/voxel_mg.py:108-111. */
 int vh_coarsen_modulus(int dtype, const void* Ef, void* Ec, int nx, int ny, int nz, void* stream);
 
-/* y = mask * (A r) for a dense row-major n x n matrix (the coarsest solve with the explicit
- * inverse of the coarsest operator).  Replaces: Multigrid.coarse_solve src/voxel_mg.py:152-159. */
-int vh_dense_mv(int dtype, int n, const void* A, const void* r, void* y, const uint8_t* mask, void* stream);
+/* y = mask * (A r) for a dense row-major n x n matrix with leading dimension ld (ld >= n,
+ * ld % 4 == 0, A 16-byte aligned; the padding columns are not read): the coarsest solve with the
+ * explicit inverse of the coarsest operator.  a_dtype is the matrix type, v_dtype the type of r
+ * and y (an FP64 matrix may take FP32 vectors: the coarsest solve in FP64 inside an FP32
+ * V-cycle; accumulation in the matrix type).
+ * Replaces: Multigrid.coarse_solve src/voxel_mg.py:152-159. */
+int vh_dense_mv(int a_dtype, int v_dtype, int n, int ld, const void* A, const void* r, void* y, const uint8_t* mask,
+                void* stream);
 
 /* Deterministic two-stage reductions with device-resident FP64 scalars.
  * vh_reduce writes FP64 partials (count = 2 for DOT2/DOTDIFF, laid out [j][part]) of length
@@ -143,11 +148,12 @@ int vh_finish(const double* partial, int nparts, int count, double* scal, int ds
  * (alpha = scal[VH_S_ALPHA]).  Replaces: the update lines of donor pcg simp_r2.py:266-284. */
 int vh_cg_update(int dtype, size_t n, void* x, const void* p, void* r, const void* q, const void* dinv, void* z,
                  const double* scal, int update_r, double* partial, void* stream);
-/* znew = (float64) zin with FP64 partials [r.znew | r.(znew - zold)] (layout of VH_RED_DOTDIFF,
- * vh_reduce_parts(n) each): the V-cycle output turned into the next preconditioned vector of a
+/* znew = zin converted to dtype_v, with FP64 partials [r.znew | r.(znew - zold)] (layout of
+ * VH_RED_DOTDIFF, vh_reduce_parts(n) each); r, zold, znew are of dtype_v (FP64 with an FP32 or
+ * FP64 input, or all FP32): the V-cycle output turned into the next preconditioned vector of a
  * flexible CG step together with both Polak-Ribiere dots.
  * Replaces: rz_new = r @ z_new and beta = r @ (z_new - z) / rz of fcg src/voxel_mg.py:209-211. */
-int vh_cvt_dotdiff(int dtype_in, size_t n, const void* zin, const double* r, const double* zold, double* znew,
+int vh_cvt_dotdiff(int dtype_in, int dtype_v, size_t n, const void* zin, const void* r, const void* zold, void* znew,
                    double* partial, void* stream);
 /* z = dinv r and partial r.z.  Replaces: z = M_inv * r; rz = r.z of simp_r2.py:239-242,282-283. */
 int vh_jacobi_dot(int dtype, size_t n, const void* r, const void* dinv, void* z, double* partial, void* stream);

@@ -21,8 +21,9 @@ template <typename T> int xpby_launch(size_t n, const T* z, T* p, const double* 
 template <typename T> int axpby_launch(size_t n, T a, const T* x, const T* w, T b, T* y, hipStream_t st);
 template <typename Ti, typename To> int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dscale,
                                                        int inv_scale, int accumulate, hipStream_t st);
-template <typename Ti> int cvt_dotdiff_launch(size_t n, const Ti* zin, const double* r, const double* zold,
-                                              double* znew, double* partial, hipStream_t st);
-template <typename T> int dense_mv_launch(int n, const T* A, const T* r, T* y, const uint8_t* mask, hipStream_t st);
+template <typename Ti, typename Tv>
+int cvt_dotdiff_launch(size_t n, const Ti* zin, const Tv* r, const Tv* zold, Tv* znew, double* partial, hipStream_t st);
+template <typename TA, typename TV>
+int dense_mv_launch(int n, int ld, const TA* A, const TV* r, TV* y, const uint8_t* mask, hipStream_t st);
 
 }  // namespace vh

@@ -3,6 +3,8 @@
 // dot product, then a single-block "finish" kernel sums the partials in a fixed order and
 // evaluates the scalar recurrence (alpha, beta, ...) in FP64 on the device, so a solver
 // iteration needs no host synchronisation and can be captured in a HIP graph.
+#include <type_traits>
+
 #include "common.h"
 #include "blas.h"
 
@@ -181,20 +183,21 @@ __global__ void __launch_bounds__(BLAS_BLOCK) convert_kernel(size_t n, const Ti*
 // z_new = (double) z_in; partials of r.z_new and r.(z_new - z_old): the preconditioned vector
 // of a flexible CG iteration taken from the V-cycle output (any precision) with both
 // Polak-Ribiere dots in the same pass.
-template <typename Ti>
+template <typename Ti, typename Tv>
 __global__ void __launch_bounds__(BLAS_BLOCK) cvt_dotdiff_kernel(size_t n, const Ti* __restrict__ zin,
-                                                                const double* __restrict__ r,
-                                                                const double* __restrict__ zold,
-                                                                double* __restrict__ znew,
+                                                                const Tv* __restrict__ r,
+                                                                const Tv* __restrict__ zold,
+                                                                Tv* __restrict__ znew,
                                                                 double* __restrict__ partial) {
   __shared__ double red[2][BLAS_BLOCK / 64];
   double s0 = 0.0, s1 = 0.0;
   for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
-    const double z = double(zin[i]);
-    const double ri = r[i];
-    znew[i] = z;
+    const Tv zv = Tv(zin[i]);
+    const double z = double(zv);
+    const double ri = double(r[i]);
+    znew[i] = zv;
     s0 += ri * z;
-    s1 += ri * (z - zold[i]);
+    s1 += ri * (z - double(zold[i]));
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
@@ -210,17 +213,44 @@ __global__ void __launch_bounds__(BLAS_BLOCK) cvt_dotdiff_kernel(size_t n, const
 }
 
 // Dense symmetric matrix-vector product y = mask * (A r): one wave per row (coarsest level).
-template <typename T>
-__global__ void __launch_bounds__(256) dense_mv_kernel(int n, const T* __restrict__ A, const T* __restrict__ r,
-                                                      T* __restrict__ y, const uint8_t* __restrict__ mask) {
+// y = mask * (A r): one wave per row, 4 rows per workgroup.  A is row-major with leading
+// dimension ld (a multiple of 4, rows 16-byte aligned), read with 16-byte loads; the ld - n
+// padding columns are never read.  Accumulation in the matrix type (FP64 for the FP64 coarsest
+// inverse, also when the level vectors are FP32).
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) dense_mv_kernel(int n, int ld, const TA* __restrict__ A,
+                                                      const TV* __restrict__ r, TV* __restrict__ y,
+                                                      const uint8_t* __restrict__ mask) {
+  constexpr int V = 16 / int(sizeof(TA));
+  using Vec = typename std::conditional<sizeof(TA) == 8, double2, float4>::type;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const T* a = A + size_t(row) * n;
-  T acc = T(0);
-  for (int j = lane; j < n; j += 64) acc += a[j] * r[j];
-  acc = wave_sum(acc);
-  if (lane == 0) y[row] = (mask && !mask[row]) ? T(0) : acc;
+  const TA* a = A + size_t(row) * ld;
+  const int nfull = (n / V) * V;
+  TA acc0 = TA(0), acc1 = TA(0);
+  int j = lane * V;
+#pragma unroll 2
+  for (; j + 64 * V < nfull; j += 128 * V) {      // two independent 16-byte loads in flight
+    const Vec x0 = *reinterpret_cast<const Vec*>(a + j);
+    const Vec x1 = *reinterpret_cast<const Vec*>(a + j + 64 * V);
+    const TA* e0 = reinterpret_cast<const TA*>(&x0);
+    const TA* e1 = reinterpret_cast<const TA*>(&x1);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      acc0 += e0[k] * TA(r[j + k]);
+      acc1 += e1[k] * TA(r[j + 64 * V + k]);
+    }
+  }
+  if (j < nfull) {
+    const Vec x0 = *reinterpret_cast<const Vec*>(a + j);
+    const TA* e0 = reinterpret_cast<const TA*>(&x0);
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc0 += e0[k] * TA(r[j + k]);
+  }
+  for (int t = nfull + lane; t < n; t += 64) acc1 += a[t] * TA(r[t]);
+  TA acc = wave_sum(acc0 + acc1);
+  if (lane == 0) y[row] = (mask && !mask[row]) ? TV(0) : TV(acc);
 }
 
 // --------------------------------------------------------------------------- launchers
@@ -286,28 +316,34 @@ int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dsc
   return hip_status(hipGetLastError(), "convert_kernel");
 }
 
-template <typename Ti>
-int cvt_dotdiff_launch(size_t n, const Ti* zin, const double* r, const double* zold, double* znew, double* partial,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(cvt_dotdiff_kernel<Ti>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, zin, r, zold, znew, partial);
+template <typename Ti, typename Tv>
+int cvt_dotdiff_launch(size_t n, const Ti* zin, const Tv* r, const Tv* zold, Tv* znew, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL((cvt_dotdiff_kernel<Ti, Tv>), dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, zin, r, zold, znew,
+                     partial);
   return hip_status(hipGetLastError(), "cvt_dotdiff_kernel");
 }
-template int cvt_dotdiff_launch<double>(size_t, const double*, const double*, const double*, double*, double*, hipStream_t);
-template int cvt_dotdiff_launch<float>(size_t, const float*, const double*, const double*, double*, double*, hipStream_t);
+template int cvt_dotdiff_launch<double, double>(size_t, const double*, const double*, const double*, double*, double*,
+                                                hipStream_t);
+template int cvt_dotdiff_launch<float, double>(size_t, const float*, const double*, const double*, double*, double*,
+                                               hipStream_t);
+template int cvt_dotdiff_launch<float, float>(size_t, const float*, const float*, const float*, float*, double*,
+                                              hipStream_t);
 
-template <typename T>
-int dense_mv_launch(int n, const T* A, const T* r, T* y, const uint8_t* mask, hipStream_t st) {
-  hipLaunchKernelGGL(dense_mv_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, A, r, y, mask);
+template <typename TA, typename TV>
+int dense_mv_launch(int n, int ld, const TA* A, const TV* r, TV* y, const uint8_t* mask, hipStream_t st) {
+  hipLaunchKernelGGL((dense_mv_kernel<TA, TV>), dim3((n + 3) / 4), dim3(256), 0, st, n, ld, A, r, y, mask);
   return hip_status(hipGetLastError(), "dense_mv_kernel");
 }
+template int dense_mv_launch<double, double>(int, int, const double*, const double*, double*, const uint8_t*, hipStream_t);
+template int dense_mv_launch<double, float>(int, int, const double*, const float*, float*, const uint8_t*, hipStream_t);
+template int dense_mv_launch<float, float>(int, int, const float*, const float*, float*, const uint8_t*, hipStream_t);
 
 #define VH_INST(T)                                                                                           \
   template int reduce_launch<T>(int, size_t, const T*, const T*, const T*, double*, hipStream_t);            \
   template int cg_update_launch<T>(size_t, T*, const T*, T*, const T*, const T*, T*, const double*, int, double*, hipStream_t); \
   template int jacobi_dot_launch<T>(size_t, const T*, const T*, T*, double*, hipStream_t);                    \
   template int xpby_launch<T>(size_t, const T*, T*, const double*, T*, hipStream_t);                              \
-  template int axpby_launch<T>(size_t, T, const T*, const T*, T, T*, hipStream_t);                            \
-  template int dense_mv_launch<T>(int, const T*, const T*, T*, const uint8_t*, hipStream_t);
+  template int axpby_launch<T>(size_t, T, const T*, const T*, T, T*, hipStream_t);
 VH_INST(double)
 VH_INST(float)
 #undef VH_INST

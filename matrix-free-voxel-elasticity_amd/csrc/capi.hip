@@ -105,7 +105,12 @@ bool cube_pattern(const double* B, double* k8) {
   return true;
 }
 
-static bool dims_ok(int nx, int ny, int nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
+// every kernel indexes DOFs with 32-bit counts below 2^31; the transfer kernels stage one z-line
+// of 3 (nz + 1) values in LDS
+static bool dims_ok(int nx, int ny, int nz) {
+  return nx >= 1 && ny >= 1 && nz >= 1 && nz <= 2048 &&
+         3.0 * (nx + 1.0) * (ny + 1.0) * (nz + 1.0) < 2147483648.0;
+}
 static bool dtype_ok(int d) { return d == F64 || d == F32; }
 
 }  // namespace vh
@@ -315,14 +320,21 @@ int vh_coarsen_modulus(int dtype, const void* Ef, void* Ec, int nx, int ny, int 
   return coarsen_modulus_launch<float>(static_cast<const float*>(Ef), static_cast<float*>(Ec), nx, ny, nz, st);
 }
 
-int vh_dense_mv(int dtype, int n, const void* A, const void* r, void* y, const uint8_t* mask, void* stream) {
-  VH_REQUIRE(dtype_ok(dtype) && n >= 1 && A && r && y && r != y, "vh_dense_mv: bad arguments");
+int vh_dense_mv(int a_dtype, int v_dtype, int n, int ld, const void* A, const void* r, void* y, const uint8_t* mask,
+                void* stream) {
+  VH_REQUIRE(dtype_ok(a_dtype) && dtype_ok(v_dtype) && n >= 1 && ld >= n && ld % 4 == 0 && A && r && y && r != y,
+             "vh_dense_mv: bad arguments");
+  VH_REQUIRE((reinterpret_cast<uintptr_t>(A) & 15) == 0, "vh_dense_mv: A must be 16-byte aligned");
+  VH_REQUIRE(!(a_dtype == F32 && v_dtype == F64), "vh_dense_mv: an FP32 matrix takes FP32 vectors");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == F64)
-    return dense_mv_launch<double>(n, static_cast<const double*>(A), static_cast<const double*>(r), static_cast<double*>(y),
-                                   mask, st);
-  return dense_mv_launch<float>(n, static_cast<const float*>(A), static_cast<const float*>(r), static_cast<float*>(y), mask,
-                                st);
+  if (a_dtype == F64 && v_dtype == F64)
+    return dense_mv_launch<double, double>(n, ld, static_cast<const double*>(A), static_cast<const double*>(r),
+                                           static_cast<double*>(y), mask, st);
+  if (a_dtype == F64)
+    return dense_mv_launch<double, float>(n, ld, static_cast<const double*>(A), static_cast<const float*>(r),
+                                          static_cast<float*>(y), mask, st);
+  return dense_mv_launch<float, float>(n, ld, static_cast<const float*>(A), static_cast<const float*>(r),
+                                       static_cast<float*>(y), mask, st);
 }
 
 int vh_reduce_parts(size_t n) { return blas_grid(n); }
@@ -361,14 +373,21 @@ int vh_cg_update(int dtype, size_t n, void* x, const void* p, void* r, const voi
                                  update_r, partial, st);
 }
 
-int vh_cvt_dotdiff(int dtype_in, size_t n, const void* zin, const double* r, const double* zold, double* znew,
+int vh_cvt_dotdiff(int dtype_in, int dtype_v, size_t n, const void* zin, const void* r, const void* zold, void* znew,
                    double* partial, void* stream) {
-  VH_REQUIRE(dtype_ok(dtype_in) && zin && r && zold && znew && partial && n >= 1, "vh_cvt_dotdiff: bad arguments");
-  VH_REQUIRE(static_cast<const void*>(znew) != zin && znew != zold && znew != r, "vh_cvt_dotdiff: znew aliases an input");
+  VH_REQUIRE(dtype_ok(dtype_in) && dtype_ok(dtype_v) && zin && r && zold && znew && partial && n >= 1,
+             "vh_cvt_dotdiff: bad arguments");
+  VH_REQUIRE(znew != zin && znew != zold && znew != r, "vh_cvt_dotdiff: znew aliases an input");
+  VH_REQUIRE(!(dtype_in == F64 && dtype_v == F32), "vh_cvt_dotdiff: FP64 input into FP32 vectors");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype_v == F32)
+    return cvt_dotdiff_launch<float, float>(n, static_cast<const float*>(zin), static_cast<const float*>(r),
+                                            static_cast<const float*>(zold), static_cast<float*>(znew), partial, st);
   if (dtype_in == F64)
-    return cvt_dotdiff_launch<double>(n, static_cast<const double*>(zin), r, zold, znew, partial, st);
-  return cvt_dotdiff_launch<float>(n, static_cast<const float*>(zin), r, zold, znew, partial, st);
+    return cvt_dotdiff_launch<double, double>(n, static_cast<const double*>(zin), static_cast<const double*>(r),
+                                              static_cast<const double*>(zold), static_cast<double*>(znew), partial, st);
+  return cvt_dotdiff_launch<float, double>(n, static_cast<const float*>(zin), static_cast<const double*>(r),
+                                           static_cast<const double*>(zold), static_cast<double*>(znew), partial, st);
 }
 
 int vh_jacobi_dot(int dtype, size_t n, const void* r, const void* dinv, void* z, double* partial, void* stream) {

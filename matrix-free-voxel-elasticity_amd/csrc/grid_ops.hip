@@ -153,89 +153,90 @@ __global__ void __launch_bounds__(256) dense_node_kernel(const T* __restrict__ u
 // 1/2 if |i - 2I| == 1 (src/voxel_mg.py:74-105: P copies even nodes, averages odd ones; R = P^T).
 //
 // restrict: rc[I] = mask_c[I] * sum_{a,b,c in -1..1} w(a) w(b) w(c) (rf * mask_f)[2I + (a,b,c)]
+//
+// Both kernels run one workgroup per z-line of nodes (all iz, 3 components: a contiguous run of
+// 3 (nz + 1) values in the author layout), so every global access is a coalesced sweep along a
+// line.  restrict: the x/y weights combine up to 9 fine lines into one weighted fine line in
+// LDS, then each coarse DOF takes its 3 z neighbours from LDS.  prolong: up to 4 coarse lines
+// are combined with the x/y weights into LDS, then each fine DOF interpolates along z.
 template <typename T>
-__global__ void restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc, const GridArgs<T> g,
-                                const uint8_t* __restrict__ cmask, int cmask_kind) {
+__global__ void __launch_bounds__(256) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
+                                                      const GridArgs<T> g, const uint8_t* __restrict__ cmask,
+                                                      int cmask_kind) {
+  extern __shared__ unsigned char smem_raw[];
+  T* line = reinterpret_cast<T*>(smem_raw);             // 3 (nz + 1) weighted fine values
   // g describes the FINE grid; coarse extents are half.
-  const int cx = g.nx / 2, cy = g.ny / 2, cz = g.nz / 2;
-  const size_t n = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
-  const size_t nc = size_t(cx + 1) * (cy + 1) * (cz + 1);
-  if (n >= nc) return;
-  const int Iz = int(n % (cz + 1));
-  const int Iy = int((n / (cz + 1)) % (cy + 1));
-  const int Ix = int(n / (size_t(cz + 1) * (cy + 1)));
-  const int fy1 = g.ny + 1, fz1 = g.nz + 1;
-  T acc[3] = {T(0), T(0), T(0)};
+  const int cy1 = g.ny / 2 + 1, cz1 = g.nz / 2 + 1;
+  const int fy1 = g.ny + 1, nlf = 3 * (g.nz + 1), nlc = 3 * cz1;
+  const int Iy = int(blockIdx.x % unsigned(cy1)), Ix = int(blockIdx.x / unsigned(cy1));
+  for (int t = threadIdx.x; t < nlf; t += blockDim.x) {
+    T acc = T(0);
 #pragma unroll
-  for (int a = -1; a <= 1; ++a) {
-    const int ix = 2 * Ix + a;
-    if (ix < 0 || ix > g.nx) continue;
-    const T wa = a == 0 ? T(1) : T(0.5);
+    for (int a = -1; a <= 1; ++a) {
+      const int ix = 2 * Ix + a;
+      if (ix < 0 || ix > g.nx || (g.mask_kind == MASK_X0 && ix == 0)) continue;
+      const T wa = a == 0 ? T(1) : T(0.5);
 #pragma unroll
-    for (int b = -1; b <= 1; ++b) {
-      const int iy = 2 * Iy + b;
-      if (iy < 0 || iy > g.ny) continue;
-      const T wb = wa * (b == 0 ? T(1) : T(0.5));
-#pragma unroll
-      for (int c = -1; c <= 1; ++c) {
-        const int iz = 2 * Iz + c;
-        if (iz < 0 || iz > g.nz) continue;
-        const T w = wb * (c == 0 ? T(1) : T(0.5));
-        const size_t fnode = (size_t(ix) * fy1 + iy) * fz1 + iz;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          T v = rf[3 * fnode + k];
-          if (g.mask_kind == MASK_X0 && ix == 0) v = T(0);
-          if (g.mask_kind == MASK_ARRAY && !g.mask[3 * fnode + k]) v = T(0);
-          acc[k] += w * v;
-        }
+      for (int b = -1; b <= 1; ++b) {
+        const int iy = 2 * Iy + b;
+        if (iy < 0 || iy > g.ny) continue;
+        const size_t dof = (size_t(ix) * fy1 + iy) * nlf + t;
+        T v = rf[dof];
+        if (g.mask_kind == MASK_ARRAY && !g.mask[dof]) v = T(0);
+        acc += (wa * (b == 0 ? T(1) : T(0.5))) * v;
       }
     }
+    line[t] = acc;
   }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  __syncthreads();
+  const size_t cbase = (size_t(Ix) * cy1 + Iy) * nlc;
+  for (int t = threadIdx.x; t < nlc; t += blockDim.x) {
+    const int Iz = t / 3, k = t - 3 * Iz;
+    T acc = line[6 * Iz + k];
+    if (Iz > 0) acc += T(0.5) * line[6 * Iz - 3 + k];
+    if (2 * Iz + 1 <= g.nz) acc += T(0.5) * line[6 * Iz + 3 + k];
     bool free_dof = true;
     if (cmask_kind == MASK_X0) free_dof = Ix > 0;
-    if (cmask_kind == MASK_ARRAY) free_dof = cmask[3 * n + k] != 0;
-    rc[3 * n + k] = free_dof ? acc[k] : T(0);
+    if (cmask_kind == MASK_ARRAY) free_dof = cmask[cbase + t] != 0;
+    rc[cbase + t] = free_dof ? acc : T(0);
   }
 }
 
-// prolong-add: xf[i] += mask_f[i] * (P ec)[i]  (src/voxel_mg.py:172);  with `assign`,
-// xf[i] = mask_f[i] * (P ec)[i].
+// prolong: xf (+)= mask_f * P ec.
 template <typename T>
-__global__ void prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf, const GridArgs<T> g,
-                               int assign) {
-  const size_t n = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
-  const size_t nn = size_t(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  if (n >= nn) return;
-  const int iz = int(n % (g.nz + 1));
-  const int iy = int((n / (g.nz + 1)) % (g.ny + 1));
-  const int ix = int(n / (size_t(g.nz + 1) * (g.ny + 1)));
-  const int cy1 = g.ny / 2 + 1, cz1 = g.nz / 2 + 1;
-  const int x0 = ix >> 1, x1 = (ix + 1) >> 1;
-  const int y0 = iy >> 1, y1 = (iy + 1) >> 1;
-  const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
-  const T wx = (ix & 1) ? T(0.5) : T(1), wy = (iy & 1) ? T(0.5) : T(1), wz = (iz & 1) ? T(0.5) : T(1);
-  T acc[3] = {T(0), T(0), T(0)};
-  const int xs[2] = {x0, x1}, ys[2] = {y0, y1}, zs[2] = {z0, z1};
-  const int nxs = (ix & 1) ? 2 : 1, nys = (iy & 1) ? 2 : 1, nzs = (iz & 1) ? 2 : 1;
-  for (int a = 0; a < nxs; ++a)
-    for (int b = 0; b < nys; ++b)
-      for (int c = 0; c < nzs; ++c) {
-        const size_t cn = (size_t(xs[a]) * cy1 + ys[b]) * cz1 + zs[c];
-        const T w = wx * wy * wz;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) acc[k] += w * ec[3 * cn + k];
-      }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const size_t dof = 3 * n + k;
-    bool free_dof = true;
-    if (g.mask_kind == MASK_X0) free_dof = ix > 0;
-    if (g.mask_kind == MASK_ARRAY) free_dof = g.mask[dof] != 0;
-    const T p = free_dof ? acc[k] : T(0);
-    xf[dof] = assign ? p : xf[dof] + p;
+__global__ void __launch_bounds__(256) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
+                                                     const GridArgs<T> g, int assign) {
+  extern __shared__ unsigned char smem_raw[];
+  T* cl = reinterpret_cast<T*>(smem_raw);               // 3 (nz/2 + 1) x/y-interpolated coarse values
+  const int fy1 = g.ny + 1, nlf = 3 * (g.nz + 1);
+  const int cy1 = g.ny / 2 + 1, nlc = 3 * (g.nz / 2 + 1);
+  const int iy = int(blockIdx.x % unsigned(fy1)), ix = int(blockIdx.x / unsigned(fy1));
+  const int x0 = ix >> 1, x1 = (ix + 1) >> 1, y0 = iy >> 1, y1 = (iy + 1) >> 1;
+  const T wxy = ((ix & 1) ? T(0.5) : T(1)) * ((iy & 1) ? T(0.5) : T(1));
+  const T* c00 = ec + (size_t(x0) * cy1 + y0) * nlc;
+  const T* c01 = ec + (size_t(x0) * cy1 + y1) * nlc;
+  const T* c10 = ec + (size_t(x1) * cy1 + y0) * nlc;
+  const T* c11 = ec + (size_t(x1) * cy1 + y1) * nlc;
+  for (int t = threadIdx.x; t < nlc; t += blockDim.x) {
+    T v = c00[t];
+    if (iy & 1) v += c01[t];
+    if (ix & 1) {
+      v += c10[t];
+      if (iy & 1) v += c11[t];
+    }
+    cl[t] = wxy * v;
+  }
+  __syncthreads();
+  const size_t fbase = (size_t(ix) * fy1 + iy) * nlf;
+  const bool plane_free = !(g.mask_kind == MASK_X0 && ix == 0);
+  for (int t = threadIdx.x; t < nlf; t += blockDim.x) {
+    const int iz = t / 3, k = t - 3 * iz, z0 = iz >> 1;
+    T p = cl[3 * z0 + k];
+    if (iz & 1) p = T(0.5) * (p + cl[3 * z0 + 3 + k]);
+    bool free_dof = plane_free;
+    if (g.mask_kind == MASK_ARRAY) free_dof = g.mask[fbase + t] != 0;
+    if (!free_dof) p = T(0);
+    xf[fbase + t] = assign ? p : xf[fbase + t] + p;
   }
 }
 
@@ -286,15 +287,17 @@ int dense_launch(const T* u, const T* E, const T* ke, T* y, int nx, int ny, int 
 
 template <typename T>
 int restrict_launch(const T* rf, T* rc, const GridArgs<T>& g, const uint8_t* cmask, int cmask_kind, hipStream_t st) {
-  const size_t nc = size_t(g.nx / 2 + 1) * (g.ny / 2 + 1) * (g.nz / 2 + 1);
-  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(nc, 256)), dim3(256), 0, st, rf, rc, g, cmask, cmask_kind);
+  const unsigned lines = unsigned(g.nx / 2 + 1) * unsigned(g.ny / 2 + 1);
+  const size_t lds = size_t(3) * (g.nz + 1) * sizeof(T);
+  hipLaunchKernelGGL(restrict_kernel<T>, dim3(lines), dim3(256), lds, st, rf, rc, g, cmask, cmask_kind);
   return hip_status(hipGetLastError(), "restrict_kernel");
 }
 
 template <typename T>
 int prolong_launch(const T* ec, T* xf, const GridArgs<T>& g, int assign, hipStream_t st) {
-  const size_t nn = size_t(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  hipLaunchKernelGGL(prolong_kernel<T>, dim3(blocks_for(nn, 256)), dim3(256), 0, st, ec, xf, g, assign);
+  const unsigned lines = unsigned(g.nx + 1) * unsigned(g.ny + 1);
+  const size_t lds = size_t(3) * (g.nz / 2 + 1) * sizeof(T);
+  hipLaunchKernelGGL(prolong_kernel<T>, dim3(lines), dim3(256), lds, st, ec, xf, g, assign);
   return hip_status(hipGetLastError(), "prolong_kernel");
 }
 

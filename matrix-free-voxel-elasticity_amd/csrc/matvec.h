@@ -66,7 +66,12 @@ struct MatvecPlan {
   int tile, lx, tiles_y, tiles_z, chunks_x, n_tiles, grid;
   double cost;
 };
-MatvecPlan matvec_plan(int dtype, int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint);
+MatvecPlan matvec_plan_f64(bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint);
+MatvecPlan matvec_plan_f32(int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint);
+inline MatvecPlan matvec_plan(int dtype, int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint) {
+  return dtype == F64 ? matvec_plan_f64(cube, nx, ny, nz, lx_hint, tile_hint)
+                      : matvec_plan_f32(storage, cube, nx, ny, nz, lx_hint, tile_hint);
+}
 bool cube_pattern(const double* blocks72, double* k8);
 
 template <typename T>

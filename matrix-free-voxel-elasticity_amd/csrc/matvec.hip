@@ -19,14 +19,20 @@
 //    half of element ex-1 before the (y,z) inverse.  Per element this is 48 + 72 + 24 + 48
 //    + 12 FP64 operations (forward, blocks, modulus, inverse, carry) instead of 240 with an
 //    8-lane shuffle butterfly.
-//  * Node planes are staged through LDS (SoA, coalesced flat loads with a one-plane
-//    register prefetch); the four corner contributions of each node are gathered from LDS in
-//    a fixed order, so results are bitwise reproducible.
+//  * Node planes are staged through LDS in their HBM order (rows of (x, y, z) triples: lane k
+//    loads and stores flat element k, so both the loads and the LDS stores are unit-stride and
+//    free of bank conflicts; the face reads then stride by 3 words, which is conflict-free too)
+//    with a one-plane register prefetch; the four corner contributions of each node are
+//    gathered from LDS in a fixed order, so results are bitwise reproducible.
 //  * The epilogue fuses the constrained operator (M K M + I - M), the residual, a damped
 //    Jacobi sweep, a Chebyshev step or a p.Ap partial sum into the single pass over y.
 //  * Workgroup ids are remapped so that each XCD (8 per MI355X, own L2) receives a contiguous
 //    range of tiles with the x chunk slowest: halo rows, columns and planes shared by
 //    neighbouring tiles are then mostly read from the same L2.
+//
+// This file is compiled twice, through matvec_f64.hip (VH_MATVEC_F64: the FP64 instantiations)
+// and matvec_f32.hip (VH_MATVEC_F32: FP32 with FP32 or BF16 storage), so the two halves build
+// in parallel.
 #include "common.h"
 #include "matvec.h"
 
@@ -103,9 +109,10 @@ template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bo
 __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
   constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
-  __shared__ T su[2][3][PLANE];       // staged node planes ix & 1 (input), SoA
+  __shared__ T su[2][3 * PLANE];      // staged node planes ix & 1 (input), AoS rows as in HBM
   __shared__ T sc[4][3][PLANE];       // corner contributions, indexed by node
   __shared__ T se[PLANE];             // per element column: E(ex-1) + E(ex), for D^-1 on the fly
+  __shared__ T sE[BLOCK];             // this thread's prefetched modulus, parked across the barrier
   __shared__ double red[BLOCK / 64];
   constexpr bool NEEDS_DINV = MODE == MODE_JACOBI || MODE == MODE_CHEB || MODE == MODE_CHEB3;
   // FP64 smoothing passes recompute the left face from the still-staged plane instead of
@@ -153,7 +160,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
 
   // ---- loader geometry (plane independent): this thread's LOADS slots of the flat plane
   // image (NY rows of 3*NZ contiguous values), their in-plane DOF offsets and validity.
-  int ld_off[LOADS], ld_lds[LOADS];
+  int ld_off[LOADS];
   unsigned ld_okbits = 0;
 #pragma unroll
   for (int j = 0; j < LOADS; ++j) {
@@ -165,7 +172,6 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     const bool ok = k < NLOAD && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
     ld_okbits |= unsigned(ok) << j;
     ld_off[j] = ok ? 3 * (iy * nz1 + zbase) + col : 0;
-    ld_lds[j] = (col - 3 * (col / 3)) * PLANE + row * NZ + col / 3;
   }
   const int e_off = col_ok ? ey * nz + ez : 0;
   const size_t pstride = size_t(3) * ny1 * nz1;     // DOFs per node plane
@@ -201,13 +207,18 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     pre_E = E[(pre_E_ok ? size_t(ex) * estride : 0) + e_off];
   };
   auto commit = [&](int ix) {
-    T* flat = &su[ix & 1][0][0];
+    // the prefetched modulus goes through LDS like the plane: a store before the barrier cannot
+    // be sunk below the next prefetch, so its register is free to receive the next load (kept in
+    // a register, the compiler may keep it live past the next load and then copy it, waiting for
+    // every load in flight)
+    sE[tid] = pre_E_ok ? pre_E : T(0);
+    T* flat = &su[ix & 1][0];
 #pragma unroll
     for (int j = 0; j < LOADS; ++j) {
       bool keep = (pre_keep >> j) & 1u;
       if (MASK == MASK_ARRAY) keep = keep && pre_m[j] != 0;
       const T v = COMBINE ? pre[j] + sw * pre_w[j] : pre[j];
-      if (tid + j * BLOCK < NLOAD) flat[ld_lds[j]] = keep ? v : T(0);
+      if (tid + j * BLOCK < NLOAD) flat[tid + j * BLOCK] = keep ? v : T(0);
     }
   };
   // 2x2 (y,z) transform of this column's face of node plane `plane`: F[q], q = py + 2 pz.
@@ -217,10 +228,10 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     T w[3][4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      w[c][0] = su[pb][c][i00];
-      w[c][1] = su[pb][c][i00 + 1];
-      w[c][2] = su[pb][c][i00 + NZ];
-      w[c][3] = su[pb][c][i00 + NZ + 1];
+      w[c][0] = su[pb][3 * i00 + c];
+      w[c][1] = su[pb][3 * (i00 + 1) + c];
+      w[c][2] = su[pb][3 * (i00 + NZ) + c];
+      w[c][3] = su[pb][3 * (i00 + NZ + 1) + c];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -248,11 +259,12 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   double dot_acc = 0.0;
   T Eprev = T(0);                  // modulus of this column in element plane ex - 1
 
-  // ---- epilogue operands of this thread's owned node in node plane ex, issued at the start of
-  // step ex (they land during the element compute): b, the previous iterate / d, dinv; the
-  // input u at the node is read from the staged plane (still in LDS: planes are double-buffered)
+  // ---- epilogue operands of this thread's owned node: b, the previous iterate / d, dinv (and u
+  // itself with a mask array); otherwise the input u at the node is read from the staged plane
+  // (still in LDS: planes are double-buffered)
   constexpr bool NEED_B = MODE == MODE_RESID || NEEDS_DINV;
   constexpr bool NEED_U = MODE != MODE_RAW || MASK != MASK_NONE || COMBINE;
+  // (issued at the start of step ex, they land during the element compute)
   T ep_b[3], ep_p[3], ep_di[3], ep_u[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = T(0);
@@ -286,10 +298,13 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   // next step's commit, and the gather before the next step's contribution writes.
   for (int ex = ox0 - 1; ex < ox1; ++ex) {
     commit(ex + 1);
-    const T Ee = pre_E_ok ? pre_E : T(0);
     __syncthreads();
-    prefetch(ex + 2);
+    const T Ee = sE[tid];
+    // this step's epilogue operands first: loads complete in issue order, so the gather's wait
+    // for them then leaves the prefetch of plane ex + 2 in flight
     fetch_epilogue(ex);
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch(ex + 2);
     if (RECOMPUTE_FL) face(Fl, ex);
     face(Fr, ex + 1);
 
@@ -333,7 +348,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
     // step overwrite this plane buffer
     if (NEED_U && MASK != MASK_ARRAY && finalize && node_own) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ep_u[c] = su[ex & 1][c][jn_own];
+      for (int c = 0; c < 3; ++c) ep_u[c] = su[ex & 1][3 * jn_own + c];
     }
     __syncthreads();
     if (finalize && node_own) {
@@ -488,16 +503,20 @@ static MatvecPlan plan_typed(int nx, int ny, int nz, int lx_hint, int tile_hint)
   return (b.cost < a.cost) ? b : a;
 }
 
-MatvecPlan matvec_plan(int dtype, int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint) {
-  if (dtype == F64)
-    return cube ? plan_typed<double, double, true>(nx, ny, nz, lx_hint, tile_hint)
-                : plan_typed<double, double, false>(nx, ny, nz, lx_hint, tile_hint);
+#if defined(VH_MATVEC_F64)
+MatvecPlan matvec_plan_f64(bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint) {
+  return cube ? plan_typed<double, double, true>(nx, ny, nz, lx_hint, tile_hint)
+              : plan_typed<double, double, false>(nx, ny, nz, lx_hint, tile_hint);
+}
+#elif defined(VH_MATVEC_F32)
+MatvecPlan matvec_plan_f32(int storage, bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint) {
   if (storage == BF16)
     return cube ? plan_typed<float, uint16_t, true>(nx, ny, nz, lx_hint, tile_hint)
                 : plan_typed<float, uint16_t, false>(nx, ny, nz, lx_hint, tile_hint);
   return cube ? plan_typed<float, float, true>(nx, ny, nz, lx_hint, tile_hint)
               : plan_typed<float, float, false>(nx, ny, nz, lx_hint, tile_hint);
 }
+#endif
 
 template <typename T, typename S, int MODE, int MASK>
 static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
@@ -563,7 +582,10 @@ int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipSt
   return hip_status(hipGetLastError(), "parity_xmarch launch");
 }
 
+#if defined(VH_MATVEC_F64)
 template int matvec_launch<double>(MatvecParams<double>, int, int, int, hipStream_t);
+#elif defined(VH_MATVEC_F32)
 template int matvec_launch<float>(MatvecParams<float>, int, int, int, hipStream_t);
+#endif
 
 }  // namespace vh

@@ -163,16 +163,34 @@ class Level:
         return ops.axpby(1.0, u, 1.0, y, w=self.fixed)
 
     # fused passes used by the V-cycle and the Krylov loops
-    def tiles(self):
-        """Number of per-tile partial sums a DOT-mode pass of this level writes."""
-        return ops.matvec_tiles(self.shape, dtype=self.dtype, blocks=self.product.blocks,
+    def _plan(self, mode, combine):
+        return getattr(self, "launch", {}).get((mode, bool(combine)))
+
+    def tiles(self, mode=L.MODE_DOT, combine=False):
+        """Number of per-tile partial sums a DOT (or RESID) pass of this level writes, under the
+        launch plan that pass uses (tuned or automatic)."""
+        t, lx = self._plan(mode, combine) or (0, 0)
+        return ops.matvec_tiles(self.shape, lx=lx, tile=t, dtype=self.dtype, blocks=self.product.blocks,
                                 storage=self.product.storage)[0]
 
     def matvec(self, u, y, **kw):
         if not self.fused:
             raise TypeError("fused passes need a HipProduct")
+        if "tile" not in kw and "lx" not in kw:
+            plan = self._plan(kw.get("mode", L.MODE_RAW), kw.get("w") is not None)
+            if plan is not None:
+                kw["tile"], kw["lx"] = plan
         return ops.matvec(self.product.blocks, self.E, u, y, self.shape, mask=self.maskobj,
                           storage=self.product.storage, **kw)
+
+    def tune(self, tuner, passes):
+        """Empirical launch plans for the given (mode, combine) pass kinds (voxelhip.tuner)."""
+        if not self.fused:
+            return
+        self.launch = getattr(self, "launch", {})
+        for mode, combine in passes:
+            dinv_array = mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3) and self.product.kdiag is None
+            self.launch[(mode, bool(combine))] = tuner.tune(self, mode, combine, dinv_array)
 
 
 def _cheb_params(lam, lower_frac):
@@ -187,7 +205,7 @@ class Multigrid:
 
     def __init__(self, xp, shapes, products, E_fine, mask_fine, coarse_factor, dtype,
                  pre=4, post=4, omegas=None, smoother="jacobi", lambdas=None,
-                 lower_frac=1.0 / 30.0, cheb_variant="saad"):
+                 lower_frac=1.0 / 30.0, cheb_variant="saad", coarse_dtype=None):
         self.xp, self.pre, self.post = xp, pre, post
         self.dtype = _tdt(dtype)
         dev = voxelhip.device()
@@ -209,10 +227,15 @@ class Multigrid:
             raise ValueError(cheb_variant)
         self.cheb_variant = cheb_variant
         self.dinv_on_the_fly = True
-        # coarsest: explicit inverse A^-1 = L^-T L^-1 of the constrained operator, in FP64
+        # coarsest: explicit inverse A^-1 = L^-T L^-1 of the constrained operator, formed in FP64
+        # and stored in the level precision or, with coarse_dtype=float64, in FP64 also inside an
+        # FP32 V-cycle (rows padded to 16-byte multiples for the vector loads of dense_mv)
+        self.coarse_dtype = self.dtype if coarse_dtype is None else _tdt(coarse_dtype)
+        if self.coarse_dtype not in (self.dtype, torch.float64):
+            raise ValueError("coarse_dtype must be the level precision or float64")
         Lf = coarse_factor if isinstance(coarse_factor, torch.Tensor) else torch.from_numpy(np.asarray(coarse_factor))
         Lf = Lf.to(dev, torch.float64)
-        self.coarse_inverse = torch.cholesky_inverse(Lf).to(self.dtype).contiguous()
+        self.coarse_inverse = ops.padded_matrix(torch.cholesky_inverse(Lf), self.coarse_dtype)
         cl = self.levels[-1]
         self._coarse_mask = ops.Mask(L.MASK_ARRAY, (cl.mask != 0).to(torch.uint8).contiguous())
         # work vectors per level: x, x2 (ping-pong), res, d (Chebyshev), rhs (levels > 0)
@@ -220,6 +243,12 @@ class Multigrid:
         for lv in self.levels:
             z = lambda: torch.zeros(lv.n, dtype=self.dtype, device=dev)  # noqa: E731
             self._work.append({"x": z(), "x2": z(), "res": z(), "d": z(), "rhs": z()})
+
+    def tune(self, tuner):
+        """Launch plans of every smoothing level's passes (smoother and residual)."""
+        sm = L.MODE_CHEB3 if self.smoother == "chebyshev" else L.MODE_JACOBI
+        for lv in self.levels[:-1]:
+            lv.tune(tuner, [(sm, False), (L.MODE_RESID, False)])
 
     # ---------------------------------------------------------------- smoothing
     def _dinv_kw(self, Lv):
@@ -265,7 +294,7 @@ class Multigrid:
 
     def coarse_solve(self, r, out=None):
         y = out if out is not None else torch.empty_like(r)
-        return ops.dense_mv(self.coarse_inverse, r, y, mask=self._coarse_mask)
+        return ops.dense_mv(self.coarse_inverse, r, y, mask=self._coarse_mask, n=r.numel())
 
     def vcycle(self, r, level=0):
         Lv, w = self.levels[level], self._work[level]

@@ -17,6 +17,7 @@ MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT, MODE_CHEB3 = 0, 1, 2, 3,
 RED_DOT, RED_DOT2, RED_DOTDIFF, RED_NORM2 = 0, 1, 2, 3
 OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ = 0, 1, 2, 3, 4
 S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
+TILE_AUTO, TILE_16x32, TILE_8x32 = 0, 1, 2
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -46,13 +47,13 @@ SIGNATURES = {
     "vh_restrict": [_i, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
     "vh_prolong": [_i, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp],
     "vh_coarsen_modulus": [_i, _vp, _vp, _i, _i, _i, _vp],
-    "vh_dense_mv": [_i, _i, _vp, _vp, _vp, _vp, _vp],
+    "vh_dense_mv": [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
     "vh_reduce_parts": [_sz],
     "vh_reduce": [_i, _i, _sz, _vp, _vp, _vp, _vp, _vp],
     "vh_finish": [_vp, _i, _i, _vp, _i, _i, _vp],
     "vh_cg_update": [_i, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "vh_jacobi_dot": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
-    "vh_cvt_dotdiff": [_i, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
+    "vh_cvt_dotdiff": [_i, _i, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "vh_xpby": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
     "vh_axpby": [_i, _sz, _d, _vp, _vp, _d, _vp, _vp],
     "vh_convert": [_i, _i, _sz, _vp, _vp, _d, _vp, _i, _i, _vp],

@@ -13,6 +13,11 @@ precision:
   "fp64"          FP64 everywhere (the reference's plain FP64 GMG-FCG path)
   "mixed"         FP64 outer flexible CG and true residual, FP32 V-cycle (matvec, smoothing,
                   transfers, coarsest solve in FP32), FP64 dot accumulation
+  "ir"            FP64 iterative refinement around an FP32 flexible CG with the FP32 V-cycle
+                  (the scheme of pcg_ir, experiments/session_a/donor/src/gpu_fem/simp_r2.py:298-385,
+                  with GMG in place of Jacobi): FP64 only for the outer residual b - A x, the
+                  correction update x += |r| d, the dot accumulation and the coarsest solve;
+                  the inner solve of A d = r / |r| stops at max(inner_tol, 0.5 tol / rel).
 """
 from __future__ import annotations
 
@@ -28,7 +33,7 @@ import voxel_mg as mg
 class GMGSolver:
     def __init__(self, shape, ke, mask, E=None, *, max_coarse_x=16, n_levels=None, smoother="jacobi",
                  degree=3, pre=4, post=4, precision="fp64", coarse="rediscretize", graph=False,
-                 lower_frac=1.0 / 30.0, omega_iterations=30):
+                 lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False):
         self.shape = tuple(int(s) for s in shape)
         self.ke = np.asarray(ke, dtype=np.float64)
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -37,10 +42,20 @@ class GMGSolver:
             shapes = shapes[:n_levels]
         self.shapes = shapes
         self.smoother, self.degree, self.pre, self.post = smoother, degree, pre, post
-        if precision not in ("fp64", "mixed"):
+        if precision not in ("fp64", "mixed", "ir"):
             raise ValueError(precision)
         self.precision = precision
         self.pdtype = torch.float64 if precision == "fp64" else torch.float32
+        self.inner_tol = inner_tol
+        # the FP64 coarsest solve inside the FP32 V-cycle of the refinement scheme
+        self.coarse_dtype = torch.float64 if (precision == "ir" and coarse_fp64) else None
+        self.last_outer = 0
+        # launch tuning of every product pass (voxelhip.tuner): False, True (a per-process
+        # LaunchTuner) or a LaunchTuner instance
+        if tune is True:
+            from .tuner import LaunchTuner
+            tune = LaunchTuner()
+        self.tuner = tune or None
         self.coarse = coarse
         self.graph = graph
         self.lower_frac = lower_frac
@@ -78,10 +93,16 @@ class GMGSolver:
             kw = dict(smoother="chebyshev", pre=self.degree, post=self.degree, lower_frac=self.lower_frac)
         else:
             kw = dict(pre=self.pre, post=self.post)
-        self.M = mg.Multigrid(None, self.shapes, prods, E64, self.mask, Lc, self.pdtype, **kw)
+        self.M = mg.Multigrid(None, self.shapes, prods, E64, self.mask, Lc, self.pdtype,
+                              coarse_dtype=self.coarse_dtype, **kw)
         om, lam = mg.spectral_omegas(None, self.M.levels, iterations=self.omega_iterations)
         self.M.omegas = om
         self.M.lambdas = lam
+        if self.tuner is not None:
+            self.M.tune(self.tuner)
+            self.fine.tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True), (L.MODE_RESID, False)])
+            if self.precision == "ir":
+                self.M.levels[0].tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True)])
         self._alloc()
         self._graphs = None
 
@@ -112,16 +133,30 @@ class GMGSolver:
 
     def _alloc(self):
         n = self.fine.n
-        z = lambda: torch.zeros(n, dtype=torch.float64, device=self.dev)  # noqa: E731
+        z = lambda dt=torch.float64: torch.zeros(n, dtype=dt, device=self.dev)  # noqa: E731
         self.b, self.r, self.q = z(), z(), z()
-        self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
         self.parts = ops.reduce_parts(n)
-        self.tiles = self.fine.tiles()
-        self.partial = torch.zeros(2 * max(self.parts, self.tiles), dtype=torch.float64, device=self.dev)
+        # per-tile partial counts of the fused passes (they depend on each pass's launch plan)
+        f, f32 = self.fine, self.M.levels[0]
+        self.t_dot, self.t_res = f.tiles(L.MODE_DOT, True), f.tiles(L.MODE_RESID, True)
+        self.t_res0 = f.tiles(L.MODE_RESID, False)
+        if self.precision == "ir":
+            self.t_dot32, self.t_res32 = f32.tiles(L.MODE_DOT, True), f32.tiles(L.MODE_RESID, True)
+        else:
+            self.t_dot32 = self.t_res32 = 0
+        self.partial = torch.zeros(2 * max(self.parts, self.t_dot, self.t_res, self.t_res0, self.t_dot32,
+                                           self.t_res32), dtype=torch.float64, device=self.dev)
         self.scal = torch.zeros(16, dtype=torch.float64, device=self.dev)
         # right-hand side of the finest V-cycle level: the FP32 copy of r written by the
-        # residual pass (mixed), or r itself (FP64)
+        # residual pass (mixed), the inner residual (ir), or r itself (FP64)
         self.rhs0 = self.M._work[0]["rhs"] if self.pdtype != torch.float64 else self.r
+        if self.precision == "ir":
+            f32 = torch.float32
+            self.x = z()
+            self.b32, self.q32 = z(f32), z(f32)
+            self.ds, self.ps, self.zs = [z(f32), z(f32)], [z(f32), z(f32)], [z(f32), z(f32)]
+        else:
+            self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
 
     # ------------------------------------------------------------------ one iteration
     def _iteration(self, i):
@@ -135,27 +170,42 @@ class GMGSolver:
         f, L_ = self.fine, L
         f.matvec(self.zs[i], self.q, mode=L_.MODE_DOT, partial=self.partial,
                  w=self.ps[o], sdev=self.scal, sidx=L_.S_BETA, u_out=self.ps[i])
-        ops.finish(self.partial, self.tiles, 1, self.scal, L_.S_PQ, L_.OP_ALPHA)
+        ops.finish(self.partial, self.t_dot, 1, self.scal, L_.S_PQ, L_.OP_ALPHA)
         f.matvec(self.xs[o], self.r, mode=L_.MODE_RESID, b=self.b, partial=self.partial,
                  w=self.ps[i], sdev=self.scal, sidx=L_.S_ALPHA, u_out=self.xs[i],
                  y32=self.rhs0 if self.pdtype != torch.float64 else None)
-        ops.finish(self.partial, self.tiles, 1, self.scal, 8)
+        ops.finish(self.partial, self.t_res, 1, self.scal, 8)
         zl = self.M.vcycle(self.rhs0)
         ops.cvt_dotdiff(zl, self.r, self.zs[i], self.zs[o], self.partial)
         ops.finish(self.partial, self.parts, 2, self.scal, 5, L_.OP_BETA_PR)
 
-    def _capture(self):
+    def _inner_iteration(self, i):
+        """FP32 flexible-CG iteration of the refinement's inner solve A d = b32 (parity i);
+        the same pass structure as _iteration with FP32 vectors and FP64 scalars."""
+        o = 1 - i
+        f = self.M.levels[0]
+        f.matvec(self.zs[i], self.q32, mode=L.MODE_DOT, partial=self.partial,
+                 w=self.ps[o], sdev=self.scal, sidx=L.S_BETA, u_out=self.ps[i])
+        ops.finish(self.partial, self.t_dot32, 1, self.scal, L.S_PQ, L.OP_ALPHA)
+        f.matvec(self.ds[o], self.rhs0, mode=L.MODE_RESID, b=self.b32, partial=self.partial,
+                 w=self.ps[i], sdev=self.scal, sidx=L.S_ALPHA, u_out=self.ds[i])
+        ops.finish(self.partial, self.t_res32, 1, self.scal, 8)
+        zl = self.M.vcycle(self.rhs0)
+        ops.cvt_dotdiff(zl, self.rhs0, self.zs[i], self.zs[o], self.partial)
+        ops.finish(self.partial, self.parts, 2, self.scal, 5, L.OP_BETA_PR)
+
+    def _capture(self, step):
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):          # warm-up on a side stream (state is reset afterwards)
-            self._iteration(0)
-            self._iteration(1)
+            step(0)
+            step(1)
         torch.cuda.current_stream().wait_stream(s)
         graphs = []
         for i in (0, 1):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._iteration(i)
+                step(i)
             graphs.append(g)
         return graphs
 
@@ -163,9 +213,13 @@ class GMGSolver:
     def solve(self, b, tol=1e-8, maxiter=200):
         """Flexible CG (the true residual is formed and tested every iteration, as the reference
         harness runs fcg with check_every=1) to ||b - A x|| / ||b|| <= tol from x = 0.
-        Returns (x, iterations, converged, rel, matvecs); x is a view of the solver's buffer."""
+        Returns (x, iterations, converged, rel, matvecs); x is a view of the solver's buffer.
+        With precision "ir", iterations counts the inner FP32 iterations (self.last_outer the
+        outer FP64 corrections)."""
+        if self.precision == "ir":
+            return self._solve_ir(b, tol, maxiter)
         if self.graph and self._graphs is None:
-            self._graphs = self._capture()
+            self._graphs = self._capture(self._iteration)
         self.b.copy_(b)
         ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
         ops.finish(self.partial, self.parts, 1, self.scal, 9)
@@ -194,6 +248,66 @@ class GMGSolver:
             if rel <= tol:
                 return self.xs[i], it, True, rel, matvecs
         return self.xs[(it - 1) % 2], it, bool(rel <= tol), rel, matvecs
+
+    def _outer_residual(self, nb):
+        self.fine.matvec(self.x, self.r, mode=L.MODE_RESID, b=self.b, partial=self.partial)
+        ops.finish(self.partial, self.t_res0, 1, self.scal, 8)
+        return float(self.scal[8].sqrt()) / nb
+
+    def _solve_ir(self, b, tol, maxiter):
+        if self.graph and self._graphs is None:
+            self._graphs = self._capture(self._inner_iteration)
+        self.b.copy_(b)
+        ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
+        ops.finish(self.partial, self.parts, 1, self.scal, 9)
+        nb = float(self.scal[9].sqrt())
+        self.x.zero_()
+        self.r.copy_(self.b)
+        rel = 1.0 if nb > 0 else 0.0
+        inner_total, matvecs, outer = 0, 0, 0
+        tol_floor = self.inner_tol
+        rel_prev, last_scale, last_d = None, 0.0, None
+        while rel > tol and inner_total < maxiter:
+            if rel_prev is not None and rel > rel_prev and last_d is not None:
+                ops.convert(last_d, self.x, scale=-last_scale, accumulate=True)    # undo the step
+                rel, last_d = rel_prev, None
+                tol_floor *= 0.1
+                if tol_floor < 1e-7:
+                    break
+                self._outer_residual(nb)
+                matvecs += 1
+                continue
+            rel_prev = rel
+            rnorm = rel * nb
+            tau = max(tol_floor, 0.5 * tol / rel)
+            # inner: A d = r / |r| in FP32 from d = 0
+            ops.convert(self.r, self.b32, scale=1.0 / rnorm)
+            self.ds[1].zero_()
+            self.ps[1].zero_()
+            self.rhs0.copy_(self.b32)
+            zl = self.M.vcycle(self.rhs0)
+            self.zs[0].copy_(zl)
+            ops.reduce(L.RED_DOT, self.rhs0, self.zs[0], partial=self.partial)
+            ops.finish(self.partial, self.parts, 1, self.scal, L.S_RZ)
+            self.scal[L.S_BETA] = 0.0
+            it = 0
+            for it in range(1, maxiter - inner_total + 1):
+                i = (it - 1) % 2
+                if self._graphs is not None:
+                    self._graphs[i].replay()
+                else:
+                    self._inner_iteration(i)
+                if float(self.scal[8].sqrt()) <= tau:
+                    break
+            inner_total += it
+            matvecs += 2 * it
+            last_d, last_scale = self.ds[(it - 1) % 2], rnorm
+            ops.convert(last_d, self.x, scale=rnorm, accumulate=True)            # x += |r| d
+            outer += 1
+            rel = self._outer_residual(nb)
+            matvecs += 1
+        self.last_outer = outer
+        return self.x, inner_total, bool(rel <= tol), rel, matvecs
 
 
 def _prolongation_matrix(coarse):

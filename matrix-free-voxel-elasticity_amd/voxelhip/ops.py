@@ -207,13 +207,28 @@ def coarsen_modulus(Ef, Ec, fine_shape):
     return Ec
 
 
-def dense_mv(A, r, y, mask=None):
+def dense_mv(A, r, y, mask=None, n=None):
+    """y = mask * (A[:n, :n] r): A is a 2-D device tensor whose row stride is its leading
+    dimension (a multiple of 4; see padded_matrix), FP64 or FP32; r, y FP32 or of A's type."""
     _dev(A, "A"); _dev(r, "r"); _dev(y, "y")
+    n = int(r.numel()) if n is None else int(n)
+    if A.dim() != 2 or A.stride(1) != 1 or A.shape[0] < n or A.shape[1] < n:
+        raise ValueError("dense_mv: A must be a row-major 2-D tensor of at least n x n")
     m = mask.array if (mask is not None and mask.kind == L.MASK_ARRAY) else None
     if mask is not None and mask.kind == L.MASK_X0:
         raise ValueError("dense_mv takes an explicit mask array")
-    L.call("vh_dense_mv", dtype_code(y), int(r.numel()), _p(A), _p(r), _p(y), _p(m), _stream())
+    L.call("vh_dense_mv", dtype_code(A), dtype_code(y), n, int(A.stride(0)), _p(A), _p(r), _p(y), _p(m), _stream())
     return y
+
+
+def padded_matrix(A, dtype=None):
+    """Copy of the square matrix A with its rows padded to a multiple of 4 entries (16-byte
+    aligned rows for the vector loads of dense_mv); the padding is zero."""
+    n = A.shape[0]
+    ld = (n + 3) // 4 * 4
+    out = torch.zeros((n, ld), dtype=dtype or A.dtype, device=A.device)
+    out[:, :n] = A
+    return out
 
 
 # --------------------------------------------------------------------------- BLAS-1
@@ -242,8 +257,8 @@ def jacobi_dot(r, dinv, z, partial):
 
 def cvt_dotdiff(zin, r, zold, znew, partial):
     _dev(zin, "zin"); _dev(r, "r"); _dev(zold, "zold"); _dev(znew, "znew")
-    L.call("vh_cvt_dotdiff", dtype_code(zin), int(r.numel()), _p(zin), _p(r), _p(zold), _p(znew), _p(partial),
-           _stream())
+    L.call("vh_cvt_dotdiff", dtype_code(zin), dtype_code(znew), int(r.numel()), _p(zin), _p(r), _p(zold), _p(znew),
+           _p(partial), _stream())
 
 
 def xpby(z, p, scal, zcopy=None):

@@ -63,12 +63,13 @@ def test_graph_replay_equals_eager(problem):
     assert torch.equal(sols[0], sols[1])
 
 
-@pytest.mark.parametrize("precision", ["mixed"])
+@pytest.mark.parametrize("precision", ["mixed", "ir"])
 def test_mixed_precision_reaches_fp64_tolerance(problem, precision):
     from voxelhip.engine import GMGSolver
     shape, E, mask, force, xo, ito = problem
     s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, smoother="chebyshev", degree=3,
                   precision=precision, graph=True)
+    print("case", precision, flush=True)
     x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
     assert ok and r <= 1e-8, (it, r)
     assert rel(x, xo) < 1e-7
@@ -77,6 +78,23 @@ def test_mixed_precision_reaches_fp64_tolerance(problem, precision):
     xn = x.cpu().numpy()
     res = force - (fine.raw(xn * mask, E) * mask + xn * (1 - mask))
     assert np.linalg.norm(res) / np.linalg.norm(force) <= 1.01e-8
+
+
+def test_refinement_graph_replay_equals_eager(problem):
+    """FP64 refinement around the FP32 flexible CG: replayed inner iterations give the same
+    iterate as eager launches, over repeated solves."""
+    from voxelhip.engine import GMGSolver
+    shape, E, mask, force, _, _ = problem
+    sols = []
+    for graph in (False, True):
+        s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, smoother="chebyshev",
+                      degree=3, precision="ir", graph=graph)
+        for _ in range(3):
+            x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
+            x = x.clone()
+        assert ok and s.last_outer >= 2
+        sols.append(x)
+    assert torch.equal(sols[0], sols[1])
 
 
 def test_jacobi_engine_matches_the_reference_vcycle_solver():

@@ -29,7 +29,7 @@ def KE(golden):
     return np.load(golden / "smoke-q4.npz")["ke"]
 
 
-def build(KE, shape, E, mask, dtype=np.float64, max_coarse_x=2, smoother="jacobi", degree=None):
+def build(KE, shape, E, mask, dtype=np.float64, max_coarse_x=2, smoother="jacobi", degree=None, coarse_dtype=None):
     import voxelhip
     import voxel_mg as mg
     xp = voxelhip.xp
@@ -42,7 +42,7 @@ def build(KE, shape, E, mask, dtype=np.float64, max_coarse_x=2, smoother="jacobi
     kw = {}
     if smoother == "chebyshev":
         kw = dict(smoother="chebyshev", pre=degree, post=degree)
-    M = mg.Multigrid(xp, shapes, products, E, mask, np.linalg.cholesky(A), dtype, **kw)
+    M = mg.Multigrid(xp, shapes, products, E, mask, np.linalg.cholesky(A), dtype, coarse_dtype=coarse_dtype, **kw)
     om, lam = mg.spectral_omegas(xp, M.levels)
     M.omegas = om
     if smoother == "chebyshev":
@@ -127,3 +127,56 @@ def test_chebyshev_vcycle_matches_the_oracle(golden, KE):
         M.cheb_variant = variant
         Mo.cheb_variant = variant
         assert rel(M(torch.from_numpy(r).cuda()), Mo(r)) < 1e-10
+
+
+@pytest.mark.parametrize("adt,vdt,n", [(torch.float64, torch.float64, 131), (torch.float64, torch.float32, 4131),
+                                       (torch.float32, torch.float32, 517), (torch.float64, torch.float64, 64)])
+def test_dense_mv_matches_torch(adt, vdt, n):
+    """Coarsest solve kernel: padded rows (the padding is never read: NaN there must not leak),
+    FP64 matrix with FP32 vectors, masked rows."""
+    from voxelhip import _lib as L
+    from voxelhip import ops
+    g = torch.Generator(device="cuda").manual_seed(n)
+    A = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+    r = torch.randn(n, generator=g, device="cuda", dtype=torch.float64).to(vdt)
+    m = torch.rand(n, generator=g, device="cuda") > 0.2
+    mask = ops.Mask(L.MASK_ARRAY, m.to(torch.uint8).contiguous())
+    Ap = ops.padded_matrix(A, adt)
+    Ap[:, n:] = float("nan")
+    y = torch.empty(n, dtype=vdt, device="cuda")
+    ops.dense_mv(Ap, r, y, mask=mask)
+    ref = (A.to(adt).double() @ r.double()) * m.double()
+    tol = 1e-13 if (adt, vdt) == (torch.float64, torch.float64) else (1e-6 if adt == torch.float64 else 1e-5)
+    assert float((y.double() - ref).norm() / ref.norm()) < tol
+    assert torch.all(y[~m] == 0)
+
+
+@pytest.mark.parametrize("zdt,vdt", [(torch.float32, torch.float32), (torch.float32, torch.float64),
+                                     (torch.float64, torch.float64)])
+def test_cvt_dotdiff_matches_torch(zdt, vdt):
+    from voxelhip import ops
+    n = 100003
+    g = torch.Generator(device="cuda").manual_seed(7)
+    zin, r, zold = (torch.randn(n, generator=g, device="cuda", dtype=torch.float64) for _ in range(3))
+    zin, r, zold = zin.to(zdt), r.to(vdt), zold.to(vdt)
+    znew = torch.empty(n, dtype=vdt, device="cuda")
+    parts = ops.reduce_parts(n)
+    partial = torch.zeros(2 * parts, dtype=torch.float64, device="cuda")
+    ops.cvt_dotdiff(zin, r, zold, znew, partial)
+    zc = zin.to(vdt).double()
+    assert torch.equal(znew, zin.to(vdt))
+    s0, s1 = float(partial[:parts].sum()), float(partial[parts:].sum())
+    assert s0 == pytest.approx(float(r.double() @ zc), rel=1e-12)
+    assert s1 == pytest.approx(float(r.double() @ (zc - zold.double())), rel=1e-12)
+
+
+def test_fp64_coarsest_inside_the_fp32_vcycle(golden, KE):
+    """coarse_dtype=float64: the FP32 V-cycle with its coarsest solve in FP64 (FP64 inverse, FP32
+    vectors) stays within FP32 rounding of the reference's FP32 and FP64 V-cycles."""
+    g = np.load(golden / "mg_golden.npz")
+    shape = tuple(int(s) for s in g["q8s3/shape"])
+    _, M, _ = build(KE, shape, g["q8s3/E"], g["q8s3/mask"], dtype=np.float32, coarse_dtype=np.float64)
+    assert M.coarse_inverse.dtype == torch.float64
+    r = torch.from_numpy(g["q8s3/vcycle_in"]).cuda()
+    out = M(r)
+    assert rel(out, g["q8s3/vcycle32_out"]) < 1e-5 and rel(out, g["q8s3/vcycle_out"]) < 1e-5
