@@ -65,6 +65,90 @@ def problem(shape, seed, rmin=1.5):
     return rho, E, f.reshape(-1).contiguous(), m.reshape(-1), ops
 
 
+def c4_problems(seed=4, n=48):
+    """Batched workload: n cantilever-type problems with grids drawn from the seed between
+    16 x 16 x 16 and 96 x 48 x 48 elements (every extent a multiple of 8), densities iid
+    U[0.01, 1] smoothed by the radius-1.5 cone filter, x = 0 face clamped, and 1-4 load cases of
+    one unit point load each at a random node of a random face other than x = 0."""
+    import torch
+    from voxelhip.batch import BatchProblem
+    from voxelhip.simp import ConeFilter, modulus
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda", 0)
+    out = []
+    for _ in range(n):
+        nx = 8 * int(rng.integers(2, 13))
+        ny = 8 * int(rng.integers(2, min(6, nx // 8) + 1))
+        nz = 8 * int(rng.integers(2, min(6, nx // 8) + 1))
+        shape = (nx, ny, nz)
+        rho = torch.from_numpy(rng.uniform(0.01, 1.0, nx * ny * nz)).to(dev)
+        E = modulus(ConeFilter(shape, 1.5, device=dev).forward(rho), 3.0, 1.0, 1e-9)
+        mask = np.ones((nx + 1, ny + 1, nz + 1, 3))
+        mask[0] = 0
+        loads = []
+        for _ in range(int(rng.integers(1, 5))):
+            f = np.zeros((nx + 1, ny + 1, nz + 1, 3))
+            face = int(rng.integers(0, 5))
+            ix, iy, iz = int(rng.integers(1, nx + 1)), int(rng.integers(0, ny + 1)), int(rng.integers(0, nz + 1))
+            ix = nx if face == 0 else ix
+            iy = 0 if face == 1 else (ny if face == 2 else iy)
+            iz = 0 if face == 3 else (nz if face == 4 else iz)
+            d = rng.normal(size=3)
+            f[ix, iy, iz] = d / np.linalg.norm(d)
+            loads.append(f.reshape(-1))
+        out.append(BatchProblem(shape, E, mask.reshape(-1), loads))
+    return out
+
+
+def timed_batch(problems, ke, steps, warmup, graph=True, sequential=True):
+    """The batched solver on all problems (one launch per pass) and, for comparison, the same
+    instances solved one after another with the single-problem engine."""
+    import torch
+    from voxelhip.batch import BatchedGMGSolver, coarse_dof_shapes
+    from voxelhip.engine import GMGSolver
+    t0 = time.perf_counter()
+    bs = BatchedGMGSolver(problems, ke, precision="mixed", graph=graph)
+    setup = time.perf_counter() - t0
+    for _ in range(warmup):
+        bs.solve(tol=1e-8, maxiter=200)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, its, ok, rel = bs.solve(tol=1e-8, maxiter=200)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    n_inst = bs.n
+    out = {"problems": len(problems), "instances": n_inst,
+           "dofs_total": int(sum(3 * (p.shape[0] + 1) * (p.shape[1] + 1) * (p.shape[2] + 1) * len(p.loads)
+                                 for p in problems)),
+           "grids": sorted({"x".join(map(str, p.shape)) for p in problems}),
+           "setup_s": setup, "seconds_per_batch": dt, "instance_solves_per_s": n_inst / dt,
+           "iterations_min_max": [int(its.min()), int(its.max())], "converged": bool(ok.all()),
+           "max_rel_residual": float(rel.max())}
+    del bs
+    if sequential:
+        solvers = []
+        for p in problems:
+            shapes = coarse_dof_shapes(p.shape)
+            solvers.append(GMGSolver(p.shape, ke, p.mask, p.E, smoother="chebyshev", degree=3, precision="mixed",
+                                     graph=graph, max_coarse_x=0, n_levels=len(shapes)))
+        loads = [[torch.from_numpy(f).cuda() for f in p.loads] for p in problems]
+        for s, fs in zip(solvers, loads):
+            for f in fs:
+                s.solve(f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for s, fs in zip(solvers, loads):
+                for f in fs:
+                    s.solve(f)
+        torch.cuda.synchronize()
+        seq = (time.perf_counter() - t0) / steps
+        out["sequential_seconds"] = seq
+        out["batched_speedup_vs_sequential"] = seq / dt
+    return out
+
+
 def timed_solves(solver, b, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -153,6 +237,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batched leg")
     ap.add_argument("--only", choices=["matvec"], default=None,
                     help="matvec: only the dominant kernel (FP64 product at 256x128x128) and a plain "
                          "copy of known size, for counter collection")
@@ -205,6 +290,9 @@ def main():
     out["c2_fp64"] = {"shape": list(shape2), "seconds_per_solve": dt2, "solves_per_s": 1 / dt2,
                       "iterations": sorted({x[0] for x in st2}), "converged": all(x[1] for x in st2)}
     del s2
+    if not args.no_batch:
+        out["c4_batched"] = timed_batch(c4_problems(), ke, max(1, args.steps // 2), 1, graph=bool(args.graph))
+        torch.cuda.empty_cache()
     mv = {f"{d}@{'x'.join(map(str, s))}": matvec_timing(s, getattr(torch, d))
           for s in ((256, 128, 128), (128, 64, 64), (384, 192, 192)) for d in ("float64", "float32")}
     dom = mv["float64@256x128x128"]

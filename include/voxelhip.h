@@ -29,8 +29,9 @@ enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3,
        VH_MODE_CHEB3 = 5 };
 enum { VH_RED_DOT = 0, VH_RED_DOT2 = 1, VH_RED_DOTDIFF = 2, VH_RED_NORM2 = 3 };
 enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4 };
-/* device scalar slots used by vh_finish / vh_cg_update / vh_xpby */
-enum { VH_S_RZ = 0, VH_S_PQ = 1, VH_S_ALPHA = 2, VH_S_BETA = 3, VH_S_FLAG = 4 };
+/* device scalar slots used by vh_finish / vh_cg_update / vh_xpby; VH_S_DONE != 0 freezes the
+ * recurrence (alpha = beta = 0), for converged problems of a batch */
+enum { VH_S_RZ = 0, VH_S_PQ = 1, VH_S_ALPHA = 2, VH_S_BETA = 3, VH_S_FLAG = 4, VH_S_DONE = 10 };
 
 int vh_abi_version(void);
 const char* vh_last_error(void);
@@ -177,6 +178,72 @@ int vh_cone_filter(int dtype, const void* x, const void* pre_div, const void* po
  * experiments/session_h/run_w1b.py:82-94 (with w = H(1/s)/n the filtered volume). */
 int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w, double lmid,
                 double move, void* xnew, double* partial, int nparts, void* stream);
+
+/* ------------------------------------------------------------------ batched problems
+ * Many voxel problems (grids of different sizes, several load cases each) run each pass of
+ * their solver in ONE launch.  A batch handle holds a descriptor table in device memory,
+ * built once (set-up; it allocates) and launched any number of times (also inside captured
+ * HIP graphs); vh_batch_destroy releases it.  These entry points have no single counterpart
+ * in the reference, which solves one problem per call (fcg src/voxel_mg.py:181-217 on one
+ * grid); they batch the per-problem operations declared above. */
+typedef struct vh_product_item {
+  const double* blocks72;       /* host, voxel-cube pattern required */
+  const void* E;
+  const void* u;
+  void* y;
+  int nx, ny, nz;
+  const uint8_t* mask;          /* VH_MASK_ARRAY only */
+  vh_epilogue ep;               /* mode, operands and scalars of this problem's pass; ep.lx hints x chunks */
+} vh_product_item;
+/* Products of all items in one launch (same dtype, mode, mask kind, tile and load-time
+ * combination).  DOT/RESID partials: every item passes the same ep.partial base; item i writes
+ * its tiles at [offs[i], offs[i+1]) of it (offs from vh_batch_offsets, for vh_seg_finish).
+ * tiles_out (n ints, optional) receives each item's tile count.  Modes RAW, RESID, DOT,
+ * JACOBI, CHEB3. */
+int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_product_item* items, void** handle,
+                            int* tiles_out);
+
+/* Segments of a concatenated vector (item i owns elements [sum_{j<i} len_j, ... + len_i)). */
+int vh_segments_create(int n, const long long* lengths, void** handle);
+/* Per-segment |x|^2 (VH_RED_NORM2) or x.y (VH_RED_DOT): one FP64 partial per workgroup. */
+int vh_seg_reduce(void* segments, int dtype, int kind, const void* x, const void* y, double* partial, void* stream);
+/* Per-segment cvt_dotdiff: znew = zin (converted), partials [r.znew | r.(znew - zold)]. */
+int vh_seg_dotdiff(void* segments, int dtype_in, int dtype_v, const void* zin, const void* r, const void* zold,
+                   void* znew, double* partial, void* stream);
+/* One workgroup per item: sums partial[j * total + offs[i] .. offs[i+1]) (j < count) into
+ * scal[i * stride + dst ..] and runs the scalar recurrence `op` on that item's scalar block.
+ * offs: a device prefix table from vh_batch_offsets (product tiles or segment blocks). */
+int vh_seg_finish(const double* partial, int total, const int* offs, int n, int count, double* scal, int stride,
+                  int dst, int op, void* stream);
+
+typedef struct vh_transfer_item {
+  const void* src;              /* restrict: fine residual; prolong: coarse correction */
+  void* dst;                    /* restrict: coarse rhs; prolong: fine iterate */
+  int nx, ny, nz;               /* FINE element grid (even extents) */
+  int fmask_kind, cmask_kind;
+  const uint8_t* fmask;
+  const uint8_t* cmask;
+} vh_transfer_item;
+/* Restriction (restrict_op = 1) or prolongation (0; assign = 1 overwrites, else adds) of all
+ * items in one launch, as vh_restrict / vh_prolong. */
+int vh_transfer_batch_create(int dtype, int restrict_op, int assign, int n, const vh_transfer_item* items,
+                             void** handle);
+
+typedef struct vh_dense_item {
+  const void* A;                /* row-major, leading dimension ld (ld % 4 == 0, 16-byte aligned) */
+  int ld, n;
+  const void* r;
+  void* y;
+  const uint8_t* mask;
+} vh_dense_item;
+/* The coarsest solves y = mask * (A r) of all items in one launch, as vh_dense_mv. */
+int vh_dense_batch_create(int a_dtype, int v_dtype, int n, const vh_dense_item* items, void** handle);
+
+/* Launch a product, transfer or dense batch on `stream`. */
+int vh_batch_launch(void* handle, void* stream);
+/* Device prefix table (n + 1 ints: tiles, segment blocks, lines or row blocks) of a handle. */
+int vh_batch_offsets(void* handle, const int** offs, int* n, int* total);
+int vh_batch_destroy(void* handle);
 
 #ifdef __cplusplus
 }

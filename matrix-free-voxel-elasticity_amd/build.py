@@ -17,8 +17,10 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "voxelhip" / "libvoxelhip.so"
-# matvec.hip is compiled through matvec_f64.hip and matvec_f32.hip (one translation unit each)
-SOURCES = ["capi.hip", "matvec_f64.hip", "matvec_f32.hip", "grid_ops.hip", "blas.hip", "simp.hip"]
+# matvec.hip is compiled through matvec_{f64,f32}.hip and matvec_batch_{f64,f32}.hip (one
+# translation unit each, built in parallel)
+SOURCES = ["capi.hip", "matvec_f64.hip", "matvec_f32.hip", "matvec_batch_f64.hip", "matvec_batch_f32.hip",
+           "grid_ops.hip", "blas.hip", "simp.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VOXELHIP_ARCH", "gfx950")
 

@@ -8,8 +8,44 @@ constexpr int BLAS_MAX_GRID = 2048;     // 8 blocks per CU on 256 CUs; partial a
 
 enum RedKind : int { RED_DOT = 0, RED_DOT2 = 1, RED_DOTDIFF = 2, RED_NORM2 = 3 };
 enum ScalarOp : int { OP_NONE = 0, OP_ALPHA = 1, OP_BETA_FR = 2, OP_BETA_PR = 3, OP_SET_RZ = 4 };
-// Device scalar slots (FP64).  Slots >= 8 are free for callers (norms, dots).
-enum Slot : int { S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_BETA = 3, S_FLAG = 4, S_RZNEW = 5, S_NUM = 6, S_NORM2 = 7 };
+// Device scalar slots (FP64).  Slots 8, 9, 11.. are free for callers (norms, dots); S_DONE != 0
+// freezes a Krylov recurrence (alpha = beta = 0: the iterate stops moving), which the batched
+// solvers set for problems that have converged while others iterate on.
+enum Slot : int { S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_BETA = 3, S_FLAG = 4, S_RZNEW = 5, S_NUM = 6, S_NORM2 = 7,
+                  S_DONE = 10 };
+
+// Stores the `count` reduced values v into s[dst..] and evaluates the scalar recurrence `op`.
+__device__ inline void apply_scalar_op(double* s, const double* v, int count, int dst, int op) {
+  for (int j = 0; j < count; ++j) s[dst + j] = v[j];
+  const bool done = s[S_DONE] != 0.0;
+  switch (op) {
+    case OP_ALPHA: {         // alpha = rz / pq  (pq just written to s[dst])
+      const double pq = v[0];
+      if (done) s[S_ALPHA] = 0.0;
+      else if (pq == 0.0 || !isfinite(pq)) { s[S_ALPHA] = 0.0; s[S_FLAG] = 1.0; }
+      else s[S_ALPHA] = s[S_RZ] / pq;
+      break;
+    }
+    case OP_BETA_FR: {       // Fletcher-Reeves: beta = rz_new / rz ; rz = rz_new
+      const double rzn = v[0];
+      const double rz = s[S_RZ];
+      s[S_BETA] = (rz == 0.0 || done) ? 0.0 : rzn / rz;
+      if (rz == 0.0) s[S_FLAG] = 1.0;
+      s[S_RZ] = rzn;
+      break;
+    }
+    case OP_BETA_PR: {       // Polak-Ribiere: beta = r_new.(z_new - z) / rz ; rz = r_new.z_new
+      const double rzn = v[0], num = v[1];
+      const double rz = s[S_RZ];
+      s[S_BETA] = (rz == 0.0 || done) ? 0.0 : num / rz;
+      if (rz == 0.0) s[S_FLAG] = 1.0;
+      s[S_RZ] = rzn;
+      break;
+    }
+    case OP_SET_RZ: s[S_RZ] = v[0]; break;
+    default: break;
+  }
+}
 
 int blas_grid(size_t n);
 template <typename T> int reduce_launch(int kind, size_t n, const T* x, const T* y, const T* w, double* partial, hipStream_t st);

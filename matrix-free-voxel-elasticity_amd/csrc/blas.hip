@@ -58,34 +58,7 @@ __global__ void __launch_bounds__(256) finish_kernel(const double* __restrict__ 
   if (threadIdx.x != 0) return;
   double v[2];
   for (int j = 0; j < 2; ++j) v[j] = red[j][0] + red[j][1] + red[j][2] + red[j][3];
-  for (int j = 0; j < count; ++j) s[dst + j] = v[j];
-  // scalar recurrences (slot names in blas.h)
-  switch (op) {
-    case OP_ALPHA: {         // alpha = rz / pq  (pq just written to s[dst])
-      const double pq = v[0];
-      if (pq == 0.0 || !isfinite(pq)) { s[S_ALPHA] = 0.0; s[S_FLAG] = 1.0; }
-      else s[S_ALPHA] = s[S_RZ] / pq;
-      break;
-    }
-    case OP_BETA_FR: {       // Fletcher-Reeves: beta = rz_new / rz ; rz = rz_new
-      const double rzn = v[0];
-      const double rz = s[S_RZ];
-      s[S_BETA] = (rz == 0.0) ? 0.0 : rzn / rz;
-      if (rz == 0.0) s[S_FLAG] = 1.0;
-      s[S_RZ] = rzn;
-      break;
-    }
-    case OP_BETA_PR: {       // Polak-Ribiere: beta = r_new.(z_new - z) / rz ; rz = r_new.z_new
-      const double rzn = v[0], num = v[1];
-      const double rz = s[S_RZ];
-      s[S_BETA] = (rz == 0.0) ? 0.0 : num / rz;
-      if (rz == 0.0) s[S_FLAG] = 1.0;
-      s[S_RZ] = rzn;
-      break;
-    }
-    case OP_SET_RZ: s[S_RZ] = v[0]; break;
-    default: break;
-  }
+  apply_scalar_op(s, v, count, dst, op);
 }
 
 // --------------------------------------------------------------------------- updates

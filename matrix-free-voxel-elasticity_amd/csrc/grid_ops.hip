@@ -149,57 +149,14 @@ __global__ void __launch_bounds__(256) dense_node_kernel(const T* __restrict__ u
 }
 
 // ------------------------------------------------------------------ transfers
-// Trilinear weights along one axis: fine index i against coarse index I: w = 1 if i == 2I,
-// 1/2 if |i - 2I| == 1 (src/voxel_mg.py:74-105: P copies even nodes, averages odd ones; R = P^T).
-//
-// restrict: rc[I] = mask_c[I] * sum_{a,b,c in -1..1} w(a) w(b) w(c) (rf * mask_f)[2I + (a,b,c)]
-//
-// Both kernels run one workgroup per z-line of nodes (all iz, 3 components: a contiguous run of
-// 3 (nz + 1) values in the author layout), so every global access is a coalesced sweep along a
-// line.  restrict: the x/y weights combine up to 9 fine lines into one weighted fine line in
-// LDS, then each coarse DOF takes its 3 z neighbours from LDS.  prolong: up to 4 coarse lines
-// are combined with the x/y weights into LDS, then each fine DOF interpolates along z.
+// One workgroup per z-line (restrict_line / prolong_line in grid_ops.h).
 template <typename T>
 __global__ void __launch_bounds__(256) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
                                                       const GridArgs<T> g, const uint8_t* __restrict__ cmask,
                                                       int cmask_kind) {
   extern __shared__ unsigned char smem_raw[];
-  T* line = reinterpret_cast<T*>(smem_raw);             // 3 (nz + 1) weighted fine values
-  // g describes the FINE grid; coarse extents are half.
-  const int cy1 = g.ny / 2 + 1, cz1 = g.nz / 2 + 1;
-  const int fy1 = g.ny + 1, nlf = 3 * (g.nz + 1), nlc = 3 * cz1;
-  const int Iy = int(blockIdx.x % unsigned(cy1)), Ix = int(blockIdx.x / unsigned(cy1));
-  for (int t = threadIdx.x; t < nlf; t += blockDim.x) {
-    T acc = T(0);
-#pragma unroll
-    for (int a = -1; a <= 1; ++a) {
-      const int ix = 2 * Ix + a;
-      if (ix < 0 || ix > g.nx || (g.mask_kind == MASK_X0 && ix == 0)) continue;
-      const T wa = a == 0 ? T(1) : T(0.5);
-#pragma unroll
-      for (int b = -1; b <= 1; ++b) {
-        const int iy = 2 * Iy + b;
-        if (iy < 0 || iy > g.ny) continue;
-        const size_t dof = (size_t(ix) * fy1 + iy) * nlf + t;
-        T v = rf[dof];
-        if (g.mask_kind == MASK_ARRAY && !g.mask[dof]) v = T(0);
-        acc += (wa * (b == 0 ? T(1) : T(0.5))) * v;
-      }
-    }
-    line[t] = acc;
-  }
-  __syncthreads();
-  const size_t cbase = (size_t(Ix) * cy1 + Iy) * nlc;
-  for (int t = threadIdx.x; t < nlc; t += blockDim.x) {
-    const int Iz = t / 3, k = t - 3 * Iz;
-    T acc = line[6 * Iz + k];
-    if (Iz > 0) acc += T(0.5) * line[6 * Iz - 3 + k];
-    if (2 * Iz + 1 <= g.nz) acc += T(0.5) * line[6 * Iz + 3 + k];
-    bool free_dof = true;
-    if (cmask_kind == MASK_X0) free_dof = Ix > 0;
-    if (cmask_kind == MASK_ARRAY) free_dof = cmask[cbase + t] != 0;
-    rc[cbase + t] = free_dof ? acc : T(0);
-  }
+  restrict_line<T>(rf, rc, g.nx, g.ny, g.nz, g.mask_kind, g.mask, cmask, cmask_kind, int(blockIdx.x),
+                   reinterpret_cast<T*>(smem_raw));
 }
 
 // prolong: xf (+)= mask_f * P ec.
@@ -207,37 +164,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
                                                      const GridArgs<T> g, int assign) {
   extern __shared__ unsigned char smem_raw[];
-  T* cl = reinterpret_cast<T*>(smem_raw);               // 3 (nz/2 + 1) x/y-interpolated coarse values
-  const int fy1 = g.ny + 1, nlf = 3 * (g.nz + 1);
-  const int cy1 = g.ny / 2 + 1, nlc = 3 * (g.nz / 2 + 1);
-  const int iy = int(blockIdx.x % unsigned(fy1)), ix = int(blockIdx.x / unsigned(fy1));
-  const int x0 = ix >> 1, x1 = (ix + 1) >> 1, y0 = iy >> 1, y1 = (iy + 1) >> 1;
-  const T wxy = ((ix & 1) ? T(0.5) : T(1)) * ((iy & 1) ? T(0.5) : T(1));
-  const T* c00 = ec + (size_t(x0) * cy1 + y0) * nlc;
-  const T* c01 = ec + (size_t(x0) * cy1 + y1) * nlc;
-  const T* c10 = ec + (size_t(x1) * cy1 + y0) * nlc;
-  const T* c11 = ec + (size_t(x1) * cy1 + y1) * nlc;
-  for (int t = threadIdx.x; t < nlc; t += blockDim.x) {
-    T v = c00[t];
-    if (iy & 1) v += c01[t];
-    if (ix & 1) {
-      v += c10[t];
-      if (iy & 1) v += c11[t];
-    }
-    cl[t] = wxy * v;
-  }
-  __syncthreads();
-  const size_t fbase = (size_t(ix) * fy1 + iy) * nlf;
-  const bool plane_free = !(g.mask_kind == MASK_X0 && ix == 0);
-  for (int t = threadIdx.x; t < nlf; t += blockDim.x) {
-    const int iz = t / 3, k = t - 3 * iz, z0 = iz >> 1;
-    T p = cl[3 * z0 + k];
-    if (iz & 1) p = T(0.5) * (p + cl[3 * z0 + 3 + k]);
-    bool free_dof = plane_free;
-    if (g.mask_kind == MASK_ARRAY) free_dof = g.mask[fbase + t] != 0;
-    if (!free_dof) p = T(0);
-    xf[fbase + t] = assign ? p : xf[fbase + t] + p;
-  }
+  prolong_line<T>(ec, xf, g.nx, g.ny, g.nz, g.mask_kind, g.mask, assign, int(blockIdx.x),
+                  reinterpret_cast<T*>(smem_raw));
 }
 
 // 2x2x2 mean of element moduli (src/voxel_mg.py:108-111); g = fine grid.

@@ -77,4 +77,17 @@ bool cube_pattern(const double* blocks72, double* k8);
 template <typename T>
 int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipStream_t st);
 
+// A batch of products in one launch: descs (device) holds one MatvecParams<T> per problem with
+// its plan filled in, offs (device, n + 1 entries) the prefix sums of the problems' tile counts.
+// Every problem shares the mode, the mask kind, the tile variant, the load-time combination and
+// the voxel-cube coefficient form.
+struct MatvecBatch {
+  int mode, mask_kind, tile, combine;
+  int n, total;
+  const void* descs;
+  const int* offs;
+};
+template <typename T> int matvec_batch_launch(const MatvecBatch& b, hipStream_t st);
+template <typename T> MatvecPlan matvec_batch_plan(int nx, int ny, int nz, int lx_hint, int tile);
+
 }  // namespace vh

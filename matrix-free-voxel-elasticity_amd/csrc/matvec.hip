@@ -105,8 +105,10 @@ __device__ __forceinline__ void element_blocks(const T (&v)[8][3], T Ee, const M
   }
 }
 
-template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE = false>
-__global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
+// One workgroup's share of the product: tile `logical` (y tile fastest, then z tile, then x
+// chunk) of the grid described by `a`.
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE>
+__device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int logical) {
   constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
   constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
   __shared__ T su[2][3 * PLANE];      // staged node planes ix & 1 (input), AoS rows as in HBM
@@ -120,12 +122,6 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   constexpr bool RECOMPUTE_FL = sizeof(T) == 8 && NEEDS_DINV;
   const bool dinv_from_E = NEEDS_DINV && a.dinv == nullptr;
 
-  // ---- XCD-aware tile decode: hardware assigns block b to XCD b % 8; give XCD k the
-  // contiguous logical range [k*per, (k+1)*per).
-  const int per = gridDim.x >> 3;
-  const int b = blockIdx.x;
-  const int logical = (b & 7) * per + (b >> 3);
-  if (logical >= a.n_tiles) return;
   const int tz = logical % a.tiles_z;
   const int ty = (logical / a.tiles_z) % a.tiles_y;
   const int cx = logical / (a.tiles_z * a.tiles_y);
@@ -421,6 +417,34 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   }
 }
 
+// XCD-aware tile decode: the hardware assigns block b to XCD b % 8; XCD k gets the contiguous
+// logical range [k*per, (k+1)*per), so neighbouring tiles (shared halo rows, columns and
+// planes) are mostly served by the same L2.
+__device__ __forceinline__ int xcd_logical(int b, int grid) { return (b & 7) * (grid >> 3) + (b >> 3); }
+
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE = false>
+__global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
+  const int logical = xcd_logical(blockIdx.x, gridDim.x);
+  if (logical >= a.n_tiles) return;
+  xmarch_tile<T, S, MODE, MASK, TL, CUBE, COMBINE>(a, logical);
+}
+
+// Batched product: problem i owns the logical tiles [offs[i], offs[i+1]) of one launch and
+// reads its parameters (pointers, grid, plan, epilogue scalars, coefficients) from descs[i].
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE = false>
+__global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecParams<T>* __restrict__ descs,
+                                                                 const int* __restrict__ offs, int n) {
+  const int logical = xcd_logical(blockIdx.x, gridDim.x);
+  if (logical >= offs[n]) return;
+  int lo = 0, hi = n;                    // offs[lo] <= logical < offs[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs[mid] <= logical) lo = mid; else hi = mid;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);   // uniform: the descriptor is read with scalar loads
+  xmarch_tile<T, S, MODE, MASK, TL, CUBE, COMBINE>(descs[lo], logical - offs[lo]);
+}
+
 // ------------------------------------------------------------------------------ launcher
 using Tile16 = Tile<16, 32>;
 using Tile8 = Tile<8, 32>;
@@ -586,6 +610,85 @@ int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipSt
 template int matvec_launch<double>(MatvecParams<double>, int, int, int, hipStream_t);
 #elif defined(VH_MATVEC_F32)
 template int matvec_launch<float>(MatvecParams<float>, int, int, int, hipStream_t);
+#endif
+
+// ------------------------------------------------------------------------ batched launcher
+// Instantiated for the voxel-cube coefficients and the passes of the batched solvers: RAW,
+// RESID and DOT (with or without the load-time combination), JACOBI and CHEB3, masks X0 and
+// ARRAY, both tiles.
+#if defined(VH_MATVEC_BATCH_F64) || defined(VH_MATVEC_BATCH_F32)
+template <typename T, int MODE, int MASK, bool COMBINE>
+static void batch_tile(const MatvecBatch& bt, hipStream_t st) {
+  const auto* d = static_cast<const MatvecParams<T>*>(bt.descs);
+  const int grid = ((bt.total + 7) / 8) * 8;
+  if (bt.tile == MV_TILE_8x32)
+    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile8, true, COMBINE>), dim3(grid), dim3(Tile8::BLOCK), 0, st,
+                       d, bt.offs, bt.n);
+  else
+    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile16, true, COMBINE>), dim3(grid), dim3(Tile16::BLOCK), 0,
+                       st, d, bt.offs, bt.n);
+}
+
+template <typename T, int MASK>
+static int batch_mode(const MatvecBatch& bt, hipStream_t st) {
+  switch (bt.mode) {
+    case MODE_RAW: batch_tile<T, MODE_RAW, MASK, false>(bt, st); break;
+    case MODE_RESID:
+      if (bt.combine) batch_tile<T, MODE_RESID, MASK, true>(bt, st);
+      else batch_tile<T, MODE_RESID, MASK, false>(bt, st);
+      break;
+    case MODE_DOT:
+      if (bt.combine) batch_tile<T, MODE_DOT, MASK, true>(bt, st);
+      else batch_tile<T, MODE_DOT, MASK, false>(bt, st);
+      break;
+    case MODE_JACOBI: batch_tile<T, MODE_JACOBI, MASK, false>(bt, st); break;
+    case MODE_CHEB3: batch_tile<T, MODE_CHEB3, MASK, false>(bt, st); break;
+    default: return fail(-2, "batched product: mode %d not available", bt.mode);
+  }
+  return hip_status(hipGetLastError(), "parity_xmarch_batch launch");
+}
+
+template <typename T>
+int matvec_batch_launch(const MatvecBatch& bt, hipStream_t st) {
+  if (bt.mask_kind == MASK_X0) return batch_mode<T, MASK_X0>(bt, st);
+  if (bt.mask_kind == MASK_ARRAY) return batch_mode<T, MASK_ARRAY>(bt, st);
+  return fail(-2, "batched product: mask kind %d not available", bt.mask_kind);
+}
+
+// occupancy of the batched kernel itself (the single-problem kernels live in other translation
+// units)
+template <typename T, typename TL>
+static int resident_batch() {
+  static int occ = 0;
+  if (occ == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch_batch<T, T, MODE_RAW, MASK_X0, TL, true, false>,
+                                                    TL::BLOCK, 0) != hipSuccess || n <= 0)
+      n = 1;
+    occ = n;
+  }
+  return occ;
+}
+
+template <typename T>
+MatvecPlan matvec_batch_plan(int nx, int ny, int nz, int lx_hint, int tile) {
+  MatvecPlan p{};
+  if (tile == MV_TILE_8x32) {
+    plan_tile<Tile8>(nx, ny, nz, lx_hint, resident_batch<T, Tile8>(), &p);
+    p.tile = MV_TILE_8x32;
+  } else {
+    plan_tile<Tile16>(nx, ny, nz, lx_hint, resident_batch<T, Tile16>(), &p);
+    p.tile = MV_TILE_16x32;
+  }
+  return p;
+}
+#endif
+#if defined(VH_MATVEC_BATCH_F64)
+template int matvec_batch_launch<double>(const MatvecBatch&, hipStream_t);
+template MatvecPlan matvec_batch_plan<double>(int, int, int, int, int);
+#elif defined(VH_MATVEC_BATCH_F32)
+template int matvec_batch_launch<float>(const MatvecBatch&, hipStream_t);
+template MatvecPlan matvec_batch_plan<float>(int, int, int, int, int);
 #endif
 
 }  // namespace vh

@@ -17,6 +17,7 @@ MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT, MODE_CHEB3 = 0, 1, 2, 3,
 RED_DOT, RED_DOT2, RED_DOTDIFF, RED_NORM2 = 0, 1, 2, 3
 OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ = 0, 1, 2, 3, 4
 S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
+S_DONE = 10
 TILE_AUTO, TILE_16x32, TILE_8x32 = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -31,6 +32,20 @@ class Epilogue(ctypes.Structure):
                 ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i), ("tile", _i),
                 ("kdiag", _d * 3), ("flags", _i), ("w", _vp), ("sdev", _vp), ("sidx", _i),
                 ("u_out", _vp), ("y32", _vp)]
+
+
+class ProductItem(ctypes.Structure):
+    _fields_ = [("blocks72", _vp), ("E", _vp), ("u", _vp), ("y", _vp), ("nx", _i), ("ny", _i), ("nz", _i),
+                ("mask", _vp), ("ep", Epilogue)]
+
+
+class TransferItem(ctypes.Structure):
+    _fields_ = [("src", _vp), ("dst", _vp), ("nx", _i), ("ny", _i), ("nz", _i), ("fmask_kind", _i),
+                ("cmask_kind", _i), ("fmask", _vp), ("cmask", _vp)]
+
+
+class DenseItem(ctypes.Structure):
+    _fields_ = [("A", _vp), ("ld", _i), ("n", _i), ("r", _vp), ("y", _vp), ("mask", _vp)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)
@@ -59,6 +74,17 @@ SIGNATURES = {
     "vh_convert": [_i, _i, _sz, _vp, _vp, _d, _vp, _i, _i, _vp],
     "vh_cone_filter": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp],
     "vh_oc_probe": [_i, _sz, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _i, _vp],
+    "vh_product_batch_create": [_i, _i, _i, _i, ctypes.POINTER(ProductItem), ctypes.POINTER(_vp),
+                                ctypes.POINTER(_i)],
+    "vh_segments_create": [_i, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_vp)],
+    "vh_seg_reduce": [_vp, _i, _i, _vp, _vp, _vp, _vp],
+    "vh_seg_dotdiff": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "vh_seg_finish": [_vp, _i, _vp, _i, _i, _vp, _i, _i, _i, _vp],
+    "vh_transfer_batch_create": [_i, _i, _i, _i, ctypes.POINTER(TransferItem), ctypes.POINTER(_vp)],
+    "vh_dense_batch_create": [_i, _i, _i, ctypes.POINTER(DenseItem), ctypes.POINTER(_vp)],
+    "vh_batch_launch": [_vp, _vp],
+    "vh_batch_offsets": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "vh_batch_destroy": [_vp],
 }
 _RESTYPES = {"vh_last_error": ctypes.c_char_p}
 

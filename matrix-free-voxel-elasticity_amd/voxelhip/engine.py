@@ -72,32 +72,12 @@ class GMGSolver:
         """(Re)build every level for a new modulus field (device or host array)."""
         E64 = E.to(self.dev, torch.float64).contiguous() if isinstance(E, torch.Tensor) else \
             torch.from_numpy(np.ascontiguousarray(E, dtype=np.float64)).to(self.dev)
-        prods = [mg.HipProduct(s, self.ke * 2 ** i, self.pdtype) for i, s in enumerate(self.shapes)]
-        self.fine = mg.Level(None, self.shape, mg.HipProduct(self.shape, self.ke, torch.float64), E64,
-                             self.mask, torch.float64)
-        # coarsest operator: rediscretised (ke * 2^L on the averaged modulus), or Galerkin
-        Ec = E64
-        masks = [self.mask]
-        for i in range(1, len(self.shapes)):
-            Ec = mg.coarsen_modulus(None, Ec, self.shapes[i - 1])
-            masks.append(masks[-1].coarsen(self.shapes[i - 1]))
-        cshape = self.shapes[-1]
-        cmask_vals = masks[-1].values(cshape, torch.float64, self.dev)
-        if self.coarse == "galerkin" and len(self.shapes) > 1:
-            A = self._galerkin_coarsest(E64)
-        else:
-            A = mg.assemble_dense(cshape, self.ke * 2 ** (len(self.shapes) - 1), Ec, cmask_vals)
-        Lc = torch.linalg.cholesky(torch.from_numpy(A).to(self.dev))
-        kw = {}
-        if self.smoother == "chebyshev":
-            kw = dict(smoother="chebyshev", pre=self.degree, post=self.degree, lower_frac=self.lower_frac)
-        else:
-            kw = dict(pre=self.pre, post=self.post)
-        self.M = mg.Multigrid(None, self.shapes, prods, E64, self.mask, Lc, self.pdtype,
-                              coarse_dtype=self.coarse_dtype, **kw)
-        om, lam = mg.spectral_omegas(None, self.M.levels, iterations=self.omega_iterations)
-        self.M.omegas = om
-        self.M.lambdas = lam
+        galerkin = self._galerkin_coarsest(E64) if (self.coarse == "galerkin" and len(self.shapes) > 1) else None
+        self.fine, self.M = build_hierarchy(self.shape, self.ke, self.mask, E64, self.shapes, self.pdtype,
+                                            smoother=self.smoother, degree=self.degree, pre=self.pre,
+                                            post=self.post, lower_frac=self.lower_frac,
+                                            omega_iterations=self.omega_iterations,
+                                            coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin)
         if self.tuner is not None:
             self.M.tune(self.tuner)
             self.fine.tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True), (L.MODE_RESID, False)])
@@ -308,6 +288,34 @@ class GMGSolver:
             matvecs += 1
         self.last_outer = outer
         return self.x, inner_total, bool(rel <= tol), rel, matvecs
+
+
+def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev", degree=3, pre=4, post=4,
+                    lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None):
+    """The FP64 fine Level and the Multigrid (levels in pdtype, rediscretised ke * 2^l, 2x2x2
+    modulus averaging, injected masks, explicit inverse of the coarsest operator, spectral
+    omegas / lambdas) of one problem.  coarse_matrix: a precomputed coarsest operator (Galerkin);
+    otherwise the rediscretised one is assembled."""
+    dev = E64.device
+    prods = [mg.HipProduct(s, ke * 2 ** i, pdtype) for i, s in enumerate(shapes)]
+    fine = mg.Level(None, shape, mg.HipProduct(shape, ke, torch.float64), E64, mask, torch.float64)
+    if coarse_matrix is None:
+        Ec, m = E64, mask
+        for i in range(1, len(shapes)):
+            Ec = mg.coarsen_modulus(None, Ec, shapes[i - 1])
+            m = m.coarsen(shapes[i - 1])
+        cshape = shapes[-1]
+        coarse_matrix = mg.assemble_dense(cshape, ke * 2 ** (len(shapes) - 1), Ec, m.values(cshape, torch.float64, dev))
+    Lc = torch.linalg.cholesky(torch.from_numpy(coarse_matrix).to(dev))
+    if smoother == "chebyshev":
+        kw = dict(smoother="chebyshev", pre=degree, post=degree, lower_frac=lower_frac)
+    else:
+        kw = dict(pre=pre, post=post)
+    M = mg.Multigrid(None, shapes, prods, E64, mask, Lc, pdtype, coarse_dtype=coarse_dtype, **kw)
+    om, lam = mg.spectral_omegas(None, M.levels, iterations=omega_iterations)
+    M.omegas = om
+    M.lambdas = lam
+    return fine, M
 
 
 def _prolongation_matrix(coarse):
