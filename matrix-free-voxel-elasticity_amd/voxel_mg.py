@@ -425,3 +425,41 @@ def spectral_omegas(xp, levels, iterations=30, seed=20260923):
         lambdas.append(lam)
         omegas.append(4.0 / (3.0 * lam))
     return omegas, lambdas
+
+
+def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923):
+    """Upper bounds of the spectrum of D^-1 A per smoothing level for the Chebyshev smoother:
+    safety x the D-weighted Rayleigh quotient v.Av / v.Dv after `iterations` power steps
+    v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from the seeded
+    random vector of src/voxel_mg.py:234-235 over the free DOFs.  (spectral_omegas' norm ratio
+    approaches lambda_max from below slowly on smooth moduli - 30 steps give 2.81 for 3.09 on a
+    uniform 256 x 128 x 128 grid - and a Chebyshev interval that ends below the spectrum
+    amplifies the top modes; the Rayleigh quotient converges twice as fast and the 10 % margin
+    covers the rest.)"""
+    lams = []
+    for Lv in levels[:-1]:
+        rng = np.random.default_rng(seed)
+        d = Lv.product.diagonal(Lv.E, mask=Lv.maskobj)               # D, 1 at fixed DOFs
+        v = voxelhip.to_device(rng.standard_normal(Lv.n), torch.float64).to(Lv.dtype)
+        v = ops.axpby(1.0, v, 0.0, v, w=Lv.mask)
+        red = _Reducer(Lv.n, v.device)
+        t = torch.empty_like(v)
+        Kv = torch.empty_like(v)
+        ops.axpby(1.0, v, 0.0, t, w=d)
+        red.dot(v, t, 8)
+        ops.axpby(1.0 / float(red.scal[8].sqrt()), v, 0.0, v)
+        lam = 0.0
+        for _ in range(iterations):
+            Lv.matvec(v, Kv, fixed_zero=True)                         # M K M v
+            red.dot(v, Kv, 8)
+            ops.axpby(1.0, v, 0.0, t, w=d)
+            red.dot(v, t, 9)
+            s89 = red.scal[8:10].cpu()
+            lam = float(s89[0]) / float(s89[1])
+            ops.axpby(1.0, Kv, 0.0, Kv, w=Lv.inv_diag)                # D^-1 K v
+            ops.axpby(1.0, Kv, 0.0, t, w=d)
+            red.dot(Kv, t, 8)
+            ops.axpby(1.0 / float(red.scal[8].sqrt()), Kv, 0.0, v)
+        lams.append(safety * lam)
+    return lams
+

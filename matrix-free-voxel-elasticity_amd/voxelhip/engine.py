@@ -312,9 +312,11 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
     else:
         kw = dict(pre=pre, post=post)
     M = mg.Multigrid(None, shapes, prods, E64, mask, Lc, pdtype, coarse_dtype=coarse_dtype, **kw)
-    om, lam = mg.spectral_omegas(None, M.levels, iterations=omega_iterations)
-    M.omegas = om
-    M.lambdas = lam
+    # Jacobi: the reference's spectral omegas; Chebyshev: safe upper bounds of the spectrum
+    if smoother == "chebyshev":
+        M.lambdas = mg.chebyshev_bounds(None, M.levels)
+    else:
+        M.omegas, M.lambdas = mg.spectral_omegas(None, M.levels, iterations=omega_iterations)
     return fine, M
 
 

@@ -13,6 +13,10 @@ hand-written adjoint behind torch.autograd, and the 100-step optimisation loop.
                                      element-energy kernel
   simp_run(problem, steps, ...)      density-filtered SIMP with the top88 OC
                                      (experiments/session_h/run_w1b.py:163-222, update "top88_gpu")
+  SimpProblem(shape, ke, mask, force, ...)
+                                     the state problem of simp_run on the GMG-FCG engine: every
+                                     step rebuilds the hierarchy for the new moduli and solves
+                                     K(E) u = f to the true relative residual `tol`
 """
 from __future__ import annotations
 
@@ -104,6 +108,36 @@ class Compliance(torch.autograd.Function):
         ops.element_energy(pb.blocks, u, ce, pb.shape, mask=pb.mask, rho=rho.detach().contiguous(), dc=dc,
                            penal=pb.penal, dE=pb.E0 - pb.Emin)
         return grad * dc, None
+
+
+class SimpProblem:
+    """State problem of the SIMP loop: K(E) u = f on the device-resident GMG-FCG engine
+    (voxelhip.engine.GMGSolver; precision "mixed" = FP32 V-cycle in the FP64 outer iteration).
+    solve(E) rebuilds the hierarchy for E and returns a copy of u; the iteration count and the
+    residual of the last solve are kept in last_iterations / last_rel."""
+
+    def __init__(self, shape, ke, mask, force, *, precision="mixed", tol=1e-8, maxiter=200, penal=3.0, E0=1.0,
+                 Emin=1e-9, graph=False, **solver_kw):
+        from .engine import GMGSolver
+        import voxel_kernels as vk
+        self.shape = tuple(int(s) for s in shape)
+        self.penal, self.E0, self.Emin = float(penal), float(E0), float(Emin)
+        self.tol, self.maxiter = float(tol), int(maxiter)
+        self.blocks = vk.Modal(ke).blocks
+        self.force = force if isinstance(force, torch.Tensor) else torch.from_numpy(force)
+        self.force = self.force.to("cuda", torch.float64).contiguous()
+        self.mask = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape)
+        solver_kw.setdefault("smoother", "chebyshev")
+        self.solver = GMGSolver(self.shape, ke, self.mask, None, precision=precision, graph=graph, **solver_kw)
+        self.last_iterations, self.last_rel = None, None
+
+    def solve(self, E):
+        self.solver.update(E)
+        x, it, ok, rel, _ = self.solver.solve(self.force, tol=self.tol, maxiter=self.maxiter)
+        if not ok:
+            raise RuntimeError(f"state solve did not reach {self.tol:g} in {it} iterations (rel {rel:.3e})")
+        self.last_iterations, self.last_rel = it, rel
+        return x.clone()
 
 
 def simp_run(problem, steps, volfrac=0.3, rmin=1.5, move=0.2, stop_change=None):

@@ -250,6 +250,28 @@ def spectral_omegas(levels, iterations=30, seed=20260923):
     return omegas, lambdas
 
 
+def chebyshev_bounds(levels, iterations=30, safety=1.1, seed=20260923):
+    """Upper bounds of the spectrum of D^-1 A per smoothing level for the Chebyshev smoother:
+    safety x the D-weighted Rayleigh quotient v.Av / v.Dv after `iterations` power steps
+    v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from the seeded
+    random vector of src/voxel_mg.py:234-235 (the donor's all-ones start is a smooth vector, far
+    from the top of the spectrum), over the free DOFs."""
+    lams = []
+    for L in levels[:-1]:
+        rng = np.random.default_rng(seed)
+        d = np.where(L.mask != 0, 1.0 / L.inv_diag, 1.0).astype(L.dtype)
+        v = rng.standard_normal(L.mask.size).astype(L.dtype) * L.mask
+        v = v / np.sqrt(float(v @ (d * v)))
+        lam = 0.0
+        for _ in range(iterations):
+            Kv = L.apply(v) * L.mask
+            lam = float(v @ Kv) / float(v @ (d * v))
+            w = L.inv_diag * Kv
+            v = w / np.sqrt(float(w @ (d * w)))
+        lams.append(safety * lam)
+    return lams
+
+
 def build_hierarchy(shape, ke, E, mask, dtype=np.float64, max_coarse_x=16, pre=4, post=4,
                     smoother="jacobi", degree=None):
     """Hierarchy as the reference tests/harness build it (tests/test_mg_cpu.py:98-108,
@@ -265,9 +287,8 @@ def build_hierarchy(shape, ke, E, mask, dtype=np.float64, max_coarse_x=16, pre=4
     if smoother == "chebyshev":
         M = Multigrid(shapes, products, E, mask, factor, dtype, pre=degree or pre,
                       post=degree or post, smoother="chebyshev")
-        _, lams = spectral_omegas(M.levels)
-        M.lambdas = lams
-        M.omegas = [4.0 / (3.0 * l) for l in lams]
+        M.omegas, _ = spectral_omegas(M.levels)
+        M.lambdas = chebyshev_bounds(M.levels)
     else:
         M = Multigrid(shapes, products, E, mask, factor, dtype, pre=pre, post=post)
         M.omegas, _ = spectral_omegas(M.levels)

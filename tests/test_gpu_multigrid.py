@@ -46,6 +46,7 @@ def build(KE, shape, E, mask, dtype=np.float64, max_coarse_x=2, smoother="jacobi
     om, lam = mg.spectral_omegas(xp, M.levels)
     M.omegas = om
     if smoother == "chebyshev":
+        lam = mg.chebyshev_bounds(xp, M.levels)
         M.lambdas = lam
     return shapes, M, lam
 

@@ -117,26 +117,10 @@ def test_oc_update_matches_the_oracle():
     assert rel(xd, xo) < 1e-12
 
 
-class _DirectProblem:
-    """Tiny problem solved exactly (dense Cholesky of the HIP-assembled columns is overkill; the
-    GMG engine solves to 1e-12 here)."""
-
-    def __init__(self, shape, ke, mask, force):
-        from voxelhip.engine import GMGSolver
-        from voxelhip import ops
-        import voxel_kernels as vk
-        self.shape, self.penal, self.E0, self.Emin = shape, 3.0, 1.0, 1e-9
-        self.blocks = vk.Modal(ke).blocks
-        self.mask = ops.Mask.from_values(mask, shape)
-        self.force = torch.from_numpy(force).cuda()
-        self.solver = GMGSolver(shape, ke, mask, None, max_coarse_x=4, smoother="chebyshev", degree=3)
-
-    def solve(self, E):
-        self.solver.update(E)
-        x, it, ok, r, _ = self.solver.solve(self.force, tol=1e-12, maxiter=200)
-        assert ok
-        self.last_iterations = it
-        return x.clone()
+def _problem(shape, ke, mask, force, precision="fp64", tol=1e-12):
+    """The SIMP state problem on the GMG engine, solved far below the FD step's truncation."""
+    from voxelhip.simp import SimpProblem
+    return SimpProblem(shape, ke, mask, force, precision=precision, tol=tol, max_coarse_x=4, degree=3)
 
 
 def test_compliance_adjoint_matches_finite_differences():
@@ -144,7 +128,7 @@ def test_compliance_adjoint_matches_finite_differences():
     shape = (8, 4, 4)
     ke = 2 * hx.quadrature([1, 1, 1], 0.3)
     rho, _, mask, force = sr.cantilever(shape, seed=2, rho_lo=0.3)
-    pb = _DirectProblem(shape, ke, mask, force)
+    pb = _problem(shape, ke, mask, force)
     r = torch.from_numpy(rho).cuda().requires_grad_(True)
     c = Compliance.apply(r, pb)
     c.backward()
@@ -169,9 +153,27 @@ def test_simp_loop_runs_and_keeps_the_volume():
     shape = (16, 8, 8)
     ke = 2 * hx.quadrature([1, 1, 1], 0.3)
     _, _, mask, force = sr.cantilever(shape, seed=1)
-    pb = _DirectProblem(shape, ke, mask, force)
+    pb = _problem(shape, ke, mask, force)
     hist, rho = simp_run(pb, steps=5, volfrac=0.3, rmin=1.5, move=0.2)
     assert len(hist) == 5
     assert all(abs(h["volume"] - 0.3) < 2e-3 for h in hist)
     assert hist[-1]["compliance"] < hist[0]["compliance"]
     assert float(rho.min()) >= 0.0 and float(rho.max()) <= 1.0
+
+
+def test_mixed_precision_state_solve_gives_the_fp64_sensitivities():
+    """The SIMP step on the mixed solver (FP32 V-cycle, FP64 outer CG to 1e-8): compliance and
+    dc/drho within 1e-6 of the FP64 solve at 1e-12."""
+    from voxelhip.simp import Compliance
+    shape = (16, 8, 8)
+    ke = 2 * hx.quadrature([1, 1, 1], 0.3)
+    rho, _, mask, force = sr.cantilever(shape, seed=3, rho_lo=0.3)
+    out = []
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-8)):
+        pb = _problem(shape, ke, mask, force, precision=prec, tol=tol)
+        r = torch.from_numpy(rho).cuda().requires_grad_(True)
+        c = Compliance.apply(r, pb)
+        c.backward()
+        out.append((float(c), r.grad.clone()))
+    assert out[1][0] == pytest.approx(out[0][0], rel=1e-7)
+    assert float((out[1][1] - out[0][1]).norm() / out[0][1].norm()) < 1e-6

@@ -187,3 +187,25 @@ def test_chebyshev_multigrid_converges(golden, KE):
     _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2, smoother="chebyshev", degree=3)
     _, it, ok, rel, _ = mg.fcg(apply_fine(KE, shape, E, mask), force, M, 1e-8, 300)
     assert ok and it <= 40, (it, rel)
+
+
+def test_chebyshev_bounds_enclose_the_spectrum(golden, KE):
+    """The Chebyshev upper bound against the dense spectrum of D^-1 A (free DOFs) on the smoke
+    state and on a uniform-modulus grid: the Rayleigh estimate is within 5 % of lambda_max, and
+    the bound (x 1.1) lies above it."""
+    d = np.load(golden / "mg_golden.npz")
+    cases = [(tuple(int(v) for v in d["smoke/shape"]), d["smoke/E"], d["smoke/mask"])]
+    shape = (8, 4, 4)
+    m = np.ones((9, 5, 5, 3)); m[0] = 0
+    cases.append((shape, np.full(128, 0.027), m.reshape(-1)))
+    for shape, E, mask in cases:
+        _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2, smoother="chebyshev", degree=3)
+        lv = M.levels[0]
+        A = hx.assemble_dense(shape, KE, E, mask)
+        free = mask != 0
+        Af = A[np.ix_(free, free)]
+        dg = np.diag(Af)
+        lam_true = np.linalg.eigvalsh(Af / np.sqrt(np.outer(dg, dg))).max()
+        bound = mg.chebyshev_bounds(M.levels)[0]
+        assert lam_true <= bound and bound / 1.1 == pytest.approx(lam_true, rel=0.05)
+        assert np.allclose(np.diag(A)[free], 1.0 / lv.inv_diag[free])
