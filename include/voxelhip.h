@@ -28,7 +28,8 @@ enum { VH_MASK_NONE = 0, VH_MASK_ARRAY = 1, VH_MASK_X0 = 2 };
 enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3, VH_MODE_DOT = 4,
        VH_MODE_CHEB3 = 5 };
 enum { VH_RED_DOT = 0, VH_RED_DOT2 = 1, VH_RED_DOTDIFF = 2, VH_RED_NORM2 = 3 };
-enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4 };
+enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4,
+       VH_OP_INVSQRT = 5 /* scal[dst] = 1 / sqrt(sum): a device-side normalisation factor */ };
 /* device scalar slots used by vh_finish / vh_cg_update / vh_xpby; VH_S_DONE != 0 freezes the
  * recurrence (alpha = beta = 0), for converged problems of a batch */
 enum { VH_S_RZ = 0, VH_S_PQ = 1, VH_S_ALPHA = 2, VH_S_BETA = 3, VH_S_FLAG = 4, VH_S_DONE = 10 };

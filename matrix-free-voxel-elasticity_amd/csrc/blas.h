@@ -7,7 +7,7 @@ constexpr int BLAS_BLOCK = 256;
 constexpr int BLAS_MAX_GRID = 2048;     // 8 blocks per CU on 256 CUs; partial arrays are this long
 
 enum RedKind : int { RED_DOT = 0, RED_DOT2 = 1, RED_DOTDIFF = 2, RED_NORM2 = 3 };
-enum ScalarOp : int { OP_NONE = 0, OP_ALPHA = 1, OP_BETA_FR = 2, OP_BETA_PR = 3, OP_SET_RZ = 4 };
+enum ScalarOp : int { OP_NONE = 0, OP_ALPHA = 1, OP_BETA_FR = 2, OP_BETA_PR = 3, OP_SET_RZ = 4, OP_INVSQRT = 5 };
 // Device scalar slots (FP64).  Slots 8, 9, 11.. are free for callers (norms, dots); S_DONE != 0
 // freezes a Krylov recurrence (alpha = beta = 0: the iterate stops moving), which the batched
 // solvers set for problems that have converged while others iterate on.
@@ -43,6 +43,7 @@ __device__ inline void apply_scalar_op(double* s, const double* v, int count, in
       break;
     }
     case OP_SET_RZ: s[S_RZ] = v[0]; break;
+    case OP_INVSQRT: s[dst] = v[0] > 0.0 ? 1.0 / sqrt(v[0]) : 0.0; break;   // normalisation factor
     default: break;
   }
 }

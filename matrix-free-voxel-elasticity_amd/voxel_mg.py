@@ -403,13 +403,28 @@ def fcg(xp, apply_A, b, precondition, tol, maxiter, check_every=5, fixed_iterati
     return x, it, bool(rel <= tol), rel, matvecs
 
 
+_STARTS = {}
+
+
+def _start_vector(n, seed, dtype):
+    """default_rng(seed).standard_normal(n) on the device (src/voxel_mg.py:234-235), kept per
+    (n, seed, dtype): drawing tens of millions of normals on the host costs more than the power
+    iteration itself, and the set-up of every SIMP step asks for the same vectors again."""
+    key = (int(n), int(seed), dtype)
+    v = _STARTS.get(key)
+    if v is None:
+        rng = np.random.default_rng(seed)
+        v = voxelhip.to_device(rng.standard_normal(int(n)), torch.float64).to(dtype)
+        _STARTS[key] = v
+    return v
+
+
 def spectral_omegas(xp, levels, iterations=30, seed=20260923):
     """omega_l = 4 / (3 lambda_l) for every non-coarsest level, lambda_l from a seeded power
     iteration on D^-1 A restricted to free DOFs (src/voxel_mg.py:228-244).  Returns (omegas, lambdas)."""
     omegas, lambdas = [], []
     for Lv in levels[:-1]:
-        rng = np.random.default_rng(seed)
-        v = voxelhip.to_device(rng.standard_normal(Lv.n), torch.float64).to(Lv.dtype)
+        v = _start_vector(Lv.n, seed, Lv.dtype).clone()
         v = ops.axpby(1.0, v, 0.0, v, w=Lv.mask)
         red = _Reducer(Lv.n, v.device)
         red.norm2(v, 8)
@@ -435,31 +450,25 @@ def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923):
     approaches lambda_max from below slowly on smooth moduli - 30 steps give 2.81 for 3.09 on a
     uniform 256 x 128 x 128 grid - and a Chebyshev interval that ends below the spectrum
     amplifies the top modes; the Rayleigh quotient converges twice as fast and the 10 % margin
-    covers the rest.)"""
+    covers the rest.)  The normalisation stays on the device: one host read per level."""
     lams = []
     for Lv in levels[:-1]:
-        rng = np.random.default_rng(seed)
         d = Lv.product.diagonal(Lv.E, mask=Lv.maskobj)               # D, 1 at fixed DOFs
-        v = voxelhip.to_device(rng.standard_normal(Lv.n), torch.float64).to(Lv.dtype)
-        v = ops.axpby(1.0, v, 0.0, v, w=Lv.mask)
-        red = _Reducer(Lv.n, v.device)
-        t = torch.empty_like(v)
-        Kv = torch.empty_like(v)
+        w = _start_vector(Lv.n, seed, Lv.dtype).clone()
+        w = ops.axpby(1.0, w, 0.0, w, w=Lv.mask)
+        red = _Reducer(Lv.n, w.device)
+        v, t = torch.empty_like(w), torch.empty_like(w)
+        for k in range(iterations + 1):
+            if k:                                                     # w = D^-1 M K M v
+                Lv.matvec(v, w, fixed_zero=True)
+                ops.axpby(1.0, w, 0.0, w, w=Lv.inv_diag)
+            ops.axpby(1.0, w, 0.0, t, w=d)
+            red.dot(w, t, 8, L.OP_INVSQRT)                            # 1 / |w|_D
+            ops.convert(w, v, dscale=red.scal[8:9])
+        Lv.matvec(v, w, fixed_zero=True)                              # Rayleigh quotient
+        red.dot(v, w, 8)
         ops.axpby(1.0, v, 0.0, t, w=d)
-        red.dot(v, t, 8)
-        ops.axpby(1.0 / float(red.scal[8].sqrt()), v, 0.0, v)
-        lam = 0.0
-        for _ in range(iterations):
-            Lv.matvec(v, Kv, fixed_zero=True)                         # M K M v
-            red.dot(v, Kv, 8)
-            ops.axpby(1.0, v, 0.0, t, w=d)
-            red.dot(v, t, 9)
-            s89 = red.scal[8:10].cpu()
-            lam = float(s89[0]) / float(s89[1])
-            ops.axpby(1.0, Kv, 0.0, Kv, w=Lv.inv_diag)                # D^-1 K v
-            ops.axpby(1.0, Kv, 0.0, t, w=d)
-            red.dot(Kv, t, 8)
-            ops.axpby(1.0 / float(red.scal[8].sqrt()), Kv, 0.0, v)
-        lams.append(safety * lam)
+        red.dot(v, t, 9)
+        s89 = red.scal[8:10].cpu()
+        lams.append(safety * float(s89[0]) / float(s89[1]))
     return lams
-

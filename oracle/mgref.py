@@ -262,13 +262,11 @@ def chebyshev_bounds(levels, iterations=30, safety=1.1, seed=20260923):
         d = np.where(L.mask != 0, 1.0 / L.inv_diag, 1.0).astype(L.dtype)
         v = rng.standard_normal(L.mask.size).astype(L.dtype) * L.mask
         v = v / np.sqrt(float(v @ (d * v)))
-        lam = 0.0
         for _ in range(iterations):
-            Kv = L.apply(v) * L.mask
-            lam = float(v @ Kv) / float(v @ (d * v))
-            w = L.inv_diag * Kv
+            w = L.inv_diag * (L.apply(v) * L.mask)
             v = w / np.sqrt(float(w @ (d * w)))
-        lams.append(safety * lam)
+        Kv = L.apply(v) * L.mask
+        lams.append(safety * float(v @ Kv) / float(v @ (d * v)))
     return lams
 
 
