@@ -255,6 +255,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--tune", type=int, default=1, help="empirical launch tuning of the product passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batched leg")
     ap.add_argument("--simp-steps", type=int, default=100,
                     help="SIMP steps of the 384x192x192 optimisation leg (0 skips it)")
@@ -269,6 +270,7 @@ def main():
     import torch
     import voxel_kernels as vk
     from voxelhip.engine import GMGSolver
+    from voxelhip.tuner import LaunchTuner
     torch.cuda.set_device(0)
     if args.only == "matvec":
         dom = matvec_timing((256, 128, 128), torch.float64)
@@ -281,13 +283,16 @@ def main():
         print(json.dumps({"only": "matvec", "matvec": dom, "copy_bytes_each_way": nd * 8}), flush=True)
         return
     ke = 2 * vk.quadrature([1, 1, 1], 0.3)          # the donor's KE_UNIT_3D (ke_unit_3d.npy)
+    # launch plans of every product pass, timed at set-up (outside the timed solves)
+    tuner = LaunchTuner() if args.tune else False
     shape = (256, 128, 128)
     rho, E, force, mask, ops = problem(shape, seed=3)
     ndof = force.numel()
     out = {}
     solvers = {}
     for prec in ("mixed", "fp64"):
-        s = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision=prec, graph=bool(args.graph))
+        s = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision=prec, graph=bool(args.graph),
+                      tune=tuner)
         solvers[prec] = s
         dt, st = timed_solves(s, force, args.steps, args.warmup)
         its = sorted({x[0] for x in st})
@@ -305,7 +310,7 @@ def main():
     shape2 = (128, 64, 64)
     _, E2, f2, m2, _ = problem(shape2, seed=2)
     s2 = GMGSolver(shape2, ke, m2, E2, smoother="chebyshev", degree=3, precision="fp64", coarse="galerkin",
-                   graph=bool(args.graph))
+                   graph=bool(args.graph), tune=tuner)
     dt2, st2 = timed_solves(s2, f2, args.steps, args.warmup)
     out["c2_fp64"] = {"shape": list(shape2), "seconds_per_solve": dt2, "solves_per_s": 1 / dt2,
                       "iterations": sorted({x[0] for x in st2}), "converged": all(x[1] for x in st2)}

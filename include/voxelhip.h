@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 1
+#define VH_ABI_VERSION 2
 
 enum { VH_F64 = 0, VH_F32 = 1, VH_BF16 = 2 };
 enum { VH_MASK_NONE = 0, VH_MASK_ARRAY = 1, VH_MASK_X0 = 2 };
@@ -79,19 +79,26 @@ typedef struct vh_epilogue {
   int tile; /* (y, z) tile variant: 0 = automatic, 1 = 16 x 32 nodes, 2 = 8 x 32 nodes */
   double kdiag[3];
   int flags; /* VH_PREV_ZERO: CHEB3's previous iterate is zero (y is then not read) */
-  /* DOT / RESID only: the pass reads u + s w with s = sdev[sidx] (device FP64 scalar) and writes
-   * that combined input to u_out at every DOF (voxel-cube blocks);  RESID also writes a float32
-   * copy of y to y32 and, with partial != NULL, per-tile sums of y^2. */
+  /* Load-time combination (voxel-cube blocks), the combined input written to u_out at every
+   * DOF:  DOT / RESID read u + s w with s = sdev[sidx] (device FP64 scalar);  CHEB3 reads
+   * load_scale * (u (.) w) (elementwise product: with u = b, w = dinv and load_scale = 1/theta
+   * this is the first Chebyshev step from a zero iterate, so the pass applies steps 1 and 2).
+   * RESID also writes a float32 copy of y to y32 and, with partial != NULL, per-tile sums of y^2. */
   const void* w;
   const double* sdev;
   int sidx;
   void* u_out;
   float* y32;
+  double load_scale;
 } vh_epilogue;
 #define VH_PREV_ZERO 1
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
                     int nx, int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep,
                     void* stream);
+/* 1 if the 72 block values have the voxel-cube pattern (8 distinct values; the form the
+ * load-time combinations and the batched product require), 0 otherwise.  A query of the
+ * kernel's coefficient form (host), with no counterpart in the reference. */
+int vh_blocks_are_cube(const double* blocks72);
 /* Number of tiles (= DOT partials) and workgroups the product launches for these arguments
  * (blocks72 host pointer or NULL for the voxel-cube pattern; lx/tile as in vh_epilogue). */
 int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int ny, int nz, int lx, int tile,

@@ -237,6 +237,7 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
     p.u_out = static_cast<T*>(ep.u_out);
     p.y32 = static_cast<float*>(ep.y32);
     p.fixed_zero = ep.fixed_zero;
+    p.load_scale = T(ep.load_scale);
     const MatvecPlan pl = matvec_batch_plan<T>(it.nx, it.ny, it.nz, ep.lx, tile);
     p.tile = pl.tile;
     p.lx = pl.lx;
@@ -302,7 +303,8 @@ int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_
   const int combine = items[0].ep.w != nullptr;
   VH_REQUIRE(mode == MODE_RAW || mode == MODE_RESID || mode == MODE_DOT || mode == MODE_JACOBI || mode == MODE_CHEB3,
              "vh_product_batch_create: mode %d", mode);
-  VH_REQUIRE(!combine || mode == MODE_RESID || mode == MODE_DOT, "vh_product_batch_create: combination needs DOT/RESID");
+  VH_REQUIRE(!combine || mode == MODE_RESID || mode == MODE_DOT || mode == MODE_CHEB3,
+             "vh_product_batch_create: combination needs DOT/RESID/CHEB3");
   for (int i = 0; i < n; ++i) {
     const vh_product_item& it = items[i];
     VH_REQUIRE(it.blocks72 && it.E && it.u && it.y && it.u != it.y, "vh_product_batch_create: item %d pointers", i);
@@ -314,8 +316,8 @@ int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_
     VH_REQUIRE(!(mode == MODE_RESID || mode == MODE_JACOBI || mode == MODE_CHEB3) || it.ep.b,
                "vh_product_batch_create: item %d b", i);
     VH_REQUIRE(mode != MODE_DOT || it.ep.partial, "vh_product_batch_create: item %d partial", i);
-    VH_REQUIRE(!combine || (it.ep.sdev && it.ep.u_out && it.ep.u_out != it.u && it.ep.u_out != it.y &&
-                            it.ep.w != it.y && it.ep.u_out != it.ep.w),
+    VH_REQUIRE(!combine || ((it.ep.sdev || mode == MODE_CHEB3) && it.ep.u_out && it.ep.u_out != it.u &&
+                            it.ep.u_out != it.y && it.ep.w != it.y && it.ep.u_out != it.ep.w && it.ep.u_out != it.ep.b),
                "vh_product_batch_create: item %d combination operands", i);
     VH_REQUIRE(!it.ep.y32 || mode == MODE_RESID, "vh_product_batch_create: item %d y32", i);
     if ((mode == MODE_JACOBI || mode == MODE_CHEB3) && !it.ep.dinv)

@@ -160,6 +160,7 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
     p.u_out = static_cast<T*>(ep->u_out);
     p.y32 = static_cast<float*>(ep->y32);
     p.fixed_zero = ep->fixed_zero;
+    p.load_scale = T(ep->load_scale);
   }
   return matvec_launch<T>(p, storage, mode, mask_kind, st);
 }
@@ -186,6 +187,11 @@ const char* vh_last_error(void) { return g_err; }
 int vh_modal_blocks(const double* ke576, double tolerance, double* blocks72, double* discarded) {
   VH_REQUIRE(ke576 && blocks72, "vh_modal_blocks: null pointer");
   return modal_blocks(ke576, tolerance, blocks72, discarded);
+}
+
+int vh_blocks_are_cube(const double* blocks72) {
+  double k8[8];
+  return blocks72 != nullptr && cube_pattern(blocks72, k8) ? 1 : 0;
 }
 
 int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int ny, int nz, int lx, int tile,
@@ -217,9 +223,11 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
     VH_REQUIRE(!(m == MODE_RESID || smoother) || ep->b, "vh_matvec: b missing");
     VH_REQUIRE(m != MODE_CHEB || ep->d, "vh_matvec: d missing");
     VH_REQUIRE(m != MODE_DOT || ep->partial, "vh_matvec: partial missing");
-    VH_REQUIRE(!ep->w || ((m == MODE_DOT || m == MODE_RESID) && ep->sdev && ep->u_out && ep->u_out != u &&
-                          ep->u_out != y && ep->w != y && ep->u_out != ep->w),
-               "vh_matvec: the load-time combination needs a DOT/RESID pass, sdev and a distinct u_out");
+    VH_REQUIRE(!ep->w || (((m == MODE_DOT || m == MODE_RESID) && ep->sdev) || m == MODE_CHEB3),
+               "vh_matvec: the load-time combination needs a DOT/RESID pass with sdev, or a CHEB3 pass");
+    VH_REQUIRE(!ep->w || (ep->u_out && ep->u_out != u && ep->u_out != y && ep->w != y && ep->u_out != ep->w &&
+                          ep->u_out != ep->b),
+               "vh_matvec: the load-time combination needs a distinct u_out");
     VH_REQUIRE(!ep->y32 || m == MODE_RESID, "vh_matvec: y32 is an output of the RESID pass");
     VH_REQUIRE(ep->b != y && (ep->dinv == nullptr || ep->dinv != y), "vh_matvec: y must not alias an input");
     if (smoother && ep->dinv == nullptr)   // D^-1 from the moduli and ke's diagonal

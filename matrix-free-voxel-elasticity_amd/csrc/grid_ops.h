@@ -79,6 +79,18 @@ __device__ __forceinline__ void prolong_line(const T* __restrict__ ec, T* __rest
   const T* c01 = ec + (size_t(x0) * cy1 + y1) * nlc;
   const T* c10 = ec + (size_t(x1) * cy1 + y0) * nlc;
   const T* c11 = ec + (size_t(x1) * cy1 + y1) * nlc;
+  const size_t fbase = (size_t(ix) * fy1 + iy) * nlf;
+  // the fine values this thread updates are loaded before the coarse lines, so both memory
+  // round trips of the workgroup overlap (lines of up to 3 * 2049 values: <= 24 per thread)
+  constexpr int PF = 4;
+  T xold[PF];
+  if (!assign) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int t = threadIdx.x + j * blockDim.x;
+      xold[j] = t < nlf ? xf[fbase + t] : T(0);
+    }
+  }
   for (int t = threadIdx.x; t < nlc; t += blockDim.x) {
     T v = c00[t];
     if (iy & 1) v += c01[t];
@@ -89,17 +101,22 @@ __device__ __forceinline__ void prolong_line(const T* __restrict__ ec, T* __rest
     cl[t] = wxy * v;
   }
   __syncthreads();
-  const size_t fbase = (size_t(ix) * fy1 + iy) * nlf;
   const bool plane_free = !(mask_kind == MASK_X0 && ix == 0);
-  for (int t = threadIdx.x; t < nlf; t += blockDim.x) {
+  auto interp = [&](int t) {
     const int iz = t / 3, k = t - 3 * iz, z0 = iz >> 1;
     T p = cl[3 * z0 + k];
     if (iz & 1) p = T(0.5) * (p + cl[3 * z0 + 3 + k]);
     bool free_dof = plane_free;
     if (mask_kind == MASK_ARRAY) free_dof = mask[fbase + t] != 0;
-    if (!free_dof) p = T(0);
-    xf[fbase + t] = assign ? p : xf[fbase + t] + p;
+    return free_dof ? p : T(0);
+  };
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int t = threadIdx.x + j * blockDim.x;
+    if (t < nlf) xf[fbase + t] = (assign ? T(0) : xold[j]) + interp(t);
   }
+  for (int t = threadIdx.x + PF * blockDim.x; t < nlf; t += blockDim.x)
+    xf[fbase + t] = (assign ? T(0) : xf[fbase + t]) + interp(t);
 }
 
 template <typename T> int diag_launch(const T* E, const GridArgs<T>& g, T* out, int invert, hipStream_t st);

@@ -16,6 +16,8 @@ struct Tile {
   static constexpr int LOADS = (NY * 3 * NZ + BLOCK - 1) / BLOCK;
 };
 constexpr int MV_PREV_ZERO = 1;
+// load-time operations on the input (template parameter LOADOP of the product kernel)
+constexpr int LOAD_NONE = 0, LOAD_AXPY = 1, LOAD_MUL = 2;
 // tile variants: 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads)
 constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2;
 
@@ -40,8 +42,9 @@ struct MatvecParams {
   T* d;                   // Chebyshev direction, updated in place (CHEB)
   const uint8_t* mask;    // per-DOF free flags (MASK_ARRAY)
   double* partial;        // per-tile FP64 partial sums (DOT: u.Au; RESID: |y|^2), length >= n_tiles
-  // load-time combination u <- u + s w (s = sdev[sidx], a device scalar); the combined input
-  // is written to u_out at the owned nodes (DOT and RESID passes of the Krylov loops)
+  // load-time combination, the combined input written to u_out at the owned nodes:
+  // LOAD_AXPY u <- u + s w (s = sdev[sidx], a device scalar; DOT and RESID passes of the Krylov
+  // loops), LOAD_MUL u <- load_scale * u (.) w (CHEB3: the first Chebyshev step from zero)
   const void* w;
   const double* sdev;
   int sidx;
@@ -54,7 +57,7 @@ struct MatvecParams {
   int flags;              // MV_PREV_ZERO: CHEB3's previous iterate is zero (y not read)
   int fixed_zero;         // with a mask: the output is 0 at fixed DOFs (M K M u, (b - A u) M)
   int tiles_y, tiles_z, chunks_x, n_tiles;
-  T omega, cheb_a, cheb_c;
+  T omega, cheb_a, cheb_c, load_scale;
   T kd[3];                // diagonal of ke per component (same at every corner of a brick): with
                           // dinv == nullptr, D^-1 = 1 / (kd[c] * sum of the incident moduli)
   T coef[72];             // parity blocks C[g][i][j], or [p-q, q, a, b, c, d, e-f, f] (cube)

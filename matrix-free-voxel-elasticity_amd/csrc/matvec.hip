@@ -107,7 +107,7 @@ __device__ __forceinline__ void element_blocks(const T (&v)[8][3], T Ee, const M
 
 // One workgroup's share of the product: tile `logical` (y tile fastest, then z tile, then x
 // chunk) of the grid described by `a`.
-template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE>
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP>
 __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int logical) {
   constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
   constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
@@ -152,7 +152,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   const S* __restrict__ u = reinterpret_cast<const S*>(a.u);
   const S* __restrict__ wv = reinterpret_cast<const S*>(a.w);
   const T* __restrict__ E = a.E;
-  const T sw = COMBINE ? T(a.sdev[a.sidx]) : T(0);
+  constexpr bool COMBINE = LOADOP != LOAD_NONE;
+  const T sw = LOADOP == LOAD_AXPY ? T(a.sdev[a.sidx]) : (LOADOP == LOAD_MUL ? a.load_scale : T(0));
 
   // ---- loader geometry (plane independent): this thread's LOADS slots of the flat plane
   // image (NY rows of 3*NZ contiguous values), their in-plane DOF offsets and validity.
@@ -213,7 +214,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int j = 0; j < LOADS; ++j) {
       bool keep = (pre_keep >> j) & 1u;
       if (MASK == MASK_ARRAY) keep = keep && pre_m[j] != 0;
-      const T v = COMBINE ? pre[j] + sw * pre_w[j] : pre[j];
+      const T v = LOADOP == LOAD_AXPY ? pre[j] + sw * pre_w[j]
+                  : (LOADOP == LOAD_MUL ? sw * (pre[j] * pre_w[j]) : pre[j]);
       if (tid + j * BLOCK < NLOAD) flat[tid + j * BLOCK] = keep ? v : T(0);
     }
   };
@@ -272,7 +274,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int c = 0; c < 3; ++c) {
       if (NEED_U && MASK == MASK_ARRAY) {       // the staged plane is masked: fetch the input
         ep_u[c] = load_as<S, T>(u, 3 * node + c);
-        if (COMBINE) ep_u[c] += sw * load_as<S, T>(wv, 3 * node + c);
+        if (LOADOP == LOAD_AXPY) ep_u[c] += sw * load_as<S, T>(wv, 3 * node + c);
+        if (LOADOP == LOAD_MUL) ep_u[c] = sw * (ep_u[c] * load_as<S, T>(wv, 3 * node + c));
       }
       if (NEED_B) ep_b[c] = a.b[3 * node + c];
       if (MODE == MODE_CHEB) ep_p[c] = a.d[3 * node + c];
@@ -422,16 +425,16 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 // planes) are mostly served by the same L2.
 __device__ __forceinline__ int xcd_logical(int b, int grid) { return (b & 7) * (grid >> 3) + (b >> 3); }
 
-template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE = false>
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP = LOAD_NONE>
 __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   if (logical >= a.n_tiles) return;
-  xmarch_tile<T, S, MODE, MASK, TL, CUBE, COMBINE>(a, logical);
+  xmarch_tile<T, S, MODE, MASK, TL, CUBE, LOADOP>(a, logical);
 }
 
 // Batched product: problem i owns the logical tiles [offs[i], offs[i+1]) of one launch and
 // reads its parameters (pointers, grid, plan, epilogue scalars, coefficients) from descs[i].
-template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, bool COMBINE = false>
+template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP = LOAD_NONE>
 __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecParams<T>* __restrict__ descs,
                                                                  const int* __restrict__ offs, int n) {
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
@@ -442,7 +445,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecPar
     if (offs[mid] <= logical) lo = mid; else hi = mid;
   }
   lo = __builtin_amdgcn_readfirstlane(lo);   // uniform: the descriptor is read with scalar loads
-  xmarch_tile<T, S, MODE, MASK, TL, CUBE, COMBINE>(descs[lo], logical - offs[lo]);
+  xmarch_tile<T, S, MODE, MASK, TL, CUBE, LOADOP>(descs[lo], logical - offs[lo]);
 }
 
 // ------------------------------------------------------------------------------ launcher
@@ -544,12 +547,15 @@ MatvecPlan matvec_plan_f32(int storage, bool cube, int nx, int ny, int nz, int l
 
 template <typename T, typename S, int MODE, int MASK>
 static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
-  if constexpr (MODE == MODE_DOT || MODE == MODE_RESID) {
-    if (p.w) {     // load-time combination (voxel-cube blocks only; checked by the caller)
+  // load-time combination (voxel-cube blocks only; checked by the caller): AXPY for the Krylov
+  // passes, MUL for the first Chebyshev step
+  constexpr int LOP = (MODE == MODE_DOT || MODE == MODE_RESID) ? LOAD_AXPY : (MODE == MODE_CHEB3 ? LOAD_MUL : LOAD_NONE);
+  if constexpr (LOP != LOAD_NONE) {
+    if (p.w) {
       if (p.tile == MV_TILE_8x32)
-        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true, true>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
+        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true, LOP>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
       else
-        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, true, true>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
+        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, true, LOP>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
       return;
     }
   }
@@ -614,35 +620,38 @@ template int matvec_launch<float>(MatvecParams<float>, int, int, int, hipStream_
 
 // ------------------------------------------------------------------------ batched launcher
 // Instantiated for the voxel-cube coefficients and the passes of the batched solvers: RAW,
-// RESID and DOT (with or without the load-time combination), JACOBI and CHEB3, masks X0 and
-// ARRAY, both tiles.
+// RESID and DOT (with or without the AXPY load), JACOBI, CHEB3 (with or without the MUL load),
+// masks X0 and ARRAY, both tiles.
 #if defined(VH_MATVEC_BATCH_F64) || defined(VH_MATVEC_BATCH_F32)
-template <typename T, int MODE, int MASK, bool COMBINE>
+template <typename T, int MODE, int MASK, int LOADOP>
 static void batch_tile(const MatvecBatch& bt, hipStream_t st) {
   const auto* d = static_cast<const MatvecParams<T>*>(bt.descs);
   const int grid = ((bt.total + 7) / 8) * 8;
   if (bt.tile == MV_TILE_8x32)
-    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile8, true, COMBINE>), dim3(grid), dim3(Tile8::BLOCK), 0, st,
+    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile8, true, LOADOP>), dim3(grid), dim3(Tile8::BLOCK), 0, st,
                        d, bt.offs, bt.n);
   else
-    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile16, true, COMBINE>), dim3(grid), dim3(Tile16::BLOCK), 0,
+    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile16, true, LOADOP>), dim3(grid), dim3(Tile16::BLOCK), 0,
                        st, d, bt.offs, bt.n);
 }
 
 template <typename T, int MASK>
 static int batch_mode(const MatvecBatch& bt, hipStream_t st) {
   switch (bt.mode) {
-    case MODE_RAW: batch_tile<T, MODE_RAW, MASK, false>(bt, st); break;
+    case MODE_RAW: batch_tile<T, MODE_RAW, MASK, LOAD_NONE>(bt, st); break;
     case MODE_RESID:
-      if (bt.combine) batch_tile<T, MODE_RESID, MASK, true>(bt, st);
-      else batch_tile<T, MODE_RESID, MASK, false>(bt, st);
+      if (bt.combine) batch_tile<T, MODE_RESID, MASK, LOAD_AXPY>(bt, st);
+      else batch_tile<T, MODE_RESID, MASK, LOAD_NONE>(bt, st);
       break;
     case MODE_DOT:
-      if (bt.combine) batch_tile<T, MODE_DOT, MASK, true>(bt, st);
-      else batch_tile<T, MODE_DOT, MASK, false>(bt, st);
+      if (bt.combine) batch_tile<T, MODE_DOT, MASK, LOAD_AXPY>(bt, st);
+      else batch_tile<T, MODE_DOT, MASK, LOAD_NONE>(bt, st);
       break;
-    case MODE_JACOBI: batch_tile<T, MODE_JACOBI, MASK, false>(bt, st); break;
-    case MODE_CHEB3: batch_tile<T, MODE_CHEB3, MASK, false>(bt, st); break;
+    case MODE_JACOBI: batch_tile<T, MODE_JACOBI, MASK, LOAD_NONE>(bt, st); break;
+    case MODE_CHEB3:
+      if (bt.combine) batch_tile<T, MODE_CHEB3, MASK, LOAD_MUL>(bt, st);
+      else batch_tile<T, MODE_CHEB3, MASK, LOAD_NONE>(bt, st);
+      break;
     default: return fail(-2, "batched product: mode %d not available", bt.mode);
   }
   return hip_status(hipGetLastError(), "parity_xmarch_batch launch");
@@ -662,7 +671,7 @@ static int resident_batch() {
   static int occ = 0;
   if (occ == 0) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch_batch<T, T, MODE_RAW, MASK_X0, TL, true, false>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch_batch<T, T, MODE_RAW, MASK_X0, TL, true, LOAD_NONE>,
                                                     TL::BLOCK, 0) != hipSuccess || n <= 0)
       n = 1;
     occ = n;

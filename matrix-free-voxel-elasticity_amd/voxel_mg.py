@@ -73,6 +73,9 @@ class HipProduct:
         self.kdiag = d[0].copy() if np.allclose(d, d[0], rtol=1e-13, atol=0) else None
         self.ndof = 3 * ops.n_nodes(self.shape)
         self.storage = L.BF16 if storage == "bf16" else None
+        hb = np.ascontiguousarray(np.asarray(self.blocks, dtype=np.float64).reshape(72))
+        # voxel-cube coefficients: the fused load-time combinations are available
+        self.cube = bool(L.load().vh_blocks_are_cube(hb.ctypes.data))
 
     def raw(self, u, E, out=None, mask=None, fixed_zero=False):
         y = out if out is not None else torch.empty(self.ndof, dtype=self.dtype, device=u.device)
@@ -268,14 +271,24 @@ class Multigrid:
             # the pass overwrites x_{k-1} (held by the ping-pong partner) with x_{k+1}
             theta, delta = _cheb_params(self.lambdas[level], self.lower_frac)
             prev_zero = False
-            if first:        # x0 = 0: x1 = D^-1 r / theta, and x_{k-1} = 0 at the next step
+            alpha = 2.0 / theta if self.cheb_variant == "saad" else 1.0 / theta
+            start = 1
+            if first and Lv.product.cube and sweeps > 1 and self.cheb_variant == "saad":
+                # x0 = 0: steps 1 and 2 in one pass, x1 = D^-1 r / theta formed at load time (and
+                # written to x, the next step's previous iterate), x2 into the partner buffer
+                c = 0.25 * delta * delta * alpha
+                alpha = 1.0 / (theta - c)
+                Lv.matvec(r, other, mode=L.MODE_CHEB3, b=r, cheb_a=alpha, cheb_c=c, prev_zero=True,
+                          w=Lv.inv_diag, load_scale=1.0 / theta, u_out=x, **dk)
+                x, other = other, x
+                start = 2
+            elif first:      # x0 = 0: x1 = D^-1 r / theta, and x_{k-1} = 0 at the next step
                 ops.axpby(1.0 / theta, r, 0.0, x, w=Lv.inv_diag)
                 prev_zero = True
             else:
                 Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
                 x, other = other, x
-            alpha = 2.0 / theta if self.cheb_variant == "saad" else 1.0 / theta
-            for k in range(1, sweeps):
+            for k in range(start, sweeps):
                 c = 0.25 * delta * delta * alpha
                 alpha = 1.0 / (theta - c)
                 cd = c * theta if (self.cheb_variant == "donor" and k == 1) else c

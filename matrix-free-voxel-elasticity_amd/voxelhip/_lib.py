@@ -10,6 +10,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libvoxelhip.so"
+ABI_VERSION = 2                 # VH_ABI_VERSION of include/voxelhip.h these bindings follow
 
 F64, F32, BF16 = 0, 1, 2
 MASK_NONE, MASK_ARRAY, MASK_X0 = 0, 1, 2
@@ -31,7 +32,7 @@ class Epilogue(ctypes.Structure):
     _fields_ = [("mode", _i), ("fixed_zero", _i), ("b", _vp), ("dinv", _vp), ("d", _vp),
                 ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i), ("tile", _i),
                 ("kdiag", _d * 3), ("flags", _i), ("w", _vp), ("sdev", _vp), ("sidx", _i),
-                ("u_out", _vp), ("y32", _vp)]
+                ("u_out", _vp), ("y32", _vp), ("load_scale", _d)]
 
 
 class ProductItem(ctypes.Structure):
@@ -51,6 +52,7 @@ class DenseItem(ctypes.Structure):
 # name -> argtypes (all return int status unless listed in _RESTYPES)
 SIGNATURES = {
     "vh_abi_version": [],
+    "vh_blocks_are_cube": [_vp],
     "vh_last_error": [],
     "vh_modal_blocks": [_vp, _d, _vp, _dp],
     "vh_matvec": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp],
@@ -108,6 +110,8 @@ def load(path=None):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = _RESTYPES.get(name, _i)
+    if lib.vh_abi_version() != ABI_VERSION:
+        raise VoxelHipError(f"{p} has ABI {lib.vh_abi_version()}, these bindings need {ABI_VERSION}: rebuild it")
     _LIB = lib
     return lib
 

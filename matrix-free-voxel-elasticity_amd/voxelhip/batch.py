@@ -97,6 +97,7 @@ def _epilogue(mode, **kw):
     for name in ("b", "dinv", "d", "partial", "w", "sdev", "u_out", "y32"):
         setattr(ep, name, kw.pop(name, None))
     ep.omega = float(kw.pop("omega", 0.0))
+    ep.load_scale = float(kw.pop("load_scale", 1.0))
     ep.cheb_a = float(kw.pop("cheb_a", 0.0))
     ep.cheb_c = float(kw.pop("cheb_c", 0.0))
     ep.lx = int(kw.pop("lx", 0))
@@ -138,6 +139,8 @@ class BatchedGMGSolver:
                  graph=True, lower_frac=1.0 / 30.0, omega_iterations=30, tile=L.TILE_16x32, lx=0):
         if precision not in ("fp64", "mixed"):
             raise ValueError(precision)
+        if int(degree) < 2:
+            raise ValueError("the batched Chebyshev smoother needs degree >= 2")
         self.dev = torch.device("cuda", torch.cuda.current_device())
         self.ke = np.asarray(ke, dtype=np.float64)
         self.precision = precision
@@ -254,7 +257,8 @@ class BatchedGMGSolver:
             o = lambda k: self.off[d][0][k]  # noqa: E731
             bufs = {"X": self.X[d], "X2": self.X2[d]}
             steps = []
-            # pre: X = D^-1 r / theta (elementwise, whole depth buffer), then CHEB3 steps
+            # pre: steps 1 and 2 in one pass (x1 = D^-1 r / theta formed at load time from the
+            # rhs and the scaled inverse diagonal DT, written to X), then the CHEB3 recurrence
             cur, oth = "X", "X2"
             alpha = {k: None for k in ks}
             th = {k: mg._cheb_params(self._lam(k, d), self.lower_frac) for k in ks}
@@ -266,9 +270,15 @@ class BatchedGMGSolver:
                     theta, delta = th[k]
                     c = 0.25 * delta * delta * alpha[k]
                     alpha[k] = 1.0 / (theta - c)
-                    items.append(self._item(lv(k), _ptr(bufs[cur], o(k)), _ptr(bufs[oth], o(k)), L.MODE_CHEB3,
-                                            b=_ptr(self.RHS[d], o(k)), cheb_a=alpha[k], cheb_c=c,
-                                            kdiag=lv(k).product.kdiag, flags=1 if s == 1 else 0))
+                    if s == 1:
+                        items.append(self._item(lv(k), _ptr(self.RHS[d], o(k)), _ptr(bufs[oth], o(k)),
+                                                L.MODE_CHEB3, b=_ptr(self.RHS[d], o(k)), cheb_a=alpha[k], cheb_c=c,
+                                                kdiag=lv(k).product.kdiag, flags=1, w=_ptr(self.DT[d], o(k)),
+                                                load_scale=1.0, u_out=_ptr(bufs[cur], o(k))))
+                    else:
+                        items.append(self._item(lv(k), _ptr(bufs[cur], o(k)), _ptr(bufs[oth], o(k)), L.MODE_CHEB3,
+                                                b=_ptr(self.RHS[d], o(k)), cheb_a=alpha[k], cheb_c=c,
+                                                kdiag=lv(k).product.kdiag))
                 steps.append(product_batch(pd, mk, self.tile, items)[0])
                 cur, oth = oth, cur
             pre_out = cur
@@ -363,7 +373,6 @@ class BatchedGMGSolver:
     def vcycle(self):
         for d in range(self.D):
             v = self.vc[d]
-            ops.axpby(1.0, self.RHS[d], 0.0, self.X[d], w=self.DT[d])      # x1 = D^-1 r / theta
             for h in v["pre"]:
                 h.launch()
             v["resid"].launch()

@@ -119,7 +119,7 @@ class Mask:
 # --------------------------------------------------------------------------- operator
 def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None, d=None,
            partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None, tile=0,
-           kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None):
+           kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None, load_scale=1.0):
     """One pass of the parity product with a fused epilogue (see include/voxelhip.h)."""
     _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
     nx, ny, nz = (int(s) for s in shape)
@@ -142,7 +142,7 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
         raise ValueError("smoothing passes need dinv or kdiag")
     ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), _p(d), _p(partial),
                     float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, int(bool(prev_zero)),
-                    _p(w), _p(sdev), int(sidx), _p(u_out), _p(y32))
+                    _p(w), _p(sdev), int(sidx), _p(u_out), _p(y32), float(load_scale))
     L.call("vh_matvec_fused", dt, st, hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
            nx, ny, nz, m.kind, m.ptr, ctypes.byref(ep), _stream())
     return y
