@@ -307,7 +307,8 @@ int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_
              "vh_product_batch_create: combination needs DOT/RESID/CHEB3");
   for (int i = 0; i < n; ++i) {
     const vh_product_item& it = items[i];
-    VH_REQUIRE(it.blocks72 && it.E && it.u && it.y && it.u != it.y, "vh_product_batch_create: item %d pointers", i);
+    VH_REQUIRE(it.blocks72 && it.E && it.u && (it.y || mode == MODE_DOT) && it.u != it.y,
+               "vh_product_batch_create: item %d pointers", i);
     VH_REQUIRE(it.nx >= 1 && it.ny >= 1 && it.nz >= 1 && it.nz <= 2048 &&
                    3.0 * (it.nx + 1.0) * (it.ny + 1.0) * (it.nz + 1.0) < 2147483648.0,
                "vh_product_batch_create: item %d grid", i);

@@ -211,7 +211,7 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
                     int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep, void* stream) {
   VH_REQUIRE(dtype_ok(dtype), "vh_matvec: dtype %d not supported", dtype);
   VH_REQUIRE(storage == dtype || (storage == BF16 && dtype == F32), "vh_matvec: storage %d with dtype %d", storage, dtype);
-  VH_REQUIRE(blocks72 && E && u && y, "vh_matvec: null pointer");
+  VH_REQUIRE(blocks72 && E && u && (y || (ep && ep->mode == MODE_DOT)), "vh_matvec: null pointer");
   VH_REQUIRE(u != y, "vh_matvec: y must not alias u");
   VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec: bad grid %d x %d x %d", nx, ny, nz);
   VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2, "vh_matvec: bad mask kind %d", mask_kind);
@@ -229,7 +229,7 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
                           ep->u_out != ep->b),
                "vh_matvec: the load-time combination needs a distinct u_out");
     VH_REQUIRE(!ep->y32 || m == MODE_RESID, "vh_matvec: y32 is an output of the RESID pass");
-    VH_REQUIRE(ep->b != y && (ep->dinv == nullptr || ep->dinv != y), "vh_matvec: y must not alias an input");
+    VH_REQUIRE(!y || (ep->b != y && ep->dinv != y), "vh_matvec: y must not alias an input");
     if (smoother && ep->dinv == nullptr)   // D^-1 from the moduli and ke's diagonal
       VH_REQUIRE(ep->kdiag[0] > 0 && ep->kdiag[1] > 0 && ep->kdiag[2] > 0, "vh_matvec: kdiag missing");
   }

@@ -399,7 +399,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
           dot_acc += double(uo) * double(au);
         }
         if (MASK != MASK_NONE && a.fixed_zero && !free_dof) out = T(0);
-        a.y[dof] = out;
+        if (MODE != MODE_DOT || a.y) a.y[dof] = out;      // a DOT pass may keep only p.Ap
         if (COMBINE) a.u_out[dof] = uo;
         if (MODE == MODE_RESID && a.y32) a.y32[dof] = float(out);
       }

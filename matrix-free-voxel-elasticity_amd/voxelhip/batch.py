@@ -190,7 +190,7 @@ class BatchedGMGSolver:
         f64 = dict(dtype=torch.float64, device=dev)
         nf = self.off[0][1]
         z = lambda: torch.zeros(nf, **f64)  # noqa: E731
-        self.b, self.r, self.q = z(), z(), z()
+        self.b, self.r = z(), z()
         self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
         self.scal = torch.zeros(16 * n, **f64)
         pd = dict(dtype=self.pdtype, device=dev)
@@ -234,7 +234,7 @@ class BatchedGMGSolver:
                     fine = self.hier[p][1]
                     fo = self.off[0][0][k]
                     sdev = _ptr(self.scal, 16 * k)
-                    items_d.append(self._item(fine, _ptr(self.zs[i], fo), _ptr(self.q, fo), L.MODE_DOT,
+                    items_d.append(self._item(fine, _ptr(self.zs[i], fo), None, L.MODE_DOT,
                                               partial=base, w=_ptr(self.ps[o], fo), sdev=sdev, sidx=L.S_BETA,
                                               u_out=_ptr(self.ps[i], fo)))
                     y32 = _ptr(self.RHS[0], fo) if self.pdtype != f64 else None

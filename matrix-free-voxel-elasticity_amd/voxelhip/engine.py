@@ -114,7 +114,7 @@ class GMGSolver:
     def _alloc(self):
         n = self.fine.n
         z = lambda dt=torch.float64: torch.zeros(n, dtype=dt, device=self.dev)  # noqa: E731
-        self.b, self.r, self.q = z(), z(), z()
+        self.b, self.r = z(), z()
         self.parts = ops.reduce_parts(n)
         # per-tile partial counts of the fused passes (they depend on each pass's launch plan)
         f, f32 = self.fine, self.M.levels[0]
@@ -133,7 +133,7 @@ class GMGSolver:
         if self.precision == "ir":
             f32 = torch.float32
             self.x = z()
-            self.b32, self.q32 = z(f32), z(f32)
+            self.b32 = z(f32)
             self.ds, self.ps, self.zs = [z(f32), z(f32)], [z(f32), z(f32)], [z(f32), z(f32)]
         else:
             self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
@@ -142,13 +142,13 @@ class GMGSolver:
     def _iteration(self, i):
         """Launch flexible-CG iteration with buffer parity i (no host synchronisation).
 
-        DOT pass:   p_i = z_i + beta p_o (read on the fly), q = A p_i, p_i.q     -> alpha
+        DOT pass:   p_i = z_i + beta p_o (read on the fly), p_i.A p_i (A p_i not stored) -> alpha
         RESID pass: x_i = x_o + alpha p_i (on the fly), r = b - A x_i, |r|^2, r -> fp32
         V-cycle on r, then z_o = its output with r.z_o and r.(z_o - z_i)          -> beta, r.z
         """
         o = 1 - i
         f, L_ = self.fine, L
-        f.matvec(self.zs[i], self.q, mode=L_.MODE_DOT, partial=self.partial,
+        f.matvec(self.zs[i], None, mode=L_.MODE_DOT, partial=self.partial,
                  w=self.ps[o], sdev=self.scal, sidx=L_.S_BETA, u_out=self.ps[i])
         ops.finish(self.partial, self.t_dot, 1, self.scal, L_.S_PQ, L_.OP_ALPHA)
         f.matvec(self.xs[o], self.r, mode=L_.MODE_RESID, b=self.b, partial=self.partial,
@@ -164,7 +164,7 @@ class GMGSolver:
         the same pass structure as _iteration with FP32 vectors and FP64 scalars."""
         o = 1 - i
         f = self.M.levels[0]
-        f.matvec(self.zs[i], self.q32, mode=L.MODE_DOT, partial=self.partial,
+        f.matvec(self.zs[i], None, mode=L.MODE_DOT, partial=self.partial,
                  w=self.ps[o], sdev=self.scal, sidx=L.S_BETA, u_out=self.ps[i])
         ops.finish(self.partial, self.t_dot32, 1, self.scal, L.S_PQ, L.OP_ALPHA)
         f.matvec(self.ds[o], self.rhs0, mode=L.MODE_RESID, b=self.b32, partial=self.partial,

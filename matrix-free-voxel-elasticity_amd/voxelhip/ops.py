@@ -121,19 +121,23 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
            partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None, tile=0,
            kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None, load_scale=1.0):
     """One pass of the parity product with a fused epilogue (see include/voxelhip.h)."""
-    _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
+    _dev(E, "E"); _dev(u, "u")
+    if y is None and mode != L.MODE_DOT:
+        raise ValueError("only a DOT pass may omit y")
+    if y is not None:
+        _dev(y, "y")
     nx, ny, nz = (int(s) for s in shape)
-    dt = dtype_code(y)
+    dt = dtype_code(E)
     st = dt if storage is None else storage
-    if E.dtype != y.dtype:
+    if y is not None and E.dtype != y.dtype:
         raise TypeError("E and y must share the compute dtype")
     if st == L.BF16:
         if u.dtype not in (torch.bfloat16, torch.int16, torch.uint16):
             raise TypeError("bf16 storage expects a bfloat16 u")
-    elif u.dtype != y.dtype:
+    elif u.dtype != E.dtype:
         raise TypeError("u and y must share dtype")
     nd = 3 * n_nodes(shape)
-    if u.numel() != nd or y.numel() != nd or E.numel() != nx * ny * nz:
+    if u.numel() != nd or (y is not None and y.numel() != nd) or E.numel() != nx * ny * nz:
         raise ValueError(f"sizes do not match grid {shape}")
     m = mask or Mask.none()
     hb = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
