@@ -136,6 +136,24 @@ int vh_prolong(int dtype, const void* ec, void* xf, int nx, int ny, int nz, int 
 /* Ec = 2x2x2 mean of Ef.  Replaces: coarsen_modulus src/voxel_mg.py:108-111. */
 int vh_coarsen_modulus(int dtype, const void* Ef, void* Ec, int nx, int ny, int nz, void* stream);
 
+/* Dense constrained operator A = M K M + (I - M) of a small grid (FP64, device arrays: ke the
+ * 24x24 element matrix in its native node order, E the element moduli, A row-major n x n with
+ * row stride ld >= n = 3 (nx+1)(ny+1)(nz+1); every entry is written).  K_ij sums E_e ke over the
+ * shared elements in ascending element order.
+ * Replaces: assemble_dense src/voxel_mg.py:61-70 (host scipy assembly of the coarsest operator,
+ * re-run at every density update). */
+int vh_assemble_dense(const double* ke, const double* E, int nx, int ny, int nz, int mask_kind, const uint8_t* mask,
+                      double* A, int ld, void* stream);
+/* Galerkin coarse operator M_c (P^T A_f P) M_c + (I - M_c) from 81 probe products (FP64 device
+ * arrays; (nx, ny, nz) the COARSE element grid, n its DOF count): row p = ((px*3+py)*3+pz)*3+c of
+ * G (81 x n) holds R A_f P v_p for v_p = the sum of the unit vectors e_(J, c) over the coarse
+ * nodes J with (Jx, Jy, Jz) = (px, py, pz) mod 3.  Writes the entries of each row's 27-node
+ * neighbourhood; A (row stride ld >= n) must be zero on entry (all other entries are 0).
+ * Replaces: the Galerkin triple product assemble_level1_galerkin multigrid_v4.py:248-270,747-772
+ * (host sparse products). */
+int vh_galerkin_scatter(const double* G, int nx, int ny, int nz, int mask_kind, const uint8_t* mask, double* A, int ld,
+                        void* stream);
+
 /* y = mask * (A r) for a dense row-major n x n matrix with leading dimension ld (ld >= n,
  * ld % 4 == 0, A 16-byte aligned; the padding columns are not read): the coarsest solve with the
  * explicit inverse of the coarsest operator.  a_dtype is the matrix type, v_dtype the type of r

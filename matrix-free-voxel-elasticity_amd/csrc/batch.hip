@@ -9,7 +9,7 @@
 //                     the Polak-Ribiere pass, one partial per workgroup
 //   segmented finish  one workgroup per problem sums its partials and runs the scalar
 //                     recurrence on its own 16-slot scalar block
-//   transfer batch    restriction or prolongation, one workgroup per z-line of any problem
+//   transfer batch    restriction or prolongation, one workgroup per row block of any problem
 //   dense batch       the coarsest solves (explicit inverses of different orders)
 #include <algorithm>
 #include <cstring>
@@ -133,17 +133,20 @@ struct TransferDesc {
 };
 
 template <typename T, bool RESTRICT>
-__global__ void __launch_bounds__(256) transfer_batch_kernel(const TransferDesc<T>* __restrict__ d,
-                                                             const int* __restrict__ offs, int n, int assign) {
+__global__ void __launch_bounds__(TR_THREADS) transfer_batch_kernel(const TransferDesc<T>* __restrict__ d,
+                                                                    const int* __restrict__ offs, int n, int assign) {
   extern __shared__ unsigned char smem_raw[];
-  const int i = find_segment(offs, n, blockIdx.x);
+  const int b = xcd_logical(blockIdx.x, gridDim.x);
+  if (b >= offs[n]) return;
+  const int i = find_segment(offs, n, b);
   const TransferDesc<T>& t = d[i];
-  const int L = blockIdx.x - offs[i];
+  const int blk = b - offs[i];
   if (RESTRICT)
-    restrict_line<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, t.cmask, t.cmask_kind, L,
-                     reinterpret_cast<T*>(smem_raw));
+    restrict_block<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, t.cmask, t.cmask_kind, blk,
+                      reinterpret_cast<T*>(smem_raw));
   else
-    prolong_line<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, assign, L, reinterpret_cast<T*>(smem_raw));
+    prolong_block<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, assign, blk,
+                     reinterpret_cast<T*>(smem_raw));
 }
 
 // ------------------------------------------------------------------------------ dense solves
@@ -257,16 +260,16 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
 template <typename T>
 static int transfer_create(int n, const vh_transfer_item* items, BatchHandle* h, std::vector<int>& prefix) {
   std::vector<TransferDesc<T>> descs(n);
-  int maxz = 0;
+  size_t lds = 0;
   for (int i = 0; i < n; ++i) {
     const vh_transfer_item& it = items[i];
     descs[i] = TransferDesc<T>{static_cast<const T*>(it.src), static_cast<T*>(it.dst), it.nx, it.ny, it.nz,
                                it.fmask_kind, it.cmask_kind, it.fmask, it.cmask};
-    const int lines = h->restrict_op ? (it.nx / 2 + 1) * (it.ny / 2 + 1) : (it.nx + 1) * (it.ny + 1);
-    prefix[i + 1] = prefix[i] + lines;
-    maxz = std::max(maxz, it.nz);
+    const TransferGeom g = h->restrict_op ? restrict_geom(it.nx, it.ny, it.nz) : prolong_geom(it.nx, it.ny, it.nz);
+    prefix[i + 1] = prefix[i] + g.nblocks;
+    lds = std::max(lds, g.lds);
   }
-  h->lds = size_t(3) * (maxz + 1) * sizeof(T);
+  h->lds = lds * sizeof(T);
   return upload(&h->descs, descs.data(), descs.size() * sizeof(TransferDesc<T>));
 }
 
@@ -414,7 +417,7 @@ int vh_batch_launch(void* handle, void* stream) {
       return h->dtype == F64 ? matvec_batch_launch<double>(b, st) : matvec_batch_launch<float>(b, st);
     }
     case BK_TRANSFER: {
-      const dim3 g(h->total), blk(256);
+      const dim3 g(unsigned((h->total + 7) / 8) * 8), blk(TR_THREADS);
       if (h->dtype == F64) {
         const auto* d = static_cast<const TransferDesc<double>*>(h->descs);
         if (h->restrict_op) hipLaunchKernelGGL((transfer_batch_kernel<double, true>), g, blk, h->lds, st, d, h->offs, h->n, h->assign);

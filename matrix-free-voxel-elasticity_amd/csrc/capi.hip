@@ -105,8 +105,7 @@ bool cube_pattern(const double* B, double* k8) {
   return true;
 }
 
-// every kernel indexes DOFs with 32-bit counts below 2^31; the transfer kernels stage one z-line
-// of 3 (nz + 1) values in LDS
+// every kernel indexes DOFs with 32-bit counts below 2^31; the z extent is capped at 2048 elements
 static bool dims_ok(int nx, int ny, int nz) {
   return nx >= 1 && ny >= 1 && nz >= 1 && nz <= 2048 &&
          3.0 * (nx + 1.0) * (ny + 1.0) * (nz + 1.0) < 2147483648.0;
@@ -326,6 +325,24 @@ int vh_coarsen_modulus(int dtype, const void* Ef, void* Ec, int nx, int ny, int 
   if (dtype == F64)
     return coarsen_modulus_launch<double>(static_cast<const double*>(Ef), static_cast<double*>(Ec), nx, ny, nz, st);
   return coarsen_modulus_launch<float>(static_cast<const float*>(Ef), static_cast<float*>(Ec), nx, ny, nz, st);
+}
+
+int vh_assemble_dense(const double* ke, const double* E, int nx, int ny, int nz, int mask_kind, const uint8_t* mask,
+                      double* A, int ld, void* stream) {
+  VH_REQUIRE(ke && E && A && dims_ok(nx, ny, nz), "vh_assemble_dense: bad arguments");
+  const double n = 3.0 * (nx + 1.0) * (ny + 1.0) * (nz + 1.0);
+  VH_REQUIRE(n * n < 4.0e9 && ld >= n, "vh_assemble_dense: %.0f DOFs or ld %d out of range", n, ld);
+  VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_assemble_dense: mask missing");
+  return assemble_dense_launch(ke, E, nx, ny, nz, mask_kind, mask, A, ld, static_cast<hipStream_t>(stream));
+}
+
+int vh_galerkin_scatter(const double* G, int nx, int ny, int nz, int mask_kind, const uint8_t* mask, double* A, int ld,
+                        void* stream) {
+  VH_REQUIRE(G && A && G != A && dims_ok(nx, ny, nz), "vh_galerkin_scatter: bad arguments");
+  const double n = 3.0 * (nx + 1.0) * (ny + 1.0) * (nz + 1.0);
+  VH_REQUIRE(81.0 * n < 2.0e9 && ld >= n, "vh_galerkin_scatter: %.0f DOFs or ld %d out of range", n, ld);
+  VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_galerkin_scatter: mask missing");
+  return galerkin_scatter_launch(G, nx, ny, nz, mask_kind, mask, A, ld, static_cast<hipStream_t>(stream));
 }
 
 int vh_dense_mv(int a_dtype, int v_dtype, int n, int ld, const void* A, const void* r, void* y, const uint8_t* mask,

@@ -48,6 +48,11 @@ template <typename T, int BLOCK> __device__ __forceinline__ T block_sum(T v, T* 
   return s;
 }
 
+// XCD-aware work decode for grids that are multiples of 8: the hardware assigns block b to XCD
+// b % 8; XCD k gets the contiguous logical range [k*per, (k+1)*per), so neighbouring work units
+// (shared halo rows, columns and planes) are mostly served by the same L2.
+__device__ __forceinline__ int xcd_logical(int b, int grid) { return (b & 7) * (grid >> 3) + (b >> 3); }
+
 // Dtype codes shared with the Python side.
 enum Dtype : int { F64 = 0, F32 = 1, BF16 = 2 };
 // Mask kinds: no mask (raw product), a per-DOF uint8 array (1 = free), the x = 0 clamp.

@@ -149,23 +149,27 @@ __global__ void __launch_bounds__(256) dense_node_kernel(const T* __restrict__ u
 }
 
 // ------------------------------------------------------------------ transfers
-// One workgroup per z-line (restrict_line / prolong_line in grid_ops.h).
+// One workgroup per block (restrict_block / prolong_block in grid_ops.h), XCD-aware order so
+// that the blocks sharing fine planes (restriction) or coarse rows (prolongation) meet in one L2.
 template <typename T>
-__global__ void __launch_bounds__(256) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
-                                                      const GridArgs<T> g, const uint8_t* __restrict__ cmask,
-                                                      int cmask_kind) {
+__global__ void __launch_bounds__(TR_THREADS) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
+                                                             const GridArgs<T> g, const uint8_t* __restrict__ cmask,
+                                                             int cmask_kind, int nblocks) {
   extern __shared__ unsigned char smem_raw[];
-  restrict_line<T>(rf, rc, g.nx, g.ny, g.nz, g.mask_kind, g.mask, cmask, cmask_kind, int(blockIdx.x),
-                   reinterpret_cast<T*>(smem_raw));
+  const int blk = xcd_logical(blockIdx.x, gridDim.x);
+  if (blk >= nblocks) return;
+  restrict_block<T>(rf, rc, g.nx, g.ny, g.nz, g.mask_kind, g.mask, cmask, cmask_kind, blk,
+                    reinterpret_cast<T*>(smem_raw));
 }
 
 // prolong: xf (+)= mask_f * P ec.
 template <typename T>
-__global__ void __launch_bounds__(256) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
-                                                     const GridArgs<T> g, int assign) {
+__global__ void __launch_bounds__(TR_THREADS) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
+                                                            const GridArgs<T> g, int assign, int nblocks) {
   extern __shared__ unsigned char smem_raw[];
-  prolong_line<T>(ec, xf, g.nx, g.ny, g.nz, g.mask_kind, g.mask, assign, int(blockIdx.x),
-                  reinterpret_cast<T*>(smem_raw));
+  const int blk = xcd_logical(blockIdx.x, gridDim.x);
+  if (blk >= nblocks) return;
+  prolong_block<T>(ec, xf, g.nx, g.ny, g.nz, g.mask_kind, g.mask, assign, blk, reinterpret_cast<T*>(smem_raw));
 }
 
 // 2x2x2 mean of element moduli (src/voxel_mg.py:108-111); g = fine grid.
@@ -182,6 +186,71 @@ __global__ void coarsen_modulus_kernel(const T* __restrict__ Ef, T* __restrict__
       for (int c = 0; c < 2; ++c)
         s += Ef[(size_t(2 * ex + a) * ny + 2 * ey + b) * nz + 2 * ez + c];
   Ec[e] = s / T(8);
+}
+
+// ------------------------------------------------------------------ dense coarsest operators
+// Free flag of DOF i under a mask (X0: every DOF of the plane ix = 0 is fixed).
+__device__ __forceinline__ bool dof_free(int mask_kind, const uint8_t* mask, int i, int plane_dofs) {
+  if (mask_kind == MASK_ARRAY) return mask[i] != 0;
+  if (mask_kind == MASK_X0) return i >= plane_dofs;
+  return true;
+}
+
+// A = M K M + (I - M) of a small grid, one thread per entry (i, j) of the n x n matrix (row
+// stride ld): K_ij sums E_e ke[(I in e, a), (J in e, b)] over the elements e shared by nodes I
+// and J, in ascending element order (src/voxel_mg.py:61-70).
+__global__ void __launch_bounds__(256) assemble_dense_kernel(const double* __restrict__ ke,
+                                                             const double* __restrict__ E, int nx, int ny, int nz,
+                                                             int mask_kind, const uint8_t* __restrict__ mask,
+                                                             double* __restrict__ A, int ld) {
+#pragma clang fp contract(off)   // E_e * ke rounded before the sum, as an assembly on the host does
+  const int ny1 = ny + 1, nz1 = nz + 1;
+  const int n = 3 * (nx + 1) * ny1 * nz1;
+  const size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (t >= size_t(n) * n) return;
+  const int i = int(t / unsigned(n)), j = int(t - size_t(i) * n);
+  const int I = i / 3, a = i - 3 * I, J = j / 3, b = j - 3 * J;
+  const int iz = I % nz1, iy = (I / nz1) % ny1, ix = I / (nz1 * ny1);
+  const int jz = J % nz1, jy = (J / nz1) % ny1, jx = J / (nz1 * ny1);
+  double k = 0.0;
+  if (abs(ix - jx) <= 1 && abs(iy - jy) <= 1 && abs(iz - jz) <= 1) {
+    for (int ex = max(max(ix, jx) - 1, 0); ex <= min(min(ix, jx), nx - 1); ++ex)
+      for (int ey = max(max(iy, jy) - 1, 0); ey <= min(min(iy, jy), ny - 1); ++ey)
+        for (int ez = max(max(iz, jz) - 1, 0); ez <= min(min(iz, jz), nz - 1); ++ez) {
+          const int li = native_of(ix - ex, iy - ey, iz - ez), lj = native_of(jx - ex, jy - ey, jz - ez);
+          k += E[(size_t(ex) * ny + ey) * nz + ez] * ke[(3 * li + a) * 24 + 3 * lj + b];
+        }
+  }
+  const int plane = 3 * ny1 * nz1;
+  const bool fi = dof_free(mask_kind, mask, i, plane), fj = dof_free(mask_kind, mask, j, plane);
+  A[size_t(i) * ld + j] = (fi && fj) ? k : ((i == j && !fi) ? 1.0 : 0.0);
+}
+
+// Galerkin coarse operator from 81 probe products: row p = ((px * 3 + py) * 3 + pz) * 3 + c of G
+// holds R A_f P v_p for the probe v_p = sum of e_(J, c) over the coarse nodes J = (px, py, pz)
+// mod 3.  P^T A_f P couples only nodes within one node of each other, so entry (I, a) of row p
+// belongs to the column of the one J = p's residues within one node of I.  Writes
+// M_c A_c M_c + (I - M_c) into those entries (A zeroed beforehand: every other entry is 0).
+__global__ void __launch_bounds__(256) galerkin_scatter_kernel(const double* __restrict__ G, int nx, int ny, int nz,
+                                                               int mask_kind, const uint8_t* __restrict__ mask,
+                                                               double* __restrict__ A, int ld) {
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  const int n = 3 * nx1 * ny1 * nz1;
+  const size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (t >= size_t(81) * n) return;
+  const int p = int(t / unsigned(n)), i = int(t - size_t(p) * n);
+  const int c = p % 3, pz = (p / 3) % 3, py = (p / 9) % 3, px = p / 27;
+  const int I = i / 3;
+  const int iz = I % nz1, iy = (I / nz1) % ny1, ix = I / (nz1 * ny1);
+  // the J in {I - 1, I, I + 1} with J = residue (mod 3), per axis
+  const int jx = ix - 1 + (px - ix + 1 + 3 * nx1) % 3;
+  const int jy = iy - 1 + (py - iy + 1 + 3 * ny1) % 3;
+  const int jz = iz - 1 + (pz - iz + 1 + 3 * nz1) % 3;
+  if (jx < 0 || jx >= nx1 || jy < 0 || jy >= ny1 || jz < 0 || jz >= nz1) return;
+  const int j = 3 * ((jx * ny1 + jy) * nz1 + jz) + c;
+  const int plane = 3 * ny1 * nz1;
+  const bool fi = dof_free(mask_kind, mask, i, plane), fj = dof_free(mask_kind, mask, j, plane);
+  A[size_t(i) * ld + j] = (fi && fj) ? G[size_t(p) * n + i] : ((i == j && !fi) ? 1.0 : 0.0);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -215,18 +284,36 @@ int dense_launch(const T* u, const T* E, const T* ke, T* y, int nx, int ny, int 
 
 template <typename T>
 int restrict_launch(const T* rf, T* rc, const GridArgs<T>& g, const uint8_t* cmask, int cmask_kind, hipStream_t st) {
-  const unsigned lines = unsigned(g.nx / 2 + 1) * unsigned(g.ny / 2 + 1);
-  const size_t lds = size_t(3) * (g.nz + 1) * sizeof(T);
-  hipLaunchKernelGGL(restrict_kernel<T>, dim3(lines), dim3(256), lds, st, rf, rc, g, cmask, cmask_kind);
+  const TransferGeom tg = restrict_geom(g.nx, g.ny, g.nz);
+  const unsigned grid = unsigned((tg.nblocks + 7) / 8) * 8;
+  hipLaunchKernelGGL(restrict_kernel<T>, dim3(grid), dim3(TR_THREADS), tg.lds * sizeof(T), st, rf, rc, g, cmask,
+                     cmask_kind, tg.nblocks);
   return hip_status(hipGetLastError(), "restrict_kernel");
 }
 
 template <typename T>
 int prolong_launch(const T* ec, T* xf, const GridArgs<T>& g, int assign, hipStream_t st) {
-  const unsigned lines = unsigned(g.nx + 1) * unsigned(g.ny + 1);
-  const size_t lds = size_t(3) * (g.nz / 2 + 1) * sizeof(T);
-  hipLaunchKernelGGL(prolong_kernel<T>, dim3(lines), dim3(256), lds, st, ec, xf, g, assign);
+  const TransferGeom tg = prolong_geom(g.nx, g.ny, g.nz);
+  const unsigned grid = unsigned((tg.nblocks + 7) / 8) * 8;
+  hipLaunchKernelGGL(prolong_kernel<T>, dim3(grid), dim3(TR_THREADS), tg.lds * sizeof(T), st, ec, xf, g, assign,
+                     tg.nblocks);
   return hip_status(hipGetLastError(), "prolong_kernel");
+}
+
+int assemble_dense_launch(const double* ke, const double* E, int nx, int ny, int nz, int mask_kind,
+                          const uint8_t* mask, double* A, int ld, hipStream_t st) {
+  const size_t n = size_t(3) * (nx + 1) * (ny + 1) * (nz + 1);
+  hipLaunchKernelGGL(assemble_dense_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, ke, E, nx, ny, nz,
+                     mask_kind, mask, A, ld);
+  return hip_status(hipGetLastError(), "assemble_dense_kernel");
+}
+
+int galerkin_scatter_launch(const double* G, int nx, int ny, int nz, int mask_kind, const uint8_t* mask, double* A,
+                            int ld, hipStream_t st) {
+  const size_t n = size_t(3) * (nx + 1) * (ny + 1) * (nz + 1);
+  hipLaunchKernelGGL(galerkin_scatter_kernel, dim3(blocks_for(81 * n, 256)), dim3(256), 0, st, G, nx, ny, nz,
+                     mask_kind, mask, A, ld);
+  return hip_status(hipGetLastError(), "galerkin_scatter_kernel");
 }
 
 template <typename T>

@@ -420,11 +420,6 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   }
 }
 
-// XCD-aware tile decode: the hardware assigns block b to XCD b % 8; XCD k gets the contiguous
-// logical range [k*per, (k+1)*per), so neighbouring tiles (shared halo rows, columns and
-// planes) are mostly served by the same L2.
-__device__ __forceinline__ int xcd_logical(int b, int grid) { return (b & 7) * (grid >> 3) + (b >> 3); }
-
 template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP = LOAD_NONE>
 __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
   const int logical = xcd_logical(blockIdx.x, gridDim.x);

@@ -251,7 +251,10 @@ class Multigrid:
         """Launch plans of every smoothing level's passes (smoother and residual)."""
         sm = L.MODE_CHEB3 if self.smoother == "chebyshev" else L.MODE_JACOBI
         for lv in self.levels[:-1]:
-            lv.tune(tuner, [(sm, False), (L.MODE_RESID, False)])
+            passes = [(sm, False), (L.MODE_RESID, False)]
+            if sm == L.MODE_CHEB3 and self.cheb_variant == "saad":
+                passes.append((sm, True))          # the first step, fused with x0 = D^-1 r / theta
+            lv.tune(tuner, passes)
 
     # ---------------------------------------------------------------- smoothing
     def _dinv_kw(self, Lv):

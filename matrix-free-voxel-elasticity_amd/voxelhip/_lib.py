@@ -65,6 +65,8 @@ SIGNATURES = {
     "vh_prolong": [_i, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp],
     "vh_coarsen_modulus": [_i, _vp, _vp, _i, _i, _i, _vp],
     "vh_dense_mv": [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    "vh_assemble_dense": [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp],
+    "vh_galerkin_scatter": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp],
     "vh_reduce_parts": [_sz],
     "vh_reduce": [_i, _i, _sz, _vp, _vp, _vp, _vp, _vp],
     "vh_finish": [_vp, _i, _i, _vp, _i, _i, _vp],

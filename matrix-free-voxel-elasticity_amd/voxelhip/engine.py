@@ -87,29 +87,14 @@ class GMGSolver:
         self._graphs = None
 
     def _galerkin_coarsest(self, E64):
-        """A_c = P^T A_{L-2} P with the coarse mask re-imposed (host sparse products, setup)."""
-        import scipy.sparse as sp
-        fshape, cshape = self.shapes[-2], self.shapes[-1]
+        """A_c = P^T A_{L-2} P with the coarse mask re-imposed, assembled on the device."""
+        fshape = self.shapes[-2]
         lvl = len(self.shapes) - 2
-        Ef = E64
-        m = self.mask
+        Ef, m = E64, self.mask
         for i in range(1, lvl + 1):
             Ef = mg.coarsen_modulus(None, Ef, self.shapes[i - 1])
             m = m.coarsen(self.shapes[i - 1])
-        Ef = Ef.cpu().numpy()
-        mf = m.values(fshape, torch.float64, self.dev).cpu().numpy()
-        edof = mg.edof_author(fshape)
-        nf = 3 * ops.n_nodes(fshape)
-        ke = self.ke * 2 ** lvl
-        K = sp.coo_matrix(((Ef[:, None, None] * ke[None]).ravel(),
-                           (np.repeat(edof, 24, axis=1).ravel(), np.tile(edof, (1, 24)).ravel())),
-                          shape=(nf, nf)).tocsr()
-        Mf = sp.diags(mf)
-        Af = Mf @ K @ Mf + sp.diags(1.0 - mf)
-        P = _prolongation_matrix(cshape)
-        Ac = (P.T @ Af @ P).toarray()
-        mc = m.coarsen(fshape).values(cshape, torch.float64, self.dev).cpu().numpy()
-        return mc[:, None] * Ac * mc[None, :] + np.diag(1.0 - mc)
+        return galerkin_coarsest(fshape, self.ke * 2 ** lvl, Ef, m)
 
     def _alloc(self):
         n = self.fine.n
@@ -305,8 +290,9 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
             Ec = mg.coarsen_modulus(None, Ec, shapes[i - 1])
             m = m.coarsen(shapes[i - 1])
         cshape = shapes[-1]
-        coarse_matrix = mg.assemble_dense(cshape, ke * 2 ** (len(shapes) - 1), Ec, m.values(cshape, torch.float64, dev))
-    Lc = torch.linalg.cholesky(torch.from_numpy(coarse_matrix).to(dev))
+        kc = torch.from_numpy(np.ascontiguousarray(ke * 2 ** (len(shapes) - 1), dtype=np.float64)).to(dev)
+        coarse_matrix = ops.assemble_dense(kc, Ec, cshape, mask=m)
+    Lc = torch.linalg.cholesky(coarse_matrix)
     if smoother == "chebyshev":
         kw = dict(smoother="chebyshev", pre=degree, post=degree, lower_frac=lower_frac)
     else:
@@ -320,17 +306,34 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
     return fine, M
 
 
-def _prolongation_matrix(coarse):
-    """Sparse trilinear P (fine x coarse) in the author AoS layout (src/voxel_mg.py:74-97)."""
-    import scipy.sparse as sp
-    nx, ny, nz = coarse
-    def axis(nc):
-        rows, cols, vals = [], [], []
-        for i in range(2 * nc + 1):
-            if i % 2 == 0:
-                rows.append(i); cols.append(i // 2); vals.append(1.0)
-            else:
-                rows += [i, i]; cols += [i // 2, i // 2 + 1]; vals += [0.5, 0.5]
-        return sp.csr_matrix((vals, (rows, cols)), shape=(2 * nc + 1, nc + 1))
-    P = sp.kron(sp.kron(sp.kron(axis(nx), axis(ny)), axis(nz)), sp.identity(3))
-    return P.tocsr()
+def galerkin_coarsest(fshape, ke, Ef, mask):
+    """Dense FP64 Galerkin operator M_c (P^T A_f P) M_c + (I - M_c) of the grid below `fshape`,
+    on the device (multigrid_v4.py:248-270,747-772 build it with host sparse triple products).
+    A_f = M K(Ef) M + (I - M) is the level's constrained operator (ke: that level's element
+    matrix).  P^T A_f P couples coarse nodes at most one apart, so 81 probes (the 27 node
+    residues mod 3 times 3 components) give every column: probe p = sum of e_(J, c) over the
+    nodes J of residue class p, pushed through prolongation, the level product and restriction,
+    and scattered into A_c by vh_galerkin_scatter."""
+    dev = Ef.device
+    cshape = tuple(int(s) // 2 for s in fshape)
+    cx, cy, cz = (s + 1 for s in cshape)
+    nn, nf = cx * cy * cz, 3 * ops.n_nodes(fshape)
+    node = torch.arange(nn, device=dev)
+    iz, iy, ix = node % cz, (node // cz) % cy, node // (cz * cy)
+    cls = ((ix % 3) * 3 + iy % 3) * 3 + iz % 3
+    V = torch.zeros((81, nn, 3), dtype=torch.float64, device=dev)
+    for c in range(3):
+        V[3 * cls + c, node, c] = 1.0
+    V = V.view(81, 3 * nn)
+    G = torch.empty_like(V)
+    blocks = mg.HipProduct(fshape, ke, torch.float64).blocks
+    mf = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, fshape, dev)
+    f, y = (torch.empty(nf, dtype=torch.float64, device=dev) for _ in range(2))
+    for p in range(81):
+        ops.prolong(V[p], f, fshape, assign=True)
+        ops.matvec(blocks, Ef, f, y, fshape, mask=mf)
+        ops.restrict(y, G[p], fshape)
+    A = torch.zeros((3 * nn, 3 * nn), dtype=torch.float64, device=dev)
+    return ops.galerkin_scatter(G, A, cshape, mask=mf.coarsen(fshape))
+
+

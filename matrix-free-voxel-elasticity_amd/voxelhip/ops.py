@@ -225,6 +225,38 @@ def dense_mv(A, r, y, mask=None, n=None):
     return y
 
 
+def assemble_dense(ke, E, shape, mask=None, A=None):
+    """Dense FP64 constrained operator M K M + (I - M) of a small grid, assembled on the device
+    (ke: 24 x 24 FP64 device tensor in the native node order; E: FP64 element moduli)."""
+    _dev(ke, "ke"); _dev(E, "E")
+    n = 3 * n_nodes(shape)
+    if ke.dtype != torch.float64 or E.dtype != torch.float64 or ke.numel() != 576:
+        raise TypeError("assemble_dense: ke (576 values) and E must be float64")
+    if A is None:
+        A = torch.empty((n, n), dtype=torch.float64, device=E.device)
+    _dev(A, "A")
+    if A.dim() != 2 or A.shape[0] != n or A.shape[1] < n or A.dtype != torch.float64:
+        raise ValueError("assemble_dense: A must be a float64 n x (>= n) matrix")
+    m = mask or Mask.none()
+    L.call("vh_assemble_dense", _p(ke), _p(E), *[int(s) for s in shape], m.kind, m.ptr, _p(A), int(A.shape[1]),
+           _stream())
+    return A
+
+
+def galerkin_scatter(G, A, coarse_shape, mask=None):
+    """Write M_c (P^T A_f P) M_c + (I - M_c) into the zeroed matrix A from the 81 probe rows G
+    (see include/voxelhip.h, vh_galerkin_scatter)."""
+    _dev(G, "G"); _dev(A, "A")
+    n = 3 * n_nodes(coarse_shape)
+    if G.shape != (81, n) or G.dtype != torch.float64 or A.dtype != torch.float64 or A.shape[0] != n \
+            or A.shape[1] < n:
+        raise ValueError("galerkin_scatter: G must be 81 x n and A n x (>= n), float64")
+    m = mask or Mask.none()
+    L.call("vh_galerkin_scatter", _p(G), *[int(s) for s in coarse_shape], m.kind, m.ptr, _p(A), int(A.shape[1]),
+           _stream())
+    return A
+
+
 def padded_matrix(A, dtype=None):
     """Copy of the square matrix A with its rows padded to a multiple of 4 entries (16-byte
     aligned rows for the vector loads of dense_mv); the padding is zero."""

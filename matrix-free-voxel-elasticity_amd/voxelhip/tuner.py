@@ -24,7 +24,7 @@ from . import _lib as L
 from . import ops
 
 TILES = (L.TILE_16x32, L.TILE_8x32)
-LX_CANDIDATES = (0, 4, 6, 8, 12, 16, 24, 32, 48, 64)
+LX_CANDIDATES = (0, 1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
 
 
 def _candidates(nx):
@@ -39,7 +39,7 @@ def _candidates(nx):
 
 
 class LaunchTuner:
-    def __init__(self, path=None, reps=8, rounds=2):
+    def __init__(self, path=None, reps=8, rounds=3):
         self.path = Path(path) if path else (Path(os.environ["VOXELHIP_TUNE_CACHE"])
                                              if os.environ.get("VOXELHIP_TUNE_CACHE") else None)
         self.reps, self.rounds = int(reps), int(rounds)
@@ -67,18 +67,27 @@ class LaunchTuner:
         os.replace(tmp, self.path)
 
     def _time(self, fn):
-        s = torch.cuda.current_stream()
+        """Mean duration of one launch, from a HIP graph of `reps` launches replayed `rounds`
+        times: the graph removes the host's per-launch cost, which otherwise hides the
+        differences between the plans of the small coarse-level products."""
         for _ in range(2):
             fn()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(self.reps):
+                fn()
+        graph.replay()
+        s = torch.cuda.current_stream()
         best = float("inf")
         for _ in range(self.rounds):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(s)
-            for _ in range(self.reps):
-                fn()
+            graph.replay()
             e1.record(s)
             e1.synchronize()
             best = min(best, e0.elapsed_time(e1) / self.reps)
+        del graph
         return best * 1e3                                    # microseconds
 
     def tune(self, level, mode, combine=False, dinv_array=False):

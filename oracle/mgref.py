@@ -10,6 +10,8 @@ M^-1 = D^-1 on [lam_max * lower_frac, lam_max]).
 Parity status:
   prolong/restrict   P trilinear (interpolation of a linear field is exact), R == P^T,
                      P^T K_f P == K_c for uniform E (tests/test_mg_cpu.py:53-67)
+  galerkin_dense     equals the rediscretised operator for uniform E, with and without the
+                     clamped face; symmetric; fixed rows are identity rows
   Multigrid/fcg      reference-held known answers (smoke-q4 iterations, compliance, solution
                      npz of results/session-d-20260923), golden fixtures of one V-cycle and of
                      spectral_omegas written from the reference in this container
@@ -66,6 +68,24 @@ def coarsen_modulus(E, shape):
     nx, ny, nz = shape
     return np.ascontiguousarray(
         np.asarray(E).reshape(nx // 2, 2, ny // 2, 2, nz // 2, 2).mean(axis=(1, 3, 5))).ravel()
+
+
+def prolongation_matrix(coarse_shape):
+    """Dense P (fine x coarse): column j is prolong(e_j) (src/voxel_mg.py:92-97)."""
+    n = 3 * int(np.prod([s + 1 for s in coarse_shape]))
+    return np.stack([prolong(np.eye(n)[j], coarse_shape) for j in range(n)], axis=1)
+
+
+def galerkin_dense(fine_shape, ke, E, mask):
+    """Galerkin coarse operator M_c (P^T A_f P) M_c + (I - M_c), A_f = assemble_dense(fine)
+    (multigrid_v4.py:248-270,747-772: K_c = R K_f P with R = P^T, the boundary re-imposed by
+    injection, solver_v2.py:716-763)."""
+    coarse = tuple(int(s) // 2 for s in fine_shape)
+    Af = assemble_dense(fine_shape, ke, E, mask)
+    P = prolongation_matrix(coarse)
+    Ac = P.T @ Af @ P
+    mc = coarsen_mask(mask, fine_shape).astype(np.float64)
+    return mc[:, None] * Ac * mc[None, :] + np.diag(1.0 - mc)
 
 
 def coarsen_mask(mask, shape):

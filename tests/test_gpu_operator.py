@@ -194,6 +194,32 @@ def test_transfers_match_the_oracle(fine):
     assert rel(mg.coarsen_modulus(None, dev(E), fine), mg_ref.coarsen_modulus(E, fine)) < 1e-15
 
 
+@pytest.mark.parametrize("kind", ["x0", "array"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("fine", [(6, 20, 340), (4, 10, 322), (10, 18, 16)])
+def test_masked_transfers_over_row_blocks_and_z_segments(kind, dtype, fine):
+    """Masked restriction and accumulating prolongation on grids whose rows span several
+    blocks in y and several segments in z (340 and 322 fine elements along z)."""
+    from voxelhip import _lib as L, ops
+    rng = np.random.default_rng(11)
+    coarse = tuple(s // 2 for s in fine)
+    mf = _masks(fine, rng)[kind]
+    mc = mg_ref.coarsen_mask(mf, fine)
+    Mf, Mc = ops.Mask.from_values(mf, fine), ops.Mask.from_values(mc, coarse)
+    want = L.MASK_X0 if kind == "x0" else L.MASK_ARRAY
+    assert Mf.kind == want and Mc.kind == want
+    rf = rng.standard_normal(mf.size)
+    uc = rng.standard_normal(mc.size)
+    xf = rng.standard_normal(mf.size)
+    tol = 1e-15 if dtype == torch.float64 else 1e-6
+    rc = ops.restrict(dev(rf, dtype), torch.empty(mc.size, dtype=dtype, device="cuda"), fine, fmask=Mf, cmask=Mc)
+    assert rel(rc, mg_ref.restrict(rf * mf, fine) * mc) < tol
+    out = ops.prolong(dev(uc, dtype), dev(xf, dtype), fine, mask=Mf, assign=False)
+    assert rel(out, xf + mf * mg_ref.prolong(uc, coarse)) < tol
+    out = ops.prolong(dev(uc, dtype), dev(xf, dtype), fine, mask=Mf, assign=True)
+    assert rel(out, mf * mg_ref.prolong(uc, coarse)) < tol
+
+
 def test_full_size_product_properties():
     """256 x 128 x 128 (12.8 M DOFs): the parity kernel against the independent dense-ke kernel,
     linearity, and rigid translations / rotations in the null space."""

@@ -66,6 +66,30 @@ def test_rediscretisation_equals_galerkin_for_uniform_modulus(KE):
     assert np.abs(P.T @ Kf @ P - Kc).max() / np.abs(Kc).max() < 1e-13
 
 
+@pytest.mark.parametrize("clamped", [False, True])
+def test_galerkin_operator_equals_rediscretisation_for_uniform_modulus(KE, clamped):
+    fine = (4, 2, 4)
+    coarse = (2, 1, 2)
+    mask = np.ones(3 * hx.n_nodes(fine))
+    if clamped:
+        mask[: 3 * 3 * 5] = 0                       # the x = 0 face
+    Ac = mg.galerkin_dense(fine, KE, np.full(32, 0.7), mask)
+    Kc = mg.assemble_dense(coarse, 2 * KE, np.full(4, 0.7), mg.coarsen_mask(mask, fine))
+    assert np.abs(Ac - Kc).max() / np.abs(Kc).max() < 1e-13
+    assert np.abs(Ac - Ac.T).max() < 1e-14 * np.abs(Ac).max()
+
+
+def test_galerkin_operator_of_random_moduli_is_spd_with_identity_rows(KE):
+    fine = (4, 4, 2)
+    rng = np.random.default_rng(5)
+    mask = (rng.random(3 * hx.n_nodes(fine)) > 0.2).astype(float)
+    Ac = mg.galerkin_dense(fine, KE, rng.uniform(0.1, 1, 32), mask)
+    mc = mg.coarsen_mask(mask, fine)
+    fixed = np.flatnonzero(mc == 0)
+    assert np.array_equal(Ac[fixed], np.eye(Ac.shape[0])[fixed])
+    assert np.linalg.eigvalsh(Ac).min() > 0
+
+
 def test_vcycle_is_symmetric(golden, KE):
     shape, E, mask, _ = smoke(golden)
     _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
