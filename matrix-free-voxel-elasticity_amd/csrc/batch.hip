@@ -232,7 +232,7 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
     p.omega = T(ep.omega);
     p.cheb_a = T(ep.cheb_a);
     p.cheb_c = T(ep.cheb_c);
-    for (int c = 0; c < 3; ++c) p.kd[c] = T(ep.kdiag[c]);
+    for (int c = 0; c < 3; ++c) p.kdi[c] = ep.kdiag[c] > 0 ? T(1.0 / ep.kdiag[c]) : T(0);
     p.flags = ep.flags;
     p.w = ep.w;
     p.sdev = ep.sdev;

@@ -151,7 +151,7 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
     p.cheb_c = T(ep->cheb_c);
     p.lx = ep->lx;
     p.tile = ep->tile;
-    for (int c = 0; c < 3; ++c) p.kd[c] = T(ep->kdiag[c]);
+    for (int c = 0; c < 3; ++c) p.kdi[c] = ep->kdiag[c] > 0 ? T(1.0 / ep->kdiag[c]) : T(0);
     p.flags = ep->flags;
     p.w = ep->w;
     p.sdev = ep->sdev;

@@ -58,8 +58,8 @@ struct MatvecParams {
   int fixed_zero;         // with a mask: the output is 0 at fixed DOFs (M K M u, (b - A u) M)
   int tiles_y, tiles_z, chunks_x, n_tiles;
   T omega, cheb_a, cheb_c, load_scale;
-  T kd[3];                // diagonal of ke per component (same at every corner of a brick): with
-                          // dinv == nullptr, D^-1 = 1 / (kd[c] * sum of the incident moduli)
+  T kdi[3];               // 1 / (diagonal of ke per component, the same at every corner of a brick):
+                          // with dinv == nullptr, D^-1 = kdi[c] / (sum of the incident moduli)
   T coef[72];             // parity blocks C[g][i][j], or [p-q, q, a, b, c, d, e-f, f] (cube)
 };
 

@@ -105,6 +105,11 @@ __device__ __forceinline__ void element_blocks(const T (&v)[8][3], T Ee, const M
   }
 }
 
+// 1 / x: IEEE division in FP64; the hardware reciprocal (1 ulp) in FP32, where it only scales
+// a smoothing correction.
+__device__ __forceinline__ double reciprocal(double x) { return 1.0 / x; }
+__device__ __forceinline__ float reciprocal(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // One workgroup's share of the product: tile `logical` (y tile fastest, then z tile, then x
 // chunk) of the grid described by `a`.
 template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP>
@@ -358,9 +363,11 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int k = 0; k < 4; ++k) part[c][k] = sc[k][c][jn];
-      T esum = T(0);
-      if (dinv_from_E)   // incident columns (jy-1|jy, jz-1|jz) of node (jy, jz) = (ly+1, lz+1)
-        esum = ((se[jn - NZ - 1] + se[jn - 1]) + se[jn - NZ]) + se[jn];
+      T rsum = T(0);     // 1 / (sum of the moduli of the node's incident elements)
+      if (dinv_from_E) { // incident columns (jy-1|jy, jz-1|jz) of node (jy, jz) = (ly+1, lz+1)
+        const T esum = ((se[jn - NZ - 1] + se[jn - 1]) + se[jn - NZ]) + se[jn];
+        rsum = reciprocal(esum);
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const T ku = ((part[c][0] + part[c][1]) + part[c][2]) + part[c][3];
@@ -380,8 +387,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         } else if (NEEDS_DINV) {
           T dinv;
           if (dinv_from_E) {
-            const T kdc = c == 0 ? a.kd[0] : (c == 1 ? a.kd[1] : a.kd[2]);
-            dinv = free_dof ? T(1) / (kdc * esum) : T(1);
+            const T kdi = c == 0 ? a.kdi[0] : (c == 1 ? a.kdi[1] : a.kdi[2]);
+            dinv = free_dof ? kdi * rsum : T(1);
           }
           else dinv = ep_di[c];
           const T z = dinv * (ep_b[c] - au);
