@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 2
+#define VH_ABI_VERSION 3
 
 enum { VH_F64 = 0, VH_F32 = 1, VH_BF16 = 2 };
 enum { VH_MASK_NONE = 0, VH_MASK_ARRAY = 1, VH_MASK_X0 = 2 };
@@ -29,10 +29,14 @@ enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3,
        VH_MODE_CHEB3 = 5 };
 enum { VH_RED_DOT = 0, VH_RED_DOT2 = 1, VH_RED_DOTDIFF = 2, VH_RED_NORM2 = 3 };
 enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4,
-       VH_OP_INVSQRT = 5 /* scal[dst] = 1 / sqrt(sum): a device-side normalisation factor */ };
+       VH_OP_INVSQRT = 5 /* scal[dst] = 1 / sqrt(sum): a device-side normalisation factor */,
+       VH_OP_RESNORM = 6 /* scal[dst] = |r|^2 of an iteration; unless VH_S_DONE: VH_S_ITS += 1 and
+                            VH_S_DONE = 1 once |r|^2 <= scal[VH_S_TOL2] (convergence on the device,
+                            so the host can queue iterations ahead without reading each one) */ };
 /* device scalar slots used by vh_finish / vh_cg_update / vh_xpby; VH_S_DONE != 0 freezes the
  * recurrence (alpha = beta = 0), for converged problems of a batch */
-enum { VH_S_RZ = 0, VH_S_PQ = 1, VH_S_ALPHA = 2, VH_S_BETA = 3, VH_S_FLAG = 4, VH_S_DONE = 10 };
+enum { VH_S_RZ = 0, VH_S_PQ = 1, VH_S_ALPHA = 2, VH_S_BETA = 3, VH_S_FLAG = 4, VH_S_DONE = 10, VH_S_TOL2 = 11,
+       VH_S_ITS = 12 };
 
 int vh_abi_version(void);
 const char* vh_last_error(void);

@@ -514,7 +514,7 @@ int vh_seg_dotdiff(void* segments, int dtype_in, int dtype_v, const void* zin, c
 int vh_seg_finish(const double* partial, int total, const int* offs, int n, int count, double* scal, int stride,
                   int dst, int op, void* stream) {
   VH_REQUIRE(partial && offs && scal && n >= 1 && total >= 1 && (count == 1 || count == 2) && stride >= 16 &&
-                 dst >= 0 && dst + count <= stride,
+                 dst >= 0 && dst + count <= stride && op >= OP_NONE && op <= OP_RESNORM,
              "vh_seg_finish: bad arguments");
   hipLaunchKernelGGL(seg_finish_kernel, dim3(n), dim3(256), 0, static_cast<hipStream_t>(stream), partial, total, offs,
                      count, scal, stride, dst, op);

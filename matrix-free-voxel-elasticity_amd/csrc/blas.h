@@ -7,12 +7,14 @@ constexpr int BLAS_BLOCK = 256;
 constexpr int BLAS_MAX_GRID = 2048;     // 8 blocks per CU on 256 CUs; partial arrays are this long
 
 enum RedKind : int { RED_DOT = 0, RED_DOT2 = 1, RED_DOTDIFF = 2, RED_NORM2 = 3 };
-enum ScalarOp : int { OP_NONE = 0, OP_ALPHA = 1, OP_BETA_FR = 2, OP_BETA_PR = 3, OP_SET_RZ = 4, OP_INVSQRT = 5 };
-// Device scalar slots (FP64).  Slots 8, 9, 11.. are free for callers (norms, dots); S_DONE != 0
+enum ScalarOp : int { OP_NONE = 0, OP_ALPHA = 1, OP_BETA_FR = 2, OP_BETA_PR = 3, OP_SET_RZ = 4, OP_INVSQRT = 5,
+                      OP_RESNORM = 6 };
+// Device scalar slots (FP64).  Slots 8, 9, 13.. are free for callers (norms, dots); S_DONE != 0
 // freezes a Krylov recurrence (alpha = beta = 0: the iterate stops moving), which the batched
-// solvers set for problems that have converged while others iterate on.
+// solvers set for problems that have converged while others iterate on, and OP_RESNORM sets
+// once the squared residual norm reaches S_TOL2 (S_ITS counts the iterations up to then).
 enum Slot : int { S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_BETA = 3, S_FLAG = 4, S_RZNEW = 5, S_NUM = 6, S_NORM2 = 7,
-                  S_DONE = 10 };
+                  S_DONE = 10, S_TOL2 = 11, S_ITS = 12 };
 
 // Stores the `count` reduced values v into s[dst..] and evaluates the scalar recurrence `op`.
 __device__ inline void apply_scalar_op(double* s, const double* v, int count, int dst, int op) {
@@ -44,6 +46,12 @@ __device__ inline void apply_scalar_op(double* s, const double* v, int count, in
     }
     case OP_SET_RZ: s[S_RZ] = v[0]; break;
     case OP_INVSQRT: s[dst] = v[0] > 0.0 ? 1.0 / sqrt(v[0]) : 0.0; break;   // normalisation factor
+    case OP_RESNORM:         // an iteration's squared residual norm (in s[dst]): count, test, freeze
+      if (!done) {
+        s[S_ITS] += 1.0;
+        if (v[0] <= s[S_TOL2]) s[S_DONE] = 1.0;
+      }
+      break;
     default: break;
   }
 }

@@ -379,7 +379,8 @@ int vh_reduce(int dtype, int kind, size_t n, const void* x, const void* y, const
 }
 
 int vh_finish(const double* partial, int nparts, int count, double* scal, int dst, int op, void* stream) {
-  VH_REQUIRE(partial && scal && nparts >= 1 && count >= 1 && count <= 2 && dst >= 0, "vh_finish: bad arguments");
+  VH_REQUIRE(partial && scal && nparts >= 1 && count >= 1 && count <= 2 && dst >= 0 && op >= OP_NONE &&
+                 op <= OP_RESNORM, "vh_finish: bad arguments");
   return finish_launch(partial, nparts, count, scal, dst, op, static_cast<hipStream_t>(stream));
 }
 

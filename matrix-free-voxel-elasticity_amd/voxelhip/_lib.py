@@ -10,15 +10,15 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libvoxelhip.so"
-ABI_VERSION = 2                 # VH_ABI_VERSION of include/voxelhip.h these bindings follow
+ABI_VERSION = 3                 # VH_ABI_VERSION of include/voxelhip.h these bindings follow
 
 F64, F32, BF16 = 0, 1, 2
 MASK_NONE, MASK_ARRAY, MASK_X0 = 0, 1, 2
 MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT, MODE_CHEB3 = 0, 1, 2, 3, 4, 5
 RED_DOT, RED_DOT2, RED_DOTDIFF, RED_NORM2 = 0, 1, 2, 3
-OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ, OP_INVSQRT = 0, 1, 2, 3, 4, 5
+OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ, OP_INVSQRT, OP_RESNORM = 0, 1, 2, 3, 4, 5, 6
 S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
-S_DONE = 10
+S_DONE, S_TOL2, S_ITS = 10, 11, 12
 TILE_AUTO, TILE_16x32, TILE_8x32 = 0, 1, 2
 
 _vp = ctypes.c_void_p

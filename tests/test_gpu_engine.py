@@ -109,3 +109,30 @@ def test_jacobi_engine_matches_the_reference_vcycle_solver():
     s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, precision="fp64", graph=True)
     x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
     assert ok and it == ito and rel(x, xo) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_iteration_limit_and_queued_iterations(problem, precision):
+    """Iterations queued ahead of the host's convergence test: a solve capped below the needed
+    count stops exactly at the cap, unconverged; a following full solve is unaffected; a
+    tolerance met early is reported at the iteration that met it, with x frozen there (every
+    later queued iteration leaves it bitwise unchanged)."""
+    from voxelhip.engine import GMGSolver
+    shape, E, mask, force, xo, ito = problem
+    f = torch.from_numpy(force).cuda()
+    s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, smoother="chebyshev", degree=3,
+                  precision=precision, graph=True)
+    print("case", precision, "capped", flush=True)
+    x, it, ok, r, _ = s.solve(f, maxiter=3)
+    assert it == 3 and not ok and r > 1e-8
+    x, it, ok, r, _ = s.solve(f)
+    assert ok and r <= 1e-8 and abs(it - ito) <= (0 if precision == "fp64" else 1)
+    x_full = x.clone()
+    print("case", precision, "loose", flush=True)
+    x, it2, ok, r2, _ = s.solve(f, tol=1e-3)
+    assert ok and r2 <= 1e-3 and it2 < it
+    xl = x.clone()
+    # the same loose solve capped at that count (no iteration queued past the cap) reaches the same x
+    y, it3, ok3, r3, _ = s.solve(f, tol=1e-3, maxiter=it2)
+    assert ok3 and it3 == it2 and torch.equal(y, xl) and r3 == r2
+    assert rel(x_full, xo) < (1e-9 if precision == "fp64" else 1e-7)
