@@ -132,8 +132,10 @@ struct TransferDesc {
   const uint8_t* cmask;
 };
 
+static_assert(TR_RESTRICT::NT == TR_PROLONG::NT, "one workgroup size for both transfer directions");
+
 template <typename T, bool RESTRICT>
-__global__ void __launch_bounds__(TR_THREADS) transfer_batch_kernel(const TransferDesc<T>* __restrict__ d,
+__global__ void __launch_bounds__(TR_RESTRICT::NT) transfer_batch_kernel(const TransferDesc<T>* __restrict__ d,
                                                                     const int* __restrict__ offs, int n, int assign) {
   extern __shared__ unsigned char smem_raw[];
   const int b = xcd_logical(blockIdx.x, gridDim.x);
@@ -265,7 +267,8 @@ static int transfer_create(int n, const vh_transfer_item* items, BatchHandle* h,
     const vh_transfer_item& it = items[i];
     descs[i] = TransferDesc<T>{static_cast<const T*>(it.src), static_cast<T*>(it.dst), it.nx, it.ny, it.nz,
                                it.fmask_kind, it.cmask_kind, it.fmask, it.cmask};
-    const TransferGeom g = h->restrict_op ? restrict_geom(it.nx, it.ny, it.nz) : prolong_geom(it.nx, it.ny, it.nz);
+    const TransferGeom g = h->restrict_op ? restrict_geom<TR_RESTRICT>(it.nx, it.ny, it.nz)
+                                          : prolong_geom<TR_PROLONG>(it.nx, it.ny, it.nz);
     prefix[i + 1] = prefix[i] + g.nblocks;
     lds = std::max(lds, g.lds);
   }
@@ -417,7 +420,7 @@ int vh_batch_launch(void* handle, void* stream) {
       return h->dtype == F64 ? matvec_batch_launch<double>(b, st) : matvec_batch_launch<float>(b, st);
     }
     case BK_TRANSFER: {
-      const dim3 g(unsigned((h->total + 7) / 8) * 8), blk(TR_THREADS);
+      const dim3 g(unsigned((h->total + 7) / 8) * 8), blk(TR_RESTRICT::NT);
       if (h->dtype == F64) {
         const auto* d = static_cast<const TransferDesc<double>*>(h->descs);
         if (h->restrict_op) hipLaunchKernelGGL((transfer_batch_kernel<double, true>), g, blk, h->lds, st, d, h->offs, h->n, h->assign);

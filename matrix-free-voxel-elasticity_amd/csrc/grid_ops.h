@@ -18,21 +18,51 @@ struct GridArgs {
 // 1/2 if |i - 2I| == 1 (src/voxel_mg.py:74-105: P copies even nodes, averages odd ones; R = P^T).
 // (nx, ny, nz) is always the FINE element grid.  Both operators work on blocks of whole rows
 // of the author layout (a row = the 3 (nz + 1) values of one (ix, iy) z-line, contiguous in HBM):
-//  restriction: coarse plane Ix x TR_RB coarse rows x a z segment of coarse nodes.  The x weights
-//    combine the three fine planes 2Ix - 1 .. 2Ix + 1 into 2 TR_RB + 1 weighted fine rows in LDS
+//  restriction: coarse plane Ix x RB coarse rows x a z segment of coarse nodes.  The x weights
+//    combine the three fine planes 2Ix - 1 .. 2Ix + 1 into 2 RB + 1 weighted fine rows in LDS
 //    (one coalesced pass, every load of the block in flight together); each coarse DOF then takes
 //    its y and z neighbours from LDS.
-//  prolongation: fine plane ix x TR_PB fine rows x a z segment of fine nodes.  The fine values
-//    (for xf +=) are loaded first, then the TR_PB / 2 + 1 coarse rows are combined with the x
+//  prolongation: fine plane ix x PB fine rows x a z segment of fine nodes.  The fine values
+//    (for xf +=) are loaded first, then the PB / 2 + 1 coarse rows are combined with the x
 //    weight into LDS; each fine DOF interpolates along y and z from LDS.
-// Blocks of one grid are numbered z segment fastest, then y block, then x plane.
-constexpr int TR_THREADS = 256;
-constexpr int TR_RB = 4;            // coarse rows per restriction block
-constexpr int TR_CZ = 80;           // coarse nodes per restriction z segment (at most)
-constexpr int TR_PB = 8;            // fine rows per prolongation block (even)
-constexpr int TR_FZ = 160;          // fine nodes per prolongation z segment (at most; even)
-constexpr int TR_PF = (TR_PB * 3 * TR_FZ + TR_THREADS - 1) / TR_THREADS;   // fine values per thread
-constexpr int TR_U = 4;             // slots per thread with loads in flight together
+// Blocks of one grid are numbered z segment fastest, then y block, then x plane.  The block
+// shape and workgroup size are compile-time configurations (RestrictCfg / ProlongCfg); the
+// library's kernels use TR_RESTRICT and TR_PROLONG.
+template <int RB_, int CZ_, int NT_, int U_>
+struct RestrictCfg {
+  static constexpr int RB = RB_;   // coarse rows per block
+  static constexpr int CZ = CZ_;   // coarse nodes per z segment (at most)
+  static constexpr int NT = NT_;   // threads per workgroup
+  static constexpr int SLOTS = ((2 * RB + 1) * 3 * (2 * CZ + 1) + NT - 1) / NT;   // fine values per thread
+  static constexpr int U = U_ < SLOTS ? U_ : SLOTS;                             // slots with loads in flight
+};
+template <int PB_, int FZ_, int NT_>
+struct ProlongCfg {
+  static constexpr int PB = PB_;   // fine rows per block (even)
+  static constexpr int FZ = FZ_;   // fine nodes per z segment (at most; even)
+  static constexpr int NT = NT_;
+  static constexpr int PF = (PB * 3 * FZ + NT - 1) / NT;                          // fine values per thread
+  static constexpr int CS = ((PB / 2 + 1) * 3 * (FZ / 2 + 1) + NT - 1) / NT;      // coarse values per thread
+};
+using TR_RESTRICT = RestrictCfg<4, 80, 256, 4>;
+using TR_PROLONG = ProlongCfg<4, 160, 256>;
+
+// Walks the slots s = start, start + step, ... of a block laid out as rows of `len` values,
+// tracking (row, col) = (s / len, s % len) without a division per slot.
+struct RowCursor {
+  int r, c, dr, dc, len;
+  __device__ RowCursor(int start, int step, int len_) : len(len_) {
+    r = start / len;
+    c = start - r * len;
+    dr = step / len;
+    dc = step - dr * len;
+  }
+  __device__ void next() {
+    r += dr;
+    c += dc;
+    if (c >= len) { c -= len; ++r; }
+  }
+};
 
 struct TransferGeom {
   int ny1, nz1;   // nodes per column / row of the grid the blocks tile (coarse: restriction)
@@ -42,41 +72,44 @@ struct TransferGeom {
   size_t lds;     // LDS values per block
 };
 
+template <typename C>
 __host__ __device__ inline TransferGeom restrict_geom(int nx, int ny, int nz) {
   TransferGeom g;
   g.ny1 = ny / 2 + 1;
   g.nz1 = nz / 2 + 1;
-  g.nzb = (g.nz1 + TR_CZ - 1) / TR_CZ;
+  g.nzb = (g.nz1 + C::CZ - 1) / C::CZ;
   g.seg = (g.nz1 + g.nzb - 1) / g.nzb;
-  g.nyb = (g.ny1 + TR_RB - 1) / TR_RB;
+  g.nyb = (g.ny1 + C::RB - 1) / C::RB;
   g.nblocks = (nx / 2 + 1) * g.nyb * g.nzb;
-  g.lds = size_t(2 * TR_RB + 1) * 3 * (2 * g.seg + 1);
+  g.lds = size_t(2 * C::RB + 1) * 3 * (2 * g.seg + 1);
   return g;
 }
 
+template <typename C>
 __host__ __device__ inline TransferGeom prolong_geom(int nx, int ny, int nz) {
   TransferGeom g;
   g.ny1 = ny + 1;
   g.nz1 = nz + 1;
-  g.nzb = (g.nz1 + TR_FZ - 1) / TR_FZ;
+  g.nzb = (g.nz1 + C::FZ - 1) / C::FZ;
   g.seg = (((g.nz1 + g.nzb - 1) / g.nzb) + 1) & ~1;
-  g.nyb = (g.ny1 + TR_PB - 1) / TR_PB;
+  g.nyb = (g.ny1 + C::PB - 1) / C::PB;
   g.nblocks = (nx + 1) * g.nyb * g.nzb;
-  g.lds = size_t(TR_PB / 2 + 1) * 3 * (g.seg / 2 + 1);
+  g.lds = size_t(C::PB / 2 + 1) * 3 * (g.seg / 2 + 1);
   return g;
 }
 
 // rc = mask_c * R (mask_f * rf) on restriction block `blk`; xl: restrict_geom(..).lds values.
-template <typename T>
+template <typename T, typename C = TR_RESTRICT>
 __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __restrict__ rc, int nx, int ny, int nz,
                                                int fmask_kind, const uint8_t* __restrict__ fmask,
                                                const uint8_t* __restrict__ cmask, int cmask_kind, int blk, T* xl) {
-  const TransferGeom g = restrict_geom(nx, ny, nz);
+  constexpr int NT = C::NT, U = C::U;
+  const TransferGeom g = restrict_geom<C>(nx, ny, nz);
   const int zb = blk % g.nzb;
   const int yb = (blk / g.nzb) % g.nyb;
   const int Ix = blk / (g.nzb * g.nyb);
-  const int Iy0 = yb * TR_RB, Iz0 = zb * g.seg;
-  const int nrow = min(TR_RB, g.ny1 - Iy0);        // coarse rows of the block
+  const int Iy0 = yb * C::RB, Iz0 = zb * g.seg;
+  const int nrow = min(C::RB, g.ny1 - Iy0);        // coarse rows of the block
   const int ncz = min(g.seg, g.nz1 - Iz0);         // coarse nodes of the segment
   const int flen = 3 * (2 * ncz + 1);              // fine nodes 2 Iz0 - 1 .. 2 (Iz0 + ncz) - 1
   const int nslot = (2 * nrow + 1) * flen;         // fine rows 2 Iy0 - 1 .. 2 (Iy0 + nrow) - 1
@@ -91,14 +124,15 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
     const T v = rf[dof];
     return (fmask_kind == MASK_ARRAY && !fmask[dof]) ? T(0) : v;
   };
-  for (int s0 = threadIdx.x; s0 < nslot; s0 += TR_U * TR_THREADS) {
-    T vm[TR_U], v0[TR_U], vp[TR_U];
+  RowCursor cur(threadIdx.x, NT, flen);
+  for (int s0 = threadIdx.x; s0 < nslot; s0 += U * NT) {
+    T vm[U], v0[U], vp[U];
 #pragma unroll
-    for (int u = 0; u < TR_U; ++u) {
-      const int s = s0 + u * TR_THREADS;
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u * NT;
       vm[u] = v0[u] = vp[u] = T(0);
-      const int j = s / flen, c = s - (s / flen) * flen;
-      const int iy = 2 * Iy0 - 1 + j, q = q0 + c;
+      const int iy = 2 * Iy0 - 1 + cur.r, q = q0 + cur.c;
+      cur.next();
       if (s < nslot && iy >= 0 && iy <= ny && q >= 0 && q < fz3) {
         const size_t off = size_t(iy) * fz3 + q;
         if (okm) vm[u] = fetch(size_t(ixm) * fplane + off);
@@ -107,8 +141,8 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
       }
     }
 #pragma unroll
-    for (int u = 0; u < TR_U; ++u) {
-      const int s = s0 + u * TR_THREADS;
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u * NT;
       if (s < nslot) xl[s] = T(0.5) * vm[u] + v0[u] + T(0.5) * vp[u];
     }
   }
@@ -116,8 +150,8 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
   const int clen = 3 * ncz;
   const size_t cz3 = size_t(3) * g.nz1;
   auto zsum = [&](const T* p) { return T(0.5) * p[0] + p[3] + T(0.5) * p[6]; };
-  for (int o = threadIdx.x; o < nrow * clen; o += TR_THREADS) {
-    const int r = o / clen, c = o - (o / clen) * clen;
+  for (RowCursor oc(threadIdx.x, NT, clen); oc.r < nrow; oc.next()) {
+    const int r = oc.r, c = oc.c;
     const int zl = c / 3, k = c - 3 * (c / 3);
     // coarse row r <-> local fine row 2r + 1, coarse node zl <-> local fine node 2 zl + 1
     const T* p = xl + 2 * r * flen + 6 * zl + k;
@@ -131,62 +165,69 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
 }
 
 // xf (+)= mask_f * P ec on prolongation block `blk`; cl: prolong_geom(..).lds values.
-template <typename T>
+template <typename T, typename C = TR_PROLONG>
 __device__ __forceinline__ void prolong_block(const T* __restrict__ ec, T* __restrict__ xf, int nx, int ny, int nz,
                                               int mask_kind, const uint8_t* __restrict__ mask, int assign, int blk,
                                               T* cl) {
-  const TransferGeom g = prolong_geom(nx, ny, nz);
+  constexpr int NT = C::NT, PF = C::PF, CS = C::CS;
+  const TransferGeom g = prolong_geom<C>(nx, ny, nz);
   const int zb = blk % g.nzb;
   const int yb = (blk / g.nzb) % g.nyb;
   const int ix = blk / (g.nzb * g.nyb);
-  const int iy0 = yb * TR_PB, iz0 = zb * g.seg;    // both even
-  const int nrow = min(TR_PB, g.ny1 - iy0);
+  const int iy0 = yb * C::PB, iz0 = zb * g.seg;    // both even
+  const int nrow = min(C::PB, g.ny1 - iy0);
   const int flen = 3 * min(g.seg, g.nz1 - iz0);
-  const int nslot = nrow * flen;                   // <= TR_PF * TR_THREADS
   const int cy1 = ny / 2 + 1, cz1 = nz / 2 + 1;
   const int Y0 = iy0 >> 1, Z0 = iz0 >> 1;
-  const int crow = min(TR_PB / 2 + 1, cy1 - Y0);
+  const int crow = min(C::PB / 2 + 1, cy1 - Y0);
   const int clen = 3 * min(g.seg / 2 + 1, cz1 - Z0);
   const int fz3 = 3 * g.nz1;
   const size_t cz3 = size_t(3) * cz1;
   const size_t fbase = (size_t(ix) * g.ny1 + iy0) * fz3 + 3 * size_t(iz0);
-  // the fine values this thread updates are loaded before the coarse rows, so both memory
-  // round trips of the workgroup overlap
-  T xold[TR_PF];
+  // every load of the block is issued before the first wait: the fine values this thread
+  // updates, then the coarse values of the x-combined rows
+  T xold[PF];
+  int fo[PF];                                      // offsets of this thread's fine values
+  {
+    RowCursor fc(threadIdx.x, NT, flen);
 #pragma unroll
-  for (int j = 0; j < TR_PF; ++j) {
-    const int s = threadIdx.x + j * TR_THREADS;
-    const int r = s / flen, c = s - (s / flen) * flen;
-    xold[j] = (!assign && s < nslot) ? xf[fbase + size_t(r) * fz3 + c] : T(0);
+    for (int j = 0; j < PF; ++j) {
+      fo[j] = fc.r < nrow ? fc.r * fz3 + fc.c : -1;
+      fc.next();
+    }
   }
+#pragma unroll
+  for (int j = 0; j < PF; ++j) xold[j] = (!assign && fo[j] >= 0) ? xf[fbase + fo[j]] : T(0);
   const int x0 = ix >> 1, x1 = (ix + 1) >> 1;
   const bool xodd = ix & 1;
   const T* c0 = ec + (size_t(x0) * cy1 + Y0) * cz3 + 3 * size_t(Z0);
   const T* c1 = ec + (size_t(x1) * cy1 + Y0) * cz3 + 3 * size_t(Z0);
-  const int ncs = crow * clen;
-  for (int s0 = threadIdx.x; s0 < ncs; s0 += TR_U * TR_THREADS) {
-    T a[TR_U], b[TR_U];
+  const int ncs = crow * clen;                     // <= CS * NT
+  T ca[CS], cb[CS];
+  {
+    RowCursor cc(threadIdx.x, NT, clen);
 #pragma unroll
-    for (int u = 0; u < TR_U; ++u) {
-      const int s = s0 + u * TR_THREADS;
-      const int j = s / clen, c = s - (s / clen) * clen;
-      const size_t off = size_t(j) * cz3 + c;
-      a[u] = s < ncs ? c0[off] : T(0);
-      b[u] = (s < ncs && xodd) ? c1[off] : T(0);
+    for (int u = 0; u < CS; ++u) {
+      const int s = threadIdx.x + u * NT;
+      const size_t off = size_t(cc.r) * cz3 + cc.c;
+      cc.next();
+      ca[u] = s < ncs ? c0[off] : T(0);
+      cb[u] = (s < ncs && xodd) ? c1[off] : T(0);
     }
+  }
 #pragma unroll
-    for (int u = 0; u < TR_U; ++u) {
-      const int s = s0 + u * TR_THREADS;
-      if (s < ncs) cl[s] = xodd ? T(0.5) * (a[u] + b[u]) : a[u];
-    }
+  for (int u = 0; u < CS; ++u) {
+    const int s = threadIdx.x + u * NT;
+    if (s < ncs) cl[s] = xodd ? T(0.5) * (ca[u] + cb[u]) : ca[u];
   }
   __syncthreads();
   const bool plane_free = !(mask_kind == MASK_X0 && ix == 0);
+  RowCursor fc(threadIdx.x, NT, flen);
 #pragma unroll
-  for (int j = 0; j < TR_PF; ++j) {
-    const int s = threadIdx.x + j * TR_THREADS;
-    if (s < nslot) {
-      const int r = s / flen, c = s - (s / flen) * flen;
+  for (int j = 0; j < PF; ++j) {
+    const int r = fc.r, c = fc.c;
+    fc.next();
+    if (fo[j] >= 0) {
       const int zl = c / 3, k = c - 3 * (c / 3);
       const T* p = cl + (r >> 1) * clen + 3 * (zl >> 1) + k;
       T v = p[0];
@@ -196,7 +237,7 @@ __device__ __forceinline__ void prolong_block(const T* __restrict__ ec, T* __res
         if (zl & 1) w = T(0.5) * (w + p[clen + 3]);
         v = T(0.5) * (v + w);
       }
-      const size_t fi = fbase + size_t(r) * fz3 + c;
+      const size_t fi = fbase + fo[j];
       bool free_dof = plane_free;
       if (mask_kind == MASK_ARRAY) free_dof = mask[fi] != 0;
       xf[fi] = (assign ? T(0) : xold[j]) + (free_dof ? v : T(0));

@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) dense_node_kernel(const T* __restrict__ u
 // One workgroup per block (restrict_block / prolong_block in grid_ops.h), XCD-aware order so
 // that the blocks sharing fine planes (restriction) or coarse rows (prolongation) meet in one L2.
 template <typename T>
-__global__ void __launch_bounds__(TR_THREADS) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
+__global__ void __launch_bounds__(TR_RESTRICT::NT) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
                                                              const GridArgs<T> g, const uint8_t* __restrict__ cmask,
                                                              int cmask_kind, int nblocks) {
   extern __shared__ unsigned char smem_raw[];
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(TR_THREADS) restrict_kernel(const T* __restric
 
 // prolong: xf (+)= mask_f * P ec.
 template <typename T>
-__global__ void __launch_bounds__(TR_THREADS) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
+__global__ void __launch_bounds__(TR_PROLONG::NT) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
                                                             const GridArgs<T> g, int assign, int nblocks) {
   extern __shared__ unsigned char smem_raw[];
   const int blk = xcd_logical(blockIdx.x, gridDim.x);
@@ -284,18 +284,18 @@ int dense_launch(const T* u, const T* E, const T* ke, T* y, int nx, int ny, int 
 
 template <typename T>
 int restrict_launch(const T* rf, T* rc, const GridArgs<T>& g, const uint8_t* cmask, int cmask_kind, hipStream_t st) {
-  const TransferGeom tg = restrict_geom(g.nx, g.ny, g.nz);
+  const TransferGeom tg = restrict_geom<TR_RESTRICT>(g.nx, g.ny, g.nz);
   const unsigned grid = unsigned((tg.nblocks + 7) / 8) * 8;
-  hipLaunchKernelGGL(restrict_kernel<T>, dim3(grid), dim3(TR_THREADS), tg.lds * sizeof(T), st, rf, rc, g, cmask,
+  hipLaunchKernelGGL(restrict_kernel<T>, dim3(grid), dim3(TR_RESTRICT::NT), tg.lds * sizeof(T), st, rf, rc, g, cmask,
                      cmask_kind, tg.nblocks);
   return hip_status(hipGetLastError(), "restrict_kernel");
 }
 
 template <typename T>
 int prolong_launch(const T* ec, T* xf, const GridArgs<T>& g, int assign, hipStream_t st) {
-  const TransferGeom tg = prolong_geom(g.nx, g.ny, g.nz);
+  const TransferGeom tg = prolong_geom<TR_PROLONG>(g.nx, g.ny, g.nz);
   const unsigned grid = unsigned((tg.nblocks + 7) / 8) * 8;
-  hipLaunchKernelGGL(prolong_kernel<T>, dim3(grid), dim3(TR_THREADS), tg.lds * sizeof(T), st, ec, xf, g, assign,
+  hipLaunchKernelGGL(prolong_kernel<T>, dim3(grid), dim3(TR_PROLONG::NT), tg.lds * sizeof(T), st, ec, xf, g, assign,
                      tg.nblocks);
   return hip_status(hipGetLastError(), "prolong_kernel");
 }
