@@ -21,12 +21,12 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 3
+#define VH_ABI_VERSION 4
 
 enum { VH_F64 = 0, VH_F32 = 1, VH_BF16 = 2 };
 enum { VH_MASK_NONE = 0, VH_MASK_ARRAY = 1, VH_MASK_X0 = 2 };
 enum { VH_MODE_RAW = 0, VH_MODE_RESID = 1, VH_MODE_JACOBI = 2, VH_MODE_CHEB = 3, VH_MODE_DOT = 4,
-       VH_MODE_CHEB3 = 5 };
+       VH_MODE_CHEB3 = 5, VH_MODE_CHEB3D = 6 };
 enum { VH_RED_DOT = 0, VH_RED_DOT2 = 1, VH_RED_DOTDIFF = 2, VH_RED_NORM2 = 3 };
 enum { VH_OP_NONE = 0, VH_OP_ALPHA = 1, VH_OP_BETA_FR = 2, VH_OP_BETA_PR = 3, VH_OP_SET_RZ = 4,
        VH_OP_INVSQRT = 5 /* scal[dst] = 1 / sqrt(sum): a device-side normalisation factor */,
@@ -64,7 +64,11 @@ int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, con
  * DOT y = A u and partial[t] = sum of u . A u over tile t (FP64, length vh_matvec_tiles());
  *     y may be NULL in a DOT pass (only the partials are kept);
  * CHEB3 y = u + cheb_a (dinv (b - A u) + cheb_c (u - y)), y holding the previous iterate on
- * entry (not read when cheb_c == 0): the three-term form of CHEB without a direction array.
+ * entry (or prev holding it; not read when cheb_c == 0): the three-term form of CHEB without a
+ * direction array; CHEB3D the same step and the two dots of the field list below.
+ * storage = VH_F32 with dtype = VH_F64: u in FP32 for a DOT pass with the AXPY load (the
+ * mixed-precision Krylov loop reads its FP32 preconditioned residual directly); the load-time
+ * combination's w is in the compute dtype.
  * (Load-time combination and extra outputs of DOT / RESID: see the w, sdev, u_out, y32 fields.)
  * JACOBI/CHEB/CHEB3 with dinv == NULL compute D^-1 = 1 / (kdiag[c] * sum of the incident E)
  * at free DOFs and 1 at fixed ones (kdiag: the diagonal of ke per component, which an
@@ -95,6 +99,12 @@ typedef struct vh_epilogue {
   void* u_out;
   float* y32;
   double load_scale;
+  /* CHEB3 / CHEB3D: the previous iterate, when it is not held by y on entry (y is then written
+   * only).  CHEB3D (the last smoothing step of a V-cycle fused with the Polak-Ribiere dots of the
+   * Krylov loop): partial[t] = sum of b . y and partial[n_tiles + t] = sum of b . (y - zold) over
+   * the DOFs of tile t (FP64; length 2 vh_matvec_tiles()). */
+  const void* prev;
+  const void* zold;
 } vh_epilogue;
 #define VH_PREV_ZERO 1
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,

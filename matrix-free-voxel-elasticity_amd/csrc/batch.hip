@@ -243,6 +243,7 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
     p.y32 = static_cast<float*>(ep.y32);
     p.fixed_zero = ep.fixed_zero;
     p.load_scale = T(ep.load_scale);
+    p.prev = static_cast<const T*>(ep.prev);
     const MatvecPlan pl = matvec_batch_plan<T>(it.nx, it.ny, it.nz, ep.lx, tile);
     p.tile = pl.tile;
     p.lx = pl.lx;
@@ -320,6 +321,8 @@ int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_
                "vh_product_batch_create: item %d grid", i);
     VH_REQUIRE(it.ep.mode == mode && (it.ep.w != nullptr) == combine, "vh_product_batch_create: item %d differs", i);
     VH_REQUIRE(mask_kind != MASK_ARRAY || it.mask, "vh_product_batch_create: item %d mask", i);
+    VH_REQUIRE(!it.ep.prev || (mode == MODE_CHEB3 && !combine && it.ep.prev != it.y),
+               "vh_product_batch_create: item %d prev", i);
     VH_REQUIRE(!(mode == MODE_RESID || mode == MODE_JACOBI || mode == MODE_CHEB3) || it.ep.b,
                "vh_product_batch_create: item %d b", i);
     VH_REQUIRE(mode != MODE_DOT || it.ep.partial, "vh_product_batch_create: item %d partial", i);

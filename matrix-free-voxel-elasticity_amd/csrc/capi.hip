@@ -160,6 +160,8 @@ static int matvec_t(int storage, const double* blocks72, const void* E, const vo
     p.y32 = static_cast<float*>(ep->y32);
     p.fixed_zero = ep->fixed_zero;
     p.load_scale = T(ep->load_scale);
+    p.prev = static_cast<const T*>(ep->prev);
+    p.zold = static_cast<const T*>(ep->zold);
   }
   return matvec_launch<T>(p, storage, mode, mask_kind, st);
 }
@@ -209,20 +211,25 @@ int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int 
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y, int nx,
                     int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep, void* stream) {
   VH_REQUIRE(dtype_ok(dtype), "vh_matvec: dtype %d not supported", dtype);
-  VH_REQUIRE(storage == dtype || (storage == BF16 && dtype == F32), "vh_matvec: storage %d with dtype %d", storage, dtype);
+  VH_REQUIRE(storage == dtype || (storage == BF16 && dtype == F32) ||
+                 (storage == F32 && dtype == F64 && ep && ep->mode == MODE_DOT && ep->w),
+             "vh_matvec: storage %d with dtype %d", storage, dtype);
   VH_REQUIRE(blocks72 && E && u && (y || (ep && ep->mode == MODE_DOT)), "vh_matvec: null pointer");
   VH_REQUIRE(u != y, "vh_matvec: y must not alias u");
   VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec: bad grid %d x %d x %d", nx, ny, nz);
   VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2, "vh_matvec: bad mask kind %d", mask_kind);
   VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_matvec: mask array missing");
   if (ep) {
-    VH_REQUIRE(ep->mode >= 0 && ep->mode <= MODE_CHEB3, "vh_matvec: bad mode %d", ep->mode);
+    VH_REQUIRE(ep->mode >= 0 && ep->mode <= MODE_CHEB3D, "vh_matvec: bad mode %d", ep->mode);
     const int m = ep->mode;
-    const bool smoother = m == MODE_JACOBI || m == MODE_CHEB || m == MODE_CHEB3;
+    const bool smoother = m == MODE_JACOBI || m == MODE_CHEB || m == MODE_CHEB3 || m == MODE_CHEB3D;
+    VH_REQUIRE(m != MODE_CHEB3D || (ep->partial && ep->zold && ep->zold != y), "vh_matvec: CHEB3D needs partial, zold");
+    VH_REQUIRE(!ep->prev || ((m == MODE_CHEB3 || m == MODE_CHEB3D) && ep->prev != y),
+               "vh_matvec: prev is the previous iterate of a CHEB3 pass (not y)");
     VH_REQUIRE(!(m == MODE_RESID || smoother) || ep->b, "vh_matvec: b missing");
     VH_REQUIRE(m != MODE_CHEB || ep->d, "vh_matvec: d missing");
     VH_REQUIRE(m != MODE_DOT || ep->partial, "vh_matvec: partial missing");
-    VH_REQUIRE(!ep->w || (((m == MODE_DOT || m == MODE_RESID) && ep->sdev) || m == MODE_CHEB3),
+    VH_REQUIRE(!ep->w || (((m == MODE_DOT || m == MODE_RESID) && ep->sdev) || (m == MODE_CHEB3 && !ep->prev)),
                "vh_matvec: the load-time combination needs a DOT/RESID pass with sdev, or a CHEB3 pass");
     VH_REQUIRE(!ep->w || (ep->u_out && ep->u_out != u && ep->u_out != y && ep->w != y && ep->u_out != ep->w &&
                           ep->u_out != ep->b),

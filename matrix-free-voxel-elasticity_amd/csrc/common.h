@@ -23,6 +23,7 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 template <typename S, typename T> __device__ __forceinline__ T load_as(const S* p, size_t i);
 template <> __device__ __forceinline__ double load_as<double, double>(const double* p, size_t i) { return p[i]; }
 template <> __device__ __forceinline__ float load_as<float, float>(const float* p, size_t i) { return p[i]; }
+template <> __device__ __forceinline__ double load_as<float, double>(const float* p, size_t i) { return double(p[i]); }
 template <> __device__ __forceinline__ float load_as<uint16_t, float>(const uint16_t* p, size_t i) {
   return bf16_to_f32(p[i]);
 }

@@ -28,8 +28,10 @@ enum Mode : int {
   MODE_JACOBI = 2,  // y = u + omega * dinv * (b - A u)
   MODE_CHEB = 3,    // d = cheb_a * (dinv (b - A u) + cheb_c * d);  y = u + d
   MODE_DOT = 4,     // y = A u;  partial[tile] = sum over owned DOFs of u . A u
-  MODE_CHEB3 = 5,   // three-term Chebyshev: y (holding x_prev on entry; not read if cheb_c == 0)
-                    //   <- u + cheb_a * (dinv (b - A u) + cheb_c * (u - y))
+  MODE_CHEB3 = 5,   // three-term Chebyshev: y (holding x_prev on entry, or prev holds it; not read
+                    //   if cheb_c == 0) <- u + cheb_a * (dinv (b - A u) + cheb_c * (u - x_prev))
+  MODE_CHEB3D = 6,  // CHEB3, then partial[t] = sum of b . y and partial[n_tiles + t] = sum of
+                    //   b . (y - zold) over tile t's DOFs (the Polak-Ribiere dots of the last step)
 };
 
 template <typename T>
@@ -50,6 +52,8 @@ struct MatvecParams {
   int sidx;
   T* u_out;
   float* y32;             // RESID: FP32 copy of the output (the V-cycle right-hand side)
+  const T* prev;          // CHEB3 / CHEB3D: the previous iterate (default: y on entry)
+  const T* zold;          // CHEB3D: the vector the new output is compared with
   int nx, ny, nz;
   int lx;                 // x chunk length (0: automatic)
   int tile;               // tile variant (MV_TILE_*)

@@ -121,7 +121,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   __shared__ T se[PLANE];             // per element column: E(ex-1) + E(ex), for D^-1 on the fly
   __shared__ T sE[BLOCK];             // this thread's prefetched modulus, parked across the barrier
   __shared__ double red[BLOCK / 64];
-  constexpr bool NEEDS_DINV = MODE == MODE_JACOBI || MODE == MODE_CHEB || MODE == MODE_CHEB3;
+  constexpr bool NEEDS_DINV = MODE == MODE_JACOBI || MODE == MODE_CHEB || MODE == MODE_CHEB3 || MODE == MODE_CHEB3D;
   // FP64 smoothing passes recompute the left face from the still-staged plane instead of
   // carrying it in 24 registers, which keeps them within the 168 VGPRs of 3 waves per SIMD
   constexpr bool RECOMPUTE_FL = sizeof(T) == 8 && NEEDS_DINV;
@@ -155,7 +155,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   }
 
   const S* __restrict__ u = reinterpret_cast<const S*>(a.u);
-  const S* __restrict__ wv = reinterpret_cast<const S*>(a.w);
+  const T* __restrict__ wv = reinterpret_cast<const T*>(a.w);     // combination operand, compute type
   const T* __restrict__ E = a.E;
   constexpr bool COMBINE = LOADOP != LOAD_NONE;
   const T sw = LOADOP == LOAD_AXPY ? T(a.sdev[a.sidx]) : (LOADOP == LOAD_MUL ? a.load_scale : T(0));
@@ -199,7 +199,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int j = 0; j < LOADS; ++j) {
       const size_t dof = base + ld_off[j];
       pre[j] = load_as<S, T>(u, dof);
-      if (COMBINE) pre_w[j] = load_as<S, T>(wv, dof);
+      if (COMBINE) pre_w[j] = wv[dof];
       if (MASK == MASK_ARRAY) pre_m[j] = a.mask[dof];
       keep |= unsigned(plane_ok && ((ld_okbits >> j) & 1u)) << j;
     }
@@ -259,7 +259,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 #pragma unroll
     for (int c = 0; c < 3; ++c) Gc[q][c] = T(0);
 
-  double dot_acc = 0.0;
+  double dot_acc = 0.0, dot_acc2 = 0.0;
   T Eprev = T(0);                  // modulus of this column in element plane ex - 1
 
   // ---- epilogue operands of this thread's owned node: b, the previous iterate / d, dinv (and u
@@ -268,9 +268,9 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   constexpr bool NEED_B = MODE == MODE_RESID || NEEDS_DINV;
   constexpr bool NEED_U = MODE != MODE_RAW || MASK != MASK_NONE || COMBINE;
   // (issued at the start of step ex, they land during the element compute)
-  T ep_b[3], ep_p[3], ep_di[3], ep_u[3];
+  T ep_b[3], ep_p[3], ep_di[3], ep_u[3], ep_z[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = T(0);
+  for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = ep_z[c] = T(0);
   const int jn_own = (ly + 1) * NZ + (lz + 1);
   auto fetch_epilogue = [&](int ix) {
     const bool use = node_own && ix >= ox0 && ix < ox1;
@@ -279,12 +279,14 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int c = 0; c < 3; ++c) {
       if (NEED_U && MASK == MASK_ARRAY) {       // the staged plane is masked: fetch the input
         ep_u[c] = load_as<S, T>(u, 3 * node + c);
-        if (LOADOP == LOAD_AXPY) ep_u[c] += sw * load_as<S, T>(wv, 3 * node + c);
-        if (LOADOP == LOAD_MUL) ep_u[c] = sw * (ep_u[c] * load_as<S, T>(wv, 3 * node + c));
+        if (LOADOP == LOAD_AXPY) ep_u[c] += sw * wv[3 * node + c];
+        if (LOADOP == LOAD_MUL) ep_u[c] = sw * (ep_u[c] * wv[3 * node + c]);
       }
       if (NEED_B) ep_b[c] = a.b[3 * node + c];
       if (MODE == MODE_CHEB) ep_p[c] = a.d[3 * node + c];
-      if (MODE == MODE_CHEB3) ep_p[c] = (a.cheb_c == T(0) || (a.flags & MV_PREV_ZERO)) ? T(0) : a.y[3 * node + c];
+      if (MODE == MODE_CHEB3 || MODE == MODE_CHEB3D)
+        ep_p[c] = (a.cheb_c == T(0) || (a.flags & MV_PREV_ZERO)) ? T(0) : (a.prev ? a.prev : a.y)[3 * node + c];
+      if (MODE == MODE_CHEB3D) ep_z[c] = a.zold[3 * node + c];
       if (NEEDS_DINV && !dinv_from_E) ep_di[c] = a.dinv[3 * node + c];
     }
   };
@@ -406,6 +408,11 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
           dot_acc += double(uo) * double(au);
         }
         if (MASK != MASK_NONE && a.fixed_zero && !free_dof) out = T(0);
+        if (MODE == MODE_CHEB3D) {     // r . z_new and r . (z_new - z_old) of the Krylov update
+          const double rb = double(ep_b[c]), zo = double(out);
+          dot_acc += rb * zo;
+          dot_acc2 += rb * (zo - double(ep_z[c]));
+        }
         if (MODE != MODE_DOT || a.y) a.y[dof] = out;      // a DOT pass may keep only p.Ap
         if (COMBINE) a.u_out[dof] = uo;
         if (MODE == MODE_RESID && a.y32) a.y32[dof] = float(out);
@@ -424,6 +431,15 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   if (MODE == MODE_DOT || (MODE == MODE_RESID && a.partial != nullptr)) {
     const double s = block_sum<double, BLOCK>(dot_acc, red);
     if (tid == 0) a.partial[logical] = s;
+  }
+  if constexpr (MODE == MODE_CHEB3D) {
+    __shared__ double red2[BLOCK / 64];
+    const double s0 = block_sum<double, BLOCK>(dot_acc, red);
+    const double s1 = block_sum<double, BLOCK>(dot_acc2, red2);
+    if (tid == 0) {
+      a.partial[logical] = s0;
+      a.partial[a.n_tiles + logical] = s1;
+    }
   }
 }
 
@@ -574,6 +590,16 @@ static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
   }
 }
 
+template <int MASK>
+static void launch_dot_f32_input(const MatvecParams<double>& p, int grid, hipStream_t st) {
+  if (p.tile == MV_TILE_8x32)
+    hipLaunchKernelGGL((parity_xmarch<double, float, MODE_DOT, MASK, Tile8, true, LOAD_AXPY>), dim3(grid),
+                       dim3(Tile8::BLOCK), 0, st, p);
+  else
+    hipLaunchKernelGGL((parity_xmarch<double, float, MODE_DOT, MASK, Tile16, true, LOAD_AXPY>), dim3(grid),
+                       dim3(Tile16::BLOCK), 0, st, p);
+}
+
 template <typename T, typename S, int MODE>
 static void launch_mask(const MatvecParams<T>& p, int mask_kind, int grid, hipStream_t st) {
   switch (mask_kind) {
@@ -591,6 +617,7 @@ static void launch_mode(const MatvecParams<T>& p, int mode, int mask_kind, int g
     case MODE_JACOBI: launch_mask<T, S, MODE_JACOBI>(p, mask_kind, grid, st); break;
     case MODE_CHEB: launch_mask<T, S, MODE_CHEB>(p, mask_kind, grid, st); break;
     case MODE_CHEB3: launch_mask<T, S, MODE_CHEB3>(p, mask_kind, grid, st); break;
+    case MODE_CHEB3D: launch_mask<T, S, MODE_CHEB3D>(p, mask_kind, grid, st); break;
     default: launch_mask<T, S, MODE_DOT>(p, mask_kind, grid, st); break;
   }
 }
@@ -608,6 +635,17 @@ int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipSt
   if (storage == BF16) {
     if constexpr (sizeof(T) == 4) launch_mode<T, uint16_t>(p, mode, mask_kind, pl.grid, st);
     else return fail(-2, "bf16 storage requires fp32 compute");
+  } else if (storage == F32 && sizeof(T) == 8) {
+    // FP32 input of an FP64 pass: the DOT pass of the mixed-precision Krylov loop, which forms
+    // p = z + beta p_old from the FP32 preconditioned residual z (voxel-cube blocks, AXPY load)
+    if constexpr (sizeof(T) == 8) {
+      if (mode != MODE_DOT || !p.w || !p.cube) return fail(-2, "fp32 input of an fp64 pass: DOT with AXPY load only");
+      switch (mask_kind) {
+        case MASK_NONE: launch_dot_f32_input<MASK_NONE>(p, pl.grid, st); break;
+        case MASK_ARRAY: launch_dot_f32_input<MASK_ARRAY>(p, pl.grid, st); break;
+        default: launch_dot_f32_input<MASK_X0>(p, pl.grid, st); break;
+      }
+    }
   } else {
     launch_mode<T, T>(p, mode, mask_kind, pl.grid, st);
   }

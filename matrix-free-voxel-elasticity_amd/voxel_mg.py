@@ -192,7 +192,8 @@ class Level:
             return
         self.launch = getattr(self, "launch", {})
         for mode, combine in passes:
-            dinv_array = mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3) and self.product.kdiag is None
+            dinv_array = (mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3, L.MODE_CHEB3D)
+                          and self.product.kdiag is None)
             self.launch[(mode, bool(combine))] = tuner.tune(self, mode, combine, dinv_array)
 
 
@@ -264,11 +265,24 @@ class Multigrid:
             return {"kdiag": kd}
         return {"dinv": Lv.inv_diag}
 
-    def _smooth(self, level, x, r, sweeps, first):
-        """Returns the buffer holding the smoothed x (x or its ping-pong partner)."""
+    def _smooth(self, level, x, r, sweeps, first, out=None, dots=None):
+        """Returns the buffer holding the smoothed x (x or its ping-pong partner).  With `out`
+        (post-smoothing, Chebyshev) the last step writes there instead and returns it; with
+        dots = (partial, zold) that step also forms the two Polak-Ribiere dots r . out and
+        r . (out - zold) per tile (MODE_CHEB3D)."""
         Lv, w = self.levels[level], self._work[level]
         other = w["x2"] if x is w["x"] else w["x"]
         dk = self._dinv_kw(Lv)
+        if out is not None and (first or self.smoother != "chebyshev"):
+            raise ValueError("out= is for the Chebyshev post-smoothing")
+
+        def last_pass(xin, partner, **kw):
+            if dots is not None:
+                Lv.matvec(xin, out, mode=L.MODE_CHEB3D, b=r, prev=partner, partial=dots[0], zold=dots[1], **kw)
+            else:
+                Lv.matvec(xin, out, mode=L.MODE_CHEB3, b=r, prev=partner, **kw)
+            return out
+
         if self.smoother == "chebyshev":
             # three-term form x_{k+1} = x_k + a_k (z_k + c_k (x_k - x_{k-1})), d_k = x_{k+1} - x_k;
             # the pass overwrites x_{k-1} (held by the ping-pong partner) with x_{k+1}
@@ -288,6 +302,8 @@ class Multigrid:
             elif first:      # x0 = 0: x1 = D^-1 r / theta, and x_{k-1} = 0 at the next step
                 ops.axpby(1.0 / theta, r, 0.0, x, w=Lv.inv_diag)
                 prev_zero = True
+            elif out is not None and sweeps == 1:
+                return last_pass(x, other, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
             else:
                 Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
                 x, other = other, x
@@ -295,6 +311,8 @@ class Multigrid:
                 c = 0.25 * delta * delta * alpha
                 alpha = 1.0 / (theta - c)
                 cd = c * theta if (self.cheb_variant == "donor" and k == 1) else c
+                if out is not None and k == sweeps - 1:
+                    return last_pass(x, other, cheb_a=alpha, cheb_c=cd, prev_zero=prev_zero, **dk)
                 Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=alpha, cheb_c=cd, prev_zero=prev_zero, **dk)
                 prev_zero = False
                 x, other = other, x
@@ -312,10 +330,12 @@ class Multigrid:
         y = out if out is not None else torch.empty_like(r)
         return ops.dense_mv(self.coarse_inverse, r, y, mask=self._coarse_mask, n=r.numel())
 
-    def vcycle(self, r, level=0):
+    def vcycle(self, r, level=0, out=None, dots=None):
+        """One V-cycle on r.  out / dots (finest level, Chebyshev smoother): the result is
+        written to `out` by the last smoothing step, which also forms the dots of _smooth."""
         Lv, w = self.levels[level], self._work[level]
         if level == len(self.levels) - 1:
-            return self.coarse_solve(r, w["x"])
+            return self.coarse_solve(r, w["x"] if out is None else out)
         x = self._smooth(level, w["x"], r, self.pre, True)
         Lv.matvec(x, w["res"], mode=L.MODE_RESID, b=r, fixed_zero=True)      # (r - A x) M
         nxt = self.levels[level + 1]
@@ -323,7 +343,7 @@ class Multigrid:
         ops.restrict(w["res"], rc, Lv.shape, cmask=nxt.maskobj)
         ec = self.vcycle(rc, level + 1)
         ops.prolong(ec, x, Lv.shape, mask=Lv.maskobj)                          # x += P ec M
-        return self._smooth(level, x, r, self.post, False)
+        return self._smooth(level, x, r, self.post, False, out=out, dots=dots)
 
     def __call__(self, r, out=None):
         w0 = self._work[0]

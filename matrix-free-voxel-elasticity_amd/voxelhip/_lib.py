@@ -10,11 +10,11 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libvoxelhip.so"
-ABI_VERSION = 3                 # VH_ABI_VERSION of include/voxelhip.h these bindings follow
+ABI_VERSION = 4                 # VH_ABI_VERSION of include/voxelhip.h these bindings follow
 
 F64, F32, BF16 = 0, 1, 2
 MASK_NONE, MASK_ARRAY, MASK_X0 = 0, 1, 2
-MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT, MODE_CHEB3 = 0, 1, 2, 3, 4, 5
+MODE_RAW, MODE_RESID, MODE_JACOBI, MODE_CHEB, MODE_DOT, MODE_CHEB3, MODE_CHEB3D = 0, 1, 2, 3, 4, 5, 6
 RED_DOT, RED_DOT2, RED_DOTDIFF, RED_NORM2 = 0, 1, 2, 3
 OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ, OP_INVSQRT, OP_RESNORM = 0, 1, 2, 3, 4, 5, 6
 S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
@@ -32,7 +32,7 @@ class Epilogue(ctypes.Structure):
     _fields_ = [("mode", _i), ("fixed_zero", _i), ("b", _vp), ("dinv", _vp), ("d", _vp),
                 ("partial", _vp), ("omega", _d), ("cheb_a", _d), ("cheb_c", _d), ("lx", _i), ("tile", _i),
                 ("kdiag", _d * 3), ("flags", _i), ("w", _vp), ("sdev", _vp), ("sidx", _i),
-                ("u_out", _vp), ("y32", _vp), ("load_scale", _d)]
+                ("u_out", _vp), ("y32", _vp), ("load_scale", _d), ("prev", _vp), ("zold", _vp)]
 
 
 class ProductItem(ctypes.Structure):

@@ -78,9 +78,15 @@ class GMGSolver:
                                             post=self.post, lower_frac=self.lower_frac,
                                             omega_iterations=self.omega_iterations,
                                             coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin)
+        # the last post-smoothing step of the V-cycle forms the Polak-Ribiere dots itself (and
+        # writes z in the V-cycle precision, which the DOT pass reads directly)
+        self.fused_pr = (self.smoother == "chebyshev" and len(self.shapes) > 1 and self.degree >= 1
+                         and self.precision != "ir")
         if self.tuner is not None:
             self.M.tune(self.tuner)
             self.fine.tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True), (L.MODE_RESID, False)])
+            if self.fused_pr:
+                self.M.levels[0].tune(self.tuner, [(L.MODE_CHEB3D, False)])
             if self.precision == "ir":
                 self.M.levels[0].tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True)])
         self._alloc()
@@ -109,8 +115,9 @@ class GMGSolver:
             self.t_dot32, self.t_res32 = f32.tiles(L.MODE_DOT, True), f32.tiles(L.MODE_RESID, True)
         else:
             self.t_dot32 = self.t_res32 = 0
+        self.t_pr = f32.tiles(L.MODE_CHEB3D, False) if self.fused_pr else 0
         self.partial = torch.zeros(2 * max(self.parts, self.t_dot, self.t_res, self.t_res0, self.t_dot32,
-                                           self.t_res32), dtype=torch.float64, device=self.dev)
+                                           self.t_res32, self.t_pr), dtype=torch.float64, device=self.dev)
         self.scal = torch.zeros(16, dtype=torch.float64, device=self.dev)
         # right-hand side of the finest V-cycle level: the FP32 copy of r written by the
         # residual pass (mixed), the inner residual (ir), or r itself (FP64)
@@ -121,7 +128,8 @@ class GMGSolver:
             self.b32 = z(f32)
             self.ds, self.ps, self.zs = [z(f32), z(f32)], [z(f32), z(f32)], [z(f32), z(f32)]
         else:
-            self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
+            zdt = self.pdtype if self.fused_pr else torch.float64
+            self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(zdt), z(zdt)]
 
     # ------------------------------------------------------------------ one iteration
     def _iteration(self, i):
@@ -129,7 +137,8 @@ class GMGSolver:
 
         DOT pass:   p_i = z_i + beta p_o (read on the fly), p_i.A p_i (A p_i not stored) -> alpha
         RESID pass: x_i = x_o + alpha p_i (on the fly), r = b - A x_i, |r|^2, r -> fp32
-        V-cycle on r, then z_o = its output with r.z_o and r.(z_o - z_i)          -> beta, r.z
+        V-cycle on r into z_o; its last smoothing step forms r.z_o and r.(z_o - z_i) -> beta, r.z
+        (z in the V-cycle precision; the Jacobi smoother converts and dots in a separate pass)
         """
         o = 1 - i
         f, L_ = self.fine, L
@@ -140,9 +149,13 @@ class GMGSolver:
                  w=self.ps[i], sdev=self.scal, sidx=L_.S_ALPHA, u_out=self.xs[i],
                  y32=self.rhs0 if self.pdtype != torch.float64 else None)
         ops.finish(self.partial, self.t_res, 1, self.scal, 8, L_.OP_RESNORM)
-        zl = self.M.vcycle(self.rhs0)
-        ops.cvt_dotdiff(zl, self.r, self.zs[i], self.zs[o], self.partial)
-        ops.finish(self.partial, self.parts, 2, self.scal, 5, L_.OP_BETA_PR)
+        if self.fused_pr:
+            self.M.vcycle(self.rhs0, out=self.zs[o], dots=(self.partial, self.zs[i]))
+            ops.finish(self.partial, self.t_pr, 2, self.scal, 5, L_.OP_BETA_PR)
+        else:
+            zl = self.M.vcycle(self.rhs0)
+            ops.cvt_dotdiff(zl, self.r, self.zs[i], self.zs[o], self.partial)
+            ops.finish(self.partial, self.parts, 2, self.scal, 5, L_.OP_BETA_PR)
 
     def _inner_iteration(self, i):
         """FP32 flexible-CG iteration of the refinement's inner solve A d = b32 (parity i);
@@ -203,9 +216,13 @@ class GMGSolver:
             ops.convert(self.b, self.rhs0)
         else:
             self.r.copy_(self.b)
-        zl = self.M.vcycle(self.rhs0)
-        ops.convert(zl, self.zs[0])
-        ops.reduce(L.RED_DOT, self.b, self.zs[0], partial=self.partial)
+        if self.fused_pr:
+            self.M.vcycle(self.rhs0, out=self.zs[0])
+            ops.reduce(L.RED_DOT, self.rhs0, self.zs[0], partial=self.partial)
+        else:
+            zl = self.M.vcycle(self.rhs0)
+            ops.convert(zl, self.zs[0])
+            ops.reduce(L.RED_DOT, self.b, self.zs[0], partial=self.partial)
         ops.finish(self.partial, self.parts, 1, self.scal, L.S_RZ)
         self.scal[L.S_BETA] = 0.0
         # S_DONE, S_TOL2, S_ITS

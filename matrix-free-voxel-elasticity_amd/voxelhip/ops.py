@@ -119,8 +119,10 @@ class Mask:
 # --------------------------------------------------------------------------- operator
 def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None, d=None,
            partial=None, omega=0.0, cheb_a=0.0, cheb_c=0.0, fixed_zero=False, lx=0, storage=None, tile=0,
-           kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None, load_scale=1.0):
-    """One pass of the parity product with a fused epilogue (see include/voxelhip.h)."""
+           kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None, load_scale=1.0,
+           prev=None, zold=None):
+    """One pass of the parity product with a fused epilogue (see include/voxelhip.h).  An FP32
+    u with FP64 moduli selects the FP32-input DOT pass of the mixed-precision Krylov loop."""
     _dev(E, "E"); _dev(u, "u")
     if y is None and mode != L.MODE_DOT:
         raise ValueError("only a DOT pass may omit y")
@@ -129,12 +131,14 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
     nx, ny, nz = (int(s) for s in shape)
     dt = dtype_code(E)
     st = dt if storage is None else storage
+    if storage is None and u.dtype == torch.float32 and E.dtype == torch.float64:
+        st = L.F32                      # FP32 input of an FP64 pass
     if y is not None and E.dtype != y.dtype:
         raise TypeError("E and y must share the compute dtype")
     if st == L.BF16:
         if u.dtype not in (torch.bfloat16, torch.int16, torch.uint16):
             raise TypeError("bf16 storage expects a bfloat16 u")
-    elif u.dtype != E.dtype:
+    elif u.dtype != E.dtype and not (st == L.F32 and dt == L.F64):
         raise TypeError("u and y must share dtype")
     nd = 3 * n_nodes(shape)
     if u.numel() != nd or (y is not None and y.numel() != nd) or E.numel() != nx * ny * nz:
@@ -142,11 +146,11 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
     m = mask or Mask.none()
     hb = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
     kd = (ctypes.c_double * 3)(*([0.0] * 3 if kdiag is None else [float(v) for v in kdiag]))
-    if mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3) and dinv is None and kdiag is None:
+    if mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3, L.MODE_CHEB3D) and dinv is None and kdiag is None:
         raise ValueError("smoothing passes need dinv or kdiag")
     ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), _p(d), _p(partial),
                     float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, int(bool(prev_zero)),
-                    _p(w), _p(sdev), int(sidx), _p(u_out), _p(y32), float(load_scale))
+                    _p(w), _p(sdev), int(sidx), _p(u_out), _p(y32), float(load_scale), _p(prev), _p(zold))
     L.call("vh_matvec_fused", dt, st, hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
            nx, ny, nz, m.kind, m.ptr, ctypes.byref(ep), _stream())
     return y

@@ -105,11 +105,12 @@ class LaunchTuner:
         cands = _candidates(level.shape[0])
         most = max(ops.matvec_tiles(level.shape, lx=lx, tile=t, dtype=dt, blocks=level.product.blocks,
                                     storage=level.product.storage)[0] for t, lx in cands)
-        partial = torch.zeros(max(most, 1), dtype=torch.float64, device=dev)
+        partial = torch.zeros(2 * max(most, 1), dtype=torch.float64, device=dev)
         kw = {"mode": mode}
-        if mode in (L.MODE_RESID, L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3):
+        smoothing = (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3, L.MODE_CHEB3D)
+        if mode in (L.MODE_RESID,) + smoothing:
             kw["b"] = b
-        if mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3):
+        if mode in smoothing:
             if dinv_array:
                 kw["dinv"] = level.inv_diag
             else:
@@ -117,8 +118,10 @@ class LaunchTuner:
             kw.update(omega=0.5, cheb_a=0.5, cheb_c=0.25)
         if mode == L.MODE_CHEB:
             kw["d"] = vec()
-        if mode in (L.MODE_DOT, L.MODE_RESID):
+        if mode in (L.MODE_DOT, L.MODE_RESID, L.MODE_CHEB3D):
             kw["partial"] = partial
+        if mode == L.MODE_CHEB3D:
+            kw.update(zold=vec(), prev=vec())
         if combine:
             kw.update(w=w, sdev=sdev, sidx=0, u_out=uo)
         res = {}

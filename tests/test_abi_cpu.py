@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(_lib.SIGNATURES)
-    assert lib.vh_abi_version() == _lib.ABI_VERSION == 3
+    assert lib.vh_abi_version() == _lib.ABI_VERSION == 4
 
 
 def test_modal_blocks_match_the_oracle():

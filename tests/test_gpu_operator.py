@@ -362,3 +362,62 @@ def test_fused_krylov_passes():
     assert torch.equal(znew, dev(zd))
     assert float(p2[:nparts].sum()) == pytest.approx(float(r_ref @ zd), rel=1e-12)
     assert float(p2[nparts:].sum()) == pytest.approx(float(r_ref @ (zd - zold)), rel=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_chebyshev_step_with_fused_polak_ribiere_dots(dtype):
+    """CHEB3D: the three-term step with the previous iterate read from `prev` writes the same y
+    as CHEB3 in place, and its per-tile partials sum to b . y and b . (y - zold)."""
+    from voxelhip import _lib as L, ops
+    shape = (18, 17, 34)
+    rng = np.random.default_rng(31)
+    E, u = case(shape, 8)
+    m = _masks(shape, rng)["x0"]
+    mask = ops.Mask.from_values(m, shape)
+    blocks = hx.Modal(KE).blocks
+    kd = np.diag(KE)[:3]
+    b, prev, zold = (rng.standard_normal(u.size) for _ in range(3))
+    Ed, ud, bd, pd, zd = dev(E, dtype), dev(u, dtype), dev(b, dtype), dev(prev, dtype), dev(zold, dtype)
+    ref = pd.clone()
+    ops.matvec(blocks, Ed, ud, ref, shape, mask=mask, mode=L.MODE_CHEB3, b=bd, cheb_a=0.7, cheb_c=0.2, kdiag=kd)
+    nt = ops.matvec_tiles(shape, dtype=dtype, blocks=blocks)[0]
+    partial = torch.full((2 * nt,), float("nan"), dtype=torch.float64, device="cuda")
+    y = torch.full_like(ud, float("nan"))
+    ops.matvec(blocks, Ed, ud, y, shape, mask=mask, mode=L.MODE_CHEB3D, b=bd, cheb_a=0.7, cheb_c=0.2, kdiag=kd,
+               prev=pd, zold=zd, partial=partial)
+    assert torch.equal(y, ref)
+    yh, bh, zh = (t.double().cpu().numpy() for t in (y, bd, zd))
+    tol = 1e-12 if dtype == torch.float64 else 1e-9
+    assert float(partial[:nt].sum()) == pytest.approx(float(bh @ yh), rel=tol)
+    assert float(partial[nt:].sum()) == pytest.approx(float(bh @ (yh - zh)), rel=tol)
+    # prev given for a plain CHEB3 step: y is then only written
+    y2 = torch.full_like(ud, float("nan"))
+    ops.matvec(blocks, Ed, ud, y2, shape, mask=mask, mode=L.MODE_CHEB3, b=bd, cheb_a=0.7, cheb_c=0.2, kdiag=kd,
+               prev=pd)
+    assert torch.equal(y2, ref)
+
+
+def test_dot_pass_reads_an_fp32_input():
+    """The FP64 DOT pass with the AXPY load reads an FP32 z exactly as its FP64 conversion."""
+    from voxelhip import _lib as L, ops
+    shape = (20, 12, 30)
+    rng = np.random.default_rng(32)
+    E, _ = case(shape, 9)
+    m = _masks(shape, rng)["x0"]
+    mask = ops.Mask.from_values(m, shape)
+    blocks = hx.Modal(KE).blocks
+    z32 = torch.from_numpy(rng.standard_normal(3 * hx.n_nodes(shape))).to("cuda", torch.float32)
+    pold = dev(rng.standard_normal(z32.numel()))
+    sdev = torch.zeros(16, dtype=torch.float64, device="cuda")
+    sdev[L.S_BETA] = 0.37
+    nt = ops.matvec_tiles(shape, blocks=blocks)[0]
+    out = []
+    for z in (z32, z32.double()):
+        partial = torch.zeros(nt, dtype=torch.float64, device="cuda")
+        p = torch.empty_like(pold)
+        ops.matvec(blocks, dev(E), z, None, shape, mask=mask, mode=L.MODE_DOT, partial=partial, w=pold, sdev=sdev,
+                   sidx=L.S_BETA, u_out=p)
+        out.append((p, partial))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    ph = out[1][0].cpu().numpy()
+    assert np.allclose(ph, z32.double().cpu().numpy() + 0.37 * pold.cpu().numpy(), rtol=1e-15, atol=1e-15)
