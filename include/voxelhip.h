@@ -62,7 +62,8 @@ int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, con
  * the output 0 at fixed DOFs, i.e. M K M u for RAW/DOT and (b - A u) M for RESID):  RESID y = b - A u;  JACOBI y = u + omega dinv (b - A u);
  * CHEB d = cheb_a (dinv (b - A u) + cheb_c d), y = u + d (d not read when cheb_c == 0);
  * DOT y = A u and partial[t] = sum of u . A u over tile t (FP64, length vh_matvec_tiles());
- *     y may be NULL in a DOT pass (only the partials are kept);
+ *     y may be NULL in a DOT pass (only the partials are kept), and in a RESID pass that
+ *     writes y32 (only the FP32 copy is kept);
  * CHEB3 y = u + cheb_a (dinv (b - A u) + cheb_c (u - y)), y holding the previous iterate on
  * entry (or prev holding it; not read when cheb_c == 0): the three-term form of CHEB without a
  * direction array; CHEB3D the same step and the two dots of the field list below.

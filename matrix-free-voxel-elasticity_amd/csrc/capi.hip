@@ -214,7 +214,8 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
   VH_REQUIRE(storage == dtype || (storage == BF16 && dtype == F32) ||
                  (storage == F32 && dtype == F64 && ep && ep->mode == MODE_DOT && ep->w),
              "vh_matvec: storage %d with dtype %d", storage, dtype);
-  VH_REQUIRE(blocks72 && E && u && (y || (ep && ep->mode == MODE_DOT)), "vh_matvec: null pointer");
+  VH_REQUIRE(blocks72 && E && u && (y || (ep && (ep->mode == MODE_DOT || (ep->mode == MODE_RESID && ep->y32)))),
+             "vh_matvec: null pointer");
   VH_REQUIRE(u != y, "vh_matvec: y must not alias u");
   VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec: bad grid %d x %d x %d", nx, ny, nz);
   VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2, "vh_matvec: bad mask kind %d", mask_kind);

@@ -413,7 +413,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
           dot_acc += rb * zo;
           dot_acc2 += rb * (zo - double(ep_z[c]));
         }
-        if (MODE != MODE_DOT || a.y) a.y[dof] = out;      // a DOT pass may keep only p.Ap
+        if ((MODE != MODE_DOT && MODE != MODE_RESID) || a.y) a.y[dof] = out;   // DOT / RESID: y optional
         if (COMBINE) a.u_out[dof] = uo;
         if (MODE == MODE_RESID && a.y32) a.y32[dof] = float(out);
       }

@@ -145,9 +145,12 @@ class GMGSolver:
         f.matvec(self.zs[i], None, mode=L_.MODE_DOT, partial=self.partial,
                  w=self.ps[o], sdev=self.scal, sidx=L_.S_BETA, u_out=self.ps[i])
         ops.finish(self.partial, self.t_dot, 1, self.scal, L_.S_PQ, L_.OP_ALPHA)
-        f.matvec(self.xs[o], self.r, mode=L_.MODE_RESID, b=self.b, partial=self.partial,
-                 w=self.ps[i], sdev=self.scal, sidx=L_.S_ALPHA, u_out=self.xs[i],
-                 y32=self.rhs0 if self.pdtype != torch.float64 else None)
+        # (with the fused dots and an FP32 V-cycle nothing reads the FP64 residual: only its FP32
+        #  copy, the V-cycle right-hand side, is written)
+        mixed = self.pdtype != torch.float64
+        f.matvec(self.xs[o], None if (mixed and self.fused_pr) else self.r, mode=L_.MODE_RESID, b=self.b,
+                 partial=self.partial, w=self.ps[i], sdev=self.scal, sidx=L_.S_ALPHA, u_out=self.xs[i],
+                 y32=self.rhs0 if mixed else None)
         ops.finish(self.partial, self.t_res, 1, self.scal, 8, L_.OP_RESNORM)
         if self.fused_pr:
             self.M.vcycle(self.rhs0, out=self.zs[o], dots=(self.partial, self.zs[i]))

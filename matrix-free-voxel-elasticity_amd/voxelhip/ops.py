@@ -124,8 +124,8 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
     """One pass of the parity product with a fused epilogue (see include/voxelhip.h).  An FP32
     u with FP64 moduli selects the FP32-input DOT pass of the mixed-precision Krylov loop."""
     _dev(E, "E"); _dev(u, "u")
-    if y is None and mode != L.MODE_DOT:
-        raise ValueError("only a DOT pass may omit y")
+    if y is None and not (mode == L.MODE_DOT or (mode == L.MODE_RESID and y32 is not None)):
+        raise ValueError("only a DOT pass, or a RESID pass writing y32, may omit y")
     if y is not None:
         _dev(y, "y")
     nx, ny, nz = (int(s) for s in shape)

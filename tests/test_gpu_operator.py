@@ -421,3 +421,23 @@ def test_dot_pass_reads_an_fp32_input():
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
     ph = out[1][0].cpu().numpy()
     assert np.allclose(ph, z32.double().cpu().numpy() + 0.37 * pold.cpu().numpy(), rtol=1e-15, atol=1e-15)
+
+
+def test_resid_pass_may_keep_only_its_fp32_copy():
+    from voxelhip import _lib as L, ops
+    shape = (20, 12, 30)
+    rng = np.random.default_rng(33)
+    E, u = case(shape, 10)
+    mask = ops.Mask.from_values(_masks(shape, rng)["x0"], shape)
+    blocks = hx.Modal(KE).blocks
+    b = dev(rng.standard_normal(u.size))
+    nt = ops.matvec_tiles(shape, blocks=blocks)[0]
+    outs = []
+    for keep in (True, False):
+        y = torch.empty_like(b) if keep else None
+        y32 = torch.full((u.size,), float("nan"), dtype=torch.float32, device="cuda")
+        partial = torch.zeros(nt, dtype=torch.float64, device="cuda")
+        ops.matvec(blocks, dev(E), dev(u), y, shape, mask=mask, mode=L.MODE_RESID, b=b, partial=partial, y32=y32)
+        outs.append((y, y32, partial))
+    assert torch.equal(outs[0][1], outs[1][1]) and torch.equal(outs[0][2], outs[1][2])
+    assert torch.equal(outs[0][0].float(), outs[1][1])
