@@ -259,9 +259,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batched leg")
     ap.add_argument("--simp-steps", type=int, default=100,
                     help="SIMP steps of the 384x192x192 optimisation leg (0 skips it)")
-    ap.add_argument("--only", choices=["matvec"], default=None,
+    ap.add_argument("--only", choices=["matvec", "solve"], default=None,
                     help="matvec: only the dominant kernel (FP64 product at 256x128x128) and a plain "
-                         "copy of known size, for counter collection")
+                         "copy of known size, for counter collection; solve: only the 256x128x128 "
+                         "solves of --precision (for a per-kernel trace of one solve)")
+    ap.add_argument("--precision", choices=["mixed", "fp64", "both"], default="both",
+                    help="precisions of the 256x128x128 solve legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -290,7 +293,8 @@ def main():
     ndof = force.numel()
     out = {}
     solvers = {}
-    for prec in ("mixed", "fp64"):
+    precs = ("mixed", "fp64") if args.precision == "both" else (args.precision,)
+    for prec in precs:
         s = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision=prec, graph=bool(args.graph),
                       tune=tuner)
         solvers[prec] = s
@@ -298,6 +302,9 @@ def main():
         its = sorted({x[0] for x in st})
         out[prec] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "iterations": its,
                      "converged": all(x[1] for x in st), "max_rel_residual": max(x[2] for x in st)}
+    if args.only == "solve" or args.precision != "both":
+        print(json.dumps({"only": "solve", "solves": out}), flush=True)
+        return
     # solution error of the mixed solve against the FP64 solve
     x64 = solvers["fp64"].solve(force)[0].clone()
     x32 = solvers["mixed"].solve(force)[0]
