@@ -18,8 +18,9 @@ struct Tile {
 constexpr int MV_PREV_ZERO = 1;
 // load-time operations on the input (template parameter LOADOP of the product kernel)
 constexpr int LOAD_NONE = 0, LOAD_AXPY = 1, LOAD_MUL = 2;
-// tile variants: 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads)
-constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2;
+// tile variants: 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads), 3 = 4 x 8 (64 threads: one
+// wave per workgroup, for the latency-bound coarse levels)
+constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2, MV_TILE_4x8 = 3;
 
 // Epilogue modes (A = constrained operator M K M + I - M when a mask is given, else K).
 enum Mode : int {

@@ -469,6 +469,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecPar
 // ------------------------------------------------------------------------------ launcher
 using Tile16 = Tile<16, 32>;
 using Tile8 = Tile<8, 32>;
+using Tile4 = Tile<4, 8>;
 
 static int device_cus() {
   static int cus = 0;
@@ -545,6 +546,12 @@ static MatvecPlan plan_typed(int nx, int ny, int nz, int lx_hint, int tile_hint)
   b.tile = MV_TILE_8x32;
   if (tile_hint == MV_TILE_16x32) return a;
   if (tile_hint == MV_TILE_8x32) return b;
+  if (tile_hint == MV_TILE_4x8) {
+    MatvecPlan c{};
+    plan_tile<Tile4>(nx, ny, nz, lx_hint, resident_per_cu<T, S, Tile4, CUBE>(), &c);
+    c.tile = MV_TILE_4x8;
+    return c;
+  }
   return (b.cost < a.cost) ? b : a;
 }
 
@@ -563,6 +570,16 @@ MatvecPlan matvec_plan_f32(int storage, bool cube, int nx, int ny, int nz, int l
 }
 #endif
 
+template <typename T, typename S, int MODE, int MASK, bool CUBE, int LOP>
+static void launch_tiles(const MatvecParams<T>& p, int grid, hipStream_t st) {
+  if (p.tile == MV_TILE_8x32)
+    hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, CUBE, LOP>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
+  else if (p.tile == MV_TILE_4x8)
+    hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile4, CUBE, LOP>), dim3(grid), dim3(Tile4::BLOCK), 0, st, p);
+  else
+    hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, CUBE, LOP>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
+}
+
 template <typename T, typename S, int MODE, int MASK>
 static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
   // load-time combination (voxel-cube blocks only; checked by the caller): AXPY for the Krylov
@@ -570,34 +587,17 @@ static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
   constexpr int LOP = (MODE == MODE_DOT || MODE == MODE_RESID) ? LOAD_AXPY : (MODE == MODE_CHEB3 ? LOAD_MUL : LOAD_NONE);
   if constexpr (LOP != LOAD_NONE) {
     if (p.w) {
-      if (p.tile == MV_TILE_8x32)
-        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true, LOP>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
-      else
-        hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, true, LOP>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
+      launch_tiles<T, S, MODE, MASK, true, LOP>(p, grid, st);
       return;
     }
   }
-  if (p.cube) {
-    if (p.tile == MV_TILE_8x32)
-      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, true>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
-    else
-      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, true>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
-  } else {
-    if (p.tile == MV_TILE_8x32)
-      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, false>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
-    else
-      hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, false>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
-  }
+  if (p.cube) launch_tiles<T, S, MODE, MASK, true, LOAD_NONE>(p, grid, st);
+  else launch_tiles<T, S, MODE, MASK, false, LOAD_NONE>(p, grid, st);
 }
 
 template <int MASK>
 static void launch_dot_f32_input(const MatvecParams<double>& p, int grid, hipStream_t st) {
-  if (p.tile == MV_TILE_8x32)
-    hipLaunchKernelGGL((parity_xmarch<double, float, MODE_DOT, MASK, Tile8, true, LOAD_AXPY>), dim3(grid),
-                       dim3(Tile8::BLOCK), 0, st, p);
-  else
-    hipLaunchKernelGGL((parity_xmarch<double, float, MODE_DOT, MASK, Tile16, true, LOAD_AXPY>), dim3(grid),
-                       dim3(Tile16::BLOCK), 0, st, p);
+  launch_tiles<double, float, MODE_DOT, MASK, true, LOAD_AXPY>(p, grid, st);
 }
 
 template <typename T, typename S, int MODE>

@@ -25,11 +25,14 @@ from . import ops
 
 TILES = (L.TILE_16x32, L.TILE_8x32)
 LX_CANDIDATES = (0, 1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
+# one-wave 4 x 8 tiles, for grids small enough to be latency-bound (elements)
+SMALL_GRID = 64 * 32 * 32
 
 
-def _candidates(nx):
+def _candidates(nx, n_el=None):
     seen, out = set(), []
-    for t in TILES:
+    tiles = TILES + ((L.TILE_4x8,) if n_el is not None and n_el <= SMALL_GRID else ())
+    for t in tiles:
         for lx in LX_CANDIDATES:
             lx_eff = min(lx, nx)
             if (t, lx_eff) not in seen:
@@ -102,7 +105,7 @@ class LaunchTuner:
         if level.product.storage == L.BF16:
             u = u.to(torch.bfloat16)
         sdev = torch.zeros(16, dtype=torch.float64, device=dev)
-        cands = _candidates(level.shape[0])
+        cands = _candidates(level.shape[0], level.shape[0] * level.shape[1] * level.shape[2])
         most = max(ops.matvec_tiles(level.shape, lx=lx, tile=t, dtype=dt, blocks=level.product.blocks,
                                     storage=level.product.storage)[0] for t, lx in cands)
         partial = torch.zeros(2 * max(most, 1), dtype=torch.float64, device=dev)
