@@ -221,6 +221,19 @@ int vh_cone_filter(int dtype, const void* x, const void* pre_div, const void* po
 int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w, double lmid,
                 double move, void* xnew, double* partial, int nparts, void* stream);
 
+/* Heaviside projection of the filtered density rho_bar (FP64, n elements): rho = (tanh(beta eta)
+ * + tanh(beta (rho_bar - eta))) / (tanh(beta eta) + tanh(beta (1 - eta))) (identity for beta <=
+ * 1e-12), its derivative drho, the modulus E = Emin + (E0 - Emin) rho^penal, and FP64 per-block
+ * sums of rho (nparts blocks: the volume); every output optional.
+ * Replaces: project / dproject simp_r2.py:388-404 and the modulus of run_simp :650-652. */
+int vh_heaviside(size_t n, const double* rho_bar, double beta, double eta, double penal, double E0, double Emin,
+                 double* rho, double* drho, double* E, double* partial, int nparts, void* stream);
+/* Candidate design of one OC bisection probe: rho_new = clip(clip(rho sqrt(-min(dc, -1e-30) /
+ * (lmid max(dv, 1e-30))), rho - move, rho + move), rho_lb, 1) (FP64).
+ * Replaces: the design() closure of oc_update simp_r2.py:497-507. */
+int vh_oc_design(size_t n, const double* rho, const double* dc, const double* dv, double lmid, double move,
+                 double rho_lb, double* rho_new, void* stream);
+
 /* ------------------------------------------------------------------ batched problems
  * Many voxel problems (grids of different sizes, several load cases each) run each pass of
  * their solver in ONE launch.  A batch handle holds a descriptor table in device memory,

@@ -1,4 +1,5 @@
 // C-ABI entry points of libvoxelhip.so: argument checks, dtype dispatch, host-side setup.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -508,6 +509,25 @@ int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* 
   return oc_probe_launch<float>(n, static_cast<const float*>(x), static_cast<const float*>(dc),
                                 static_cast<const float*>(dv), static_cast<const float*>(w), float(lmid), float(move),
                                 static_cast<float*>(xnew), partial, nparts, st);
+}
+
+int vh_heaviside(size_t n, const double* rho_bar, double beta, double eta, double penal, double E0, double Emin,
+                 double* rho, double* drho, double* E, double* partial, int nparts, void* stream) {
+  VH_REQUIRE(rho_bar && n >= 1 && (rho || drho || E || partial), "vh_heaviside: bad arguments");
+  VH_REQUIRE(!partial || nparts >= 1, "vh_heaviside: nparts");
+  VH_REQUIRE(beta >= 0 && eta > 0 && eta < 1, "vh_heaviside: beta %g, eta %g", beta, eta);
+  const double tb = std::tanh(beta * eta);
+  const double den = tb + std::tanh(beta * (1.0 - eta));
+  const int grid = partial ? nparts : int(std::min<size_t>((n + 255) / 256, 4096));
+  return heaviside_launch(n, rho_bar, beta, eta, tb, den, penal, E0, Emin, rho, drho, E, partial, grid,
+                          static_cast<hipStream_t>(stream));
+}
+
+int vh_oc_design(size_t n, const double* rho, const double* dc, const double* dv, double lmid, double move,
+                 double rho_lb, double* rho_new, void* stream) {
+  VH_REQUIRE(rho && dc && dv && rho_new && n >= 1 && rho_new != rho, "vh_oc_design: bad arguments");
+  VH_REQUIRE(lmid > 0 && move > 0, "vh_oc_design: multiplier and move limit must be positive");
+  return oc_design_launch(n, rho, dc, dv, lmid, move, rho_lb, rho_new, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

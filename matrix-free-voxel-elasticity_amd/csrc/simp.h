@@ -17,4 +17,10 @@ template <typename T> int cone_launch(const T* x, const T* pre_div, const T* pos
 template <typename T> int oc_probe_launch(size_t n, const T* x, const T* dc, const T* dv, const T* w, T lmid, T move,
                                           T* xnew, double* partial, int nparts, hipStream_t st);
 
+int heaviside_launch(size_t n, const double* rb, double beta, double eta, double tb, double den, double penal,
+                     double E0, double Emin, double* rho, double* drho, double* E, double* partial, int nparts,
+                     hipStream_t st);
+int oc_design_launch(size_t n, const double* rho, const double* dc, const double* dv, double lmid, double move,
+                     double lb, double* rho_new, hipStream_t st);
+
 }  // namespace vh

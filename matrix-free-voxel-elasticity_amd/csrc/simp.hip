@@ -9,6 +9,8 @@
 // volume mean(H x_new / s).  Because H is symmetric, mean(H x_new / s) = sum_j x_new_j w_j / n
 // with w = H (1/s), so one probe is a single fused elementwise pass plus a deterministic
 // reduction instead of a filter application.
+#include <algorithm>
+
 #include "common.h"
 #include "simp.h"
 
@@ -55,6 +57,72 @@ __global__ void __launch_bounds__(256) oc_probe_kernel(size_t n, const T* __rest
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Heaviside projection of the filtered density (simp_r2.py:388-404) with its derivative, the
+// SIMP modulus of the projected field and per-block sums of the projected field (its volume):
+//   rho = (tb + tanh(beta (rb - eta))) / den,  drho/drb = beta / (den cosh^2(beta (rb - eta))),
+//   tb = tanh(beta eta), den = tb + tanh(beta (1 - eta)) (host), identity for beta <= 1e-12;
+//   E = Emin + (E0 - Emin) rho^penal.  Every output is optional.
+__global__ void __launch_bounds__(256) heaviside_kernel(size_t n, const double* __restrict__ rb, double beta,
+                                                       double eta, double tb, double den, double penal, double E0,
+                                                       double Emin, double* __restrict__ rho, double* __restrict__ drho,
+                                                       double* __restrict__ E, double* __restrict__ partial) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  const bool identity = beta <= 1e-12;
+  for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const double x = rb[i];
+    double r = x, d = 1.0;
+    if (!identity) {
+      const double t = beta * (x - eta);
+      r = (tb + tanh(t)) / den;
+      const double ch = cosh(t);
+      d = beta / (den * ch * ch);
+    }
+    if (rho) rho[i] = r;
+    if (drho) drho[i] = d;
+    if (E) E[i] = Emin + (E0 - Emin) * pow(r, penal);
+    acc += r;
+  }
+  if (partial) {
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+// Candidate design of one OC bisection probe on the projected formulation (simp_r2.py:497-507):
+// rho_new = clip(clip(rho sqrt(-min(dc, -1e-30) / (lmid max(dv, 1e-30))), rho - move, rho + move), lb, 1).
+__global__ void __launch_bounds__(256) oc_design_kernel(size_t n, const double* __restrict__ rho,
+                                                       const double* __restrict__ dc, const double* __restrict__ dv,
+                                                       double lmid, double move, double lb,
+                                                       double* __restrict__ rho_new) {
+  for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const double x = rho[i];
+    const double dcn = dc[i] < -1e-30 ? dc[i] : -1e-30;
+    const double dvp = dv[i] > 1e-30 ? dv[i] : 1e-30;
+    double v = x * sqrt(-dcn / (lmid * dvp));
+    v = v < x - move ? x - move : (v > x + move ? x + move : v);
+    v = v < lb ? lb : (v > 1.0 ? 1.0 : v);
+    rho_new[i] = v;
+  }
+}
+
+int heaviside_launch(size_t n, const double* rb, double beta, double eta, double tb, double den, double penal,
+                     double E0, double Emin, double* rho, double* drho, double* E, double* partial, int nparts,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(heaviside_kernel, dim3(nparts), dim3(256), 0, st, n, rb, beta, eta, tb, den, penal, E0, Emin, rho,
+                     drho, E, partial);
+  return hip_status(hipGetLastError(), "heaviside_kernel");
+}
+
+int oc_design_launch(size_t n, const double* rho, const double* dc, const double* dv, double lmid, double move,
+                     double lb, double* rho_new, hipStream_t st) {
+  const int grid = int(std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(oc_design_kernel, dim3(grid), dim3(256), 0, st, n, rho, dc, dv, lmid, move, lb, rho_new);
+  return hip_status(hipGetLastError(), "oc_design_kernel");
 }
 
 template <typename T>

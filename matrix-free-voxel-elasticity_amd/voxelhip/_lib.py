@@ -66,6 +66,8 @@ SIGNATURES = {
     "vh_coarsen_modulus": [_i, _vp, _vp, _i, _i, _i, _vp],
     "vh_dense_mv": [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
     "vh_assemble_dense": [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp],
+    "vh_heaviside": [_sz, _vp, _d, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i, _vp],
+    "vh_oc_design": [_sz, _vp, _vp, _vp, _d, _d, _d, _vp, _vp],
     "vh_galerkin_scatter": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp],
     "vh_reduce_parts": [_sz],
     "vh_reduce": [_i, _i, _sz, _vp, _vp, _vp, _vp, _vp],

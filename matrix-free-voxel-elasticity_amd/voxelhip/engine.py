@@ -191,12 +191,13 @@ class GMGSolver:
         return graphs
 
     # ------------------------------------------------------------------ solve
-    def solve(self, b, tol=1e-8, maxiter=200):
+    def solve(self, b, tol=1e-8, maxiter=200, x0=None):
         """Flexible CG (the true residual is formed and tested every iteration, as the reference
         harness runs fcg with check_every=1) to ||b - A x|| / ||b|| <= tol from x = 0.
         Returns (x, iterations, converged, rel, matvecs); x is a view of the solver's buffer.
         With precision "ir", iterations counts the inner FP32 iterations (self.last_outer the
-        outer FP64 corrections).
+        outer FP64 corrections).  x0 (optional): the initial iterate (a warm start); a start that
+        already meets tol returns it with 0 iterations.
 
         The convergence test runs on the device (OP_RESNORM freezes the recurrence once
         |r|^2 <= tol^2 |b|^2 and counts the iterations), so while the residual history shows the
@@ -204,6 +205,8 @@ class GMGSolver:
         never waits for the host between iterations, and an iteration queued past convergence
         leaves x unchanged."""
         if self.precision == "ir":
+            if x0 is not None:
+                raise ValueError("the refinement scheme starts from zero")
             return self._solve_ir(b, tol, maxiter)
         if self.graph and self._graphs is None:
             self._graphs = self._capture(self._iteration)
@@ -211,21 +214,33 @@ class GMGSolver:
         ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
         ops.finish(self.partial, self.parts, 1, self.scal, 9)
         nb = float(self.scal[9].sqrt())
-        # x = 0, z = M b, p = z (beta = 0 against a zero previous direction), rz = b.z; the first
-        # iteration forms r itself
-        self.xs[1].zero_()
+        start_mv = 0
+        if x0 is None:
+            # x = 0: r = b (the first iteration forms r itself)
+            self.xs[1].zero_()
+            r0 = self.b
+        else:
+            self.xs[1].copy_(x0)
+            self.fine.matvec(self.xs[1], self.r, mode=L.MODE_RESID, b=self.b, partial=self.partial)
+            ops.finish(self.partial, self.t_res0, 1, self.scal, 8)
+            start_mv = 1
+            rel0 = float(self.scal[8].sqrt()) / nb if nb > 0 else 0.0
+            if rel0 <= tol:
+                return self.xs[1], 0, True, rel0, start_mv
+            r0 = self.r
+        # z = M r, p = z (beta = 0 against a zero previous direction), rz = r.z
         self.ps[1].zero_()
         if self.pdtype != torch.float64:
-            ops.convert(self.b, self.rhs0)
-        else:
-            self.r.copy_(self.b)
+            ops.convert(r0, self.rhs0)
+        elif r0 is not self.r:
+            self.r.copy_(r0)
         if self.fused_pr:
             self.M.vcycle(self.rhs0, out=self.zs[0])
             ops.reduce(L.RED_DOT, self.rhs0, self.zs[0], partial=self.partial)
         else:
             zl = self.M.vcycle(self.rhs0)
             ops.convert(zl, self.zs[0])
-            ops.reduce(L.RED_DOT, self.b, self.zs[0], partial=self.partial)
+            ops.reduce(L.RED_DOT, r0, self.zs[0], partial=self.partial)
         ops.finish(self.partial, self.parts, 1, self.scal, L.S_RZ)
         self.scal[L.S_BETA] = 0.0
         # S_DONE, S_TOL2, S_ITS
@@ -266,7 +281,7 @@ class GMGSolver:
                     self._hev[q % 2].synchronize()
                 done = v[L.S_DONE] != 0.0
                 its = int(v[L.S_ITS]) if done else k
-                return self.xs[(its - 1) % 2], its, done, rel, 2 * its
+                return self.xs[(its - 1) % 2], its, done, rel, 2 * its + start_mv
             if not pending:
                 launch(k + 1)
 

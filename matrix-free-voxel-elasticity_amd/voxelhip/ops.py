@@ -326,6 +326,29 @@ def cone_filter(x, y, shape, rmin, pre_div=None, post_div=None):
     return y
 
 
+def heaviside(rho_bar, beta, eta=0.5, penal=3.0, E0=1.0, Emin=1e-9, rho=None, drho=None, E=None, partial=None):
+    """Projected density, its derivative, the SIMP modulus and per-block sums of the projected
+    density (FP64; see include/voxelhip.h, vh_heaviside).  Returns the number of partial blocks."""
+    _dev(rho_bar, "rho_bar")
+    for t in (rho_bar, rho, drho, E):
+        if t is not None and (t.dtype != torch.float64 or t.numel() != rho_bar.numel()):
+            raise TypeError("heaviside: float64 vectors of one length")
+    nparts = int(partial.numel()) if partial is not None else 0
+    L.call("vh_heaviside", int(rho_bar.numel()), _p(rho_bar), float(beta), float(eta), float(penal), float(E0),
+           float(Emin), _p(rho), _p(drho), _p(E), _p(partial), nparts, _stream())
+    return nparts
+
+
+def oc_design(rho, dc, dv, lmid, move, rho_lb, rho_new):
+    for t in (rho, dc, dv, rho_new):
+        _dev(t, "oc_design")
+        if t.dtype != torch.float64 or t.numel() != rho.numel():
+            raise TypeError("oc_design: float64 vectors of one length")
+    L.call("vh_oc_design", int(rho.numel()), _p(rho), _p(dc), _p(dv), float(lmid), float(move), float(rho_lb),
+           _p(rho_new), _stream())
+    return rho_new
+
+
 def oc_probe(x, dc, dv, w, lmid, move, xnew, partial, nparts):
     L.call("vh_oc_probe", dtype_code(x), int(x.numel()), _p(x), _p(dc), _p(dv), _p(w), float(lmid),
            float(move), _p(xnew), _p(partial), int(nparts), _stream())

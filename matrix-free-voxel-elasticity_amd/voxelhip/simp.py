@@ -17,6 +17,17 @@ hand-written adjoint behind torch.autograd, and the 100-step optimisation loop.
                                      the state problem of simp_run on the GMG-FCG engine: every
                                      step rebuilds the hierarchy for the new moduli and solves
                                      K(E) u = f to the true relative residual `tol`
+  run_simp(problem, ...)             the donor's full driver (simp_r2.py:388-433, 552-918): the
+                                     four-phase continuation of penalty, Heaviside sharpness,
+                                     move limit and filter radius, the projected OC update with a
+                                     warm-started multiplier bracket, warm-started state solves
+                                     with the fail-closed retry from zero, the physical-change and
+                                     compliance-window stopping test, and the closing solve on the
+                                     final design
+  project / dproject, schedule_at    Heaviside projection (vh_heaviside) and the continuation table
+  FilterFn, ProjectFn, design_compliance
+                                     the design gradient end to end through filter and projection
+                                     behind torch.autograd
 """
 from __future__ import annotations
 
@@ -38,6 +49,17 @@ class ConeFilter:
         ops.cone_filter(ones, self.s, self.shape, self.rmin)
         # weights of the filtered-volume identity: mean(H x / s) = sum_j x_j (H (1/s))_j / n
         self.w_vol = torch.empty_like(ones)
+        ops.cone_filter(ones, self.w_vol, self.shape, self.rmin, pre_div=self.s)
+        ops.axpby(1.0 / self.n, self.w_vol, 0.0, self.w_vol)
+
+    def set_rmin(self, rmin):
+        """Change the radius (element units) and recompute the row sums (filter_r2.py:115-129)."""
+        rmin = float(rmin)
+        if abs(rmin - self.rmin) < 1e-12:
+            return
+        self.rmin = rmin
+        ones = torch.ones_like(self.s)
+        ops.cone_filter(ones, self.s, self.shape, self.rmin)
         ops.cone_filter(ones, self.w_vol, self.shape, self.rmin, pre_div=self.s)
         ops.axpby(1.0 / self.n, self.w_vol, 0.0, self.w_vol)
 
@@ -164,3 +186,241 @@ def simp_run(problem, steps, volfrac=0.3, rmin=1.5, move=0.2, stop_change=None):
         if stop_change is not None and change <= stop_change:
             break
     return history, rho
+
+
+# --------------------------------------------------------------------------- projected SIMP driver
+# (until_iter, penal, beta, move, rmin): the donor's four-phase continuation (simp_r2.py:411-417)
+DEFAULT_SCHEDULE = [
+    (15, 1.5, 1.0, 0.20, 1.50),
+    (40, 3.5, 4.0, 0.15, 1.35),
+    (65, 4.5, 16.0, 0.08, 1.25),
+    (120, 4.5, 32.0, 0.05, 1.20),
+]
+
+
+def schedule_at(it, schedule=None):
+    """(penal, beta, move, rmin, in_last_phase) at outer iteration `it` (1-based); past the last
+    phase the parameters stay fixed (simp_r2.py:420-433)."""
+    sch = schedule or DEFAULT_SCHEDULE
+    for until, p, beta, move, rmin in sch:
+        if it <= until:
+            return p, beta, move, rmin, until == sch[-1][0]
+    p, beta, move, rmin = sch[-1][1:]
+    return p, beta, move, rmin, True
+
+
+def project(rho_bar, beta, eta=0.5, out=None):
+    y = out if out is not None else torch.empty_like(rho_bar)
+    ops.heaviside(rho_bar, beta, eta, rho=y)
+    return y
+
+
+def dproject(rho_bar, beta, eta=0.5, out=None):
+    y = out if out is not None else torch.empty_like(rho_bar)
+    ops.heaviside(rho_bar, beta, eta, drho=y)
+    return y
+
+
+def _parts(n):
+    return min(1024, max(1, (n + 255) // 256))
+
+
+def oc_update(rho, dc, dv, volfrac, move, filt, beta, eta, rho_lb, bisect_tol=1e-8, max_bisect=80,
+              lam_hint=None):
+    """One OC step bisecting on the physical (projected) volume, with the bracket seeded from
+    the previous multiplier (simp_r2.py:467-549).  Returns (rho_new, achieved_volume, lam, probes)."""
+    n = rho.numel()
+    nparts = _parts(n)
+    partial = torch.zeros(nparts, dtype=torch.float64, device=rho.device)
+    cand, fb = torch.empty_like(rho), torch.empty_like(rho)
+
+    def design(lmid):
+        ops.oc_design(rho, dc, dv, lmid, move, rho_lb, cand)
+        filt.forward(cand, out=fb)
+        ops.heaviside(fb, beta, eta, partial=partial)
+        return float(partial.sum()) / n
+
+    probes = 0
+    if lam_hint is not None and lam_hint > 0:
+        l1, l2 = lam_hint / 4.0, lam_hint * 4.0
+        for _ in range(30):              # expand until the root is bracketed
+            v1 = design(l1)
+            v2 = design(l2)
+            probes += 2
+            if v1 > volfrac and v2 <= volfrac:
+                break
+            if v1 <= volfrac:
+                l1 /= 8.0
+            if v2 > volfrac:
+                l2 *= 8.0
+            if l1 < 1e-30 or l2 > 1e30:
+                l1, l2 = 0.0, 1e9
+                break
+    else:
+        l1, l2 = 0.0, 1e9
+    lam = 0.5 * (l1 + l2)
+    achieved = design(lam)
+    probes += 1
+    for _ in range(max_bisect):
+        if (l2 - l1) <= bisect_tol * (l1 + l2 + 1e-30):
+            break
+        lam = 0.5 * (l1 + l2)
+        achieved = design(lam)
+        probes += 1
+        if achieved > volfrac:
+            l1 = lam
+        else:
+            l2 = lam
+    return cand.clone(), achieved, lam, probes
+
+
+class LinearSolveNotConverged(RuntimeError):
+    """A state solve missed its tolerance from zero as well (the fail-closed rule)."""
+
+
+def _state_solve(problem, E, x0, tol):
+    """Warm-started solve with the fail-closed retry from zero (simp_r2.py:680-708).  Returns
+    (u, iterations including a retry, rel, retried)."""
+    s = problem.solver
+    s.update(E)
+    x, it, ok, rel, _ = s.solve(problem.force, tol=tol, maxiter=problem.maxiter, x0=x0)
+    retried = False
+    if not ok and x0 is not None:
+        retried = True
+        x, it2, ok, rel, _ = s.solve(problem.force, tol=tol, maxiter=problem.maxiter)
+        it += it2
+    if not ok:
+        raise LinearSolveNotConverged(f"state solve missed tol={tol:g} (relative residual {rel:.3e} after {it} "
+                                      f"iterations)")
+    return x.clone(), it, rel, retried
+
+
+def run_simp(problem, volfrac=0.30, rmin=1.5, penal=None, E0=1.0, Emin=1e-9, rho_lb=1e-3, eta=0.5, cg_tol=1e-5,
+             warm_start=True, max_outer=400, schedule=None, conv_drho=0.01, conv_dc_rel=1e-4, conv_window=10,
+             fixed_budget=None, verbose=False):
+    """The donor's SIMP driver (simp_r2.py:552-918) on the device: filter -> Heaviside projection ->
+    modulus, warm-started GMG-FCG state solve (fail-closed), adjoint sensitivities through the
+    projection and the filter, projected OC update; stops after the last continuation phase when
+    the physical change and the compliance spread over `conv_window` steps fall below their
+    bounds (or after `fixed_budget` steps), then solves once more on the final design.  `problem`
+    is a SimpProblem (its solver, force, mask and element blocks are used).  Returns a dict with
+    the history, the solve log and the final design's compliance, volume and grayness."""
+    t_start = time.perf_counter()
+    dev = problem.force.device
+    filt = ConeFilter(problem.shape, rmin, device=dev)
+    n = filt.n
+    rho = torch.full((n,), float(volfrac), dtype=torch.float64, device=dev)
+    rho_bar, rho_phys, dprj, E = (torch.empty_like(rho) for _ in range(4))
+    ce, dc_drp = torch.empty_like(rho), torch.empty_like(rho)
+    u = None
+    history, solve_log = [], []
+    total_cg, retries = 0, 0
+    converged, stop_reason = False, "guard"
+    lam_prev, rho_phys_prev = None, None
+    budget = fixed_budget or max_outer
+    it = 0
+    while it < budget:
+        it += 1
+        p_it, beta_it, move_it, rmin_it, last_phase = schedule_at(it, schedule)
+        if penal is not None:
+            p_it = penal
+        filt.set_rmin(rmin_it)
+        filt.forward(rho, out=rho_bar)
+        ops.heaviside(rho_bar, beta_it, eta, p_it, E0, Emin, rho=rho_phys, drho=dprj, E=E)
+        t0 = time.perf_counter()
+        u, iters, rel, retried = _state_solve(problem, E, u if (warm_start and it > 1) else None, cg_tol)
+        t_solve = time.perf_counter() - t0
+        total_cg += iters
+        retries += int(retried)
+        solve_log.append({"iter": it, "cg_iters": iters, "rel_resid": rel, "retried_from_zero": retried,
+                          "solve_s": t_solve, "penal": p_it, "beta": beta_it, "rmin": filt.rmin})
+        c = float(problem.force @ u)
+        ops.element_energy(problem.blocks, u, ce, problem.shape, mask=problem.mask, rho=rho_phys, dc=dc_drp,
+                           penal=p_it, dE=E0 - Emin)
+        dc = filt.transpose(dc_drp * dprj)
+        dv = filt.transpose(dprj / n)
+        rho_old = rho
+        rho, achieved, lam, probes = oc_update(rho, dc, dv, volfrac, move_it, filt, beta_it, eta, rho_lb,
+                                               lam_hint=lam_prev)
+        lam_prev = lam
+        change = float((rho - rho_old).abs().max())
+        change_phys = float((rho_phys - rho_phys_prev).abs().max()) if rho_phys_prev is not None else float("inf")
+        rho_phys_prev = rho_phys.clone()
+        gray = float((4.0 * rho_phys * (1.0 - rho_phys)).mean())
+        history.append({"iter": it, "compliance": c, "change": change,
+                        "change_phys": None if change_phys == float("inf") else change_phys,
+                        "vol_phys": achieved, "vol_raw": float(rho.mean()), "grayness": gray, "cg_iters": iters,
+                        "rel_resid": rel, "penal": p_it, "beta": beta_it, "move": move_it, "rmin": filt.rmin,
+                        "oc_probes": probes, "retried": retried})
+        if verbose:
+            print(f"  it {it:4d}  c={c:.6f}  dr={change:.4f}  drp={change_phys:.4f}  V={achieved:.4f}  "
+                  f"g={gray:.4f}  cg={iters:4d}  p={p_it:.1f} b={beta_it:.0f}", flush=True)
+        if fixed_budget is None and last_phase and len(history) > conv_window:
+            recent = [h["compliance"] for h in history[-(conv_window + 1):]]
+            spread = (max(recent) - min(recent)) / max(abs(recent[-1]), 1e-30)
+            if change_phys <= conv_drho and spread <= conv_dc_rel:
+                converged, stop_reason = True, "outer_criterion"
+                break
+    if fixed_budget is not None:
+        stop_reason = "fixed_budget"
+    # one more solve, on the design the run ends on (simp_r2.py:834-880)
+    p_end, beta_end = schedule_at(it, schedule)[:2]
+    if penal is not None:
+        p_end = penal
+    filt.forward(rho, out=rho_bar)
+    ops.heaviside(rho_bar, beta_end, eta, p_end, E0, Emin, rho=rho_phys, E=E)
+    u_f, it_f, rel_f, _ = _state_solve(problem, E, None, cg_tol)
+    total_cg += it_f
+    solve_log.append({"iter": it + 1, "cg_iters": it_f, "rel_resid": rel_f, "retried_from_zero": False,
+                      "final_iterate": True, "penal": p_end, "beta": beta_end, "rmin": filt.rmin})
+    return {
+        "outer_iters": it, "outer_converged": converged, "stop_reason": stop_reason,
+        "final_compliance": float(problem.force @ u_f),
+        "final_compliance_pre_update": history[-1]["compliance"] if history else float("nan"),
+        "final_vol_phys": float(rho_phys.mean()),
+        "final_grayness": float((4.0 * rho_phys * (1.0 - rho_phys)).mean()),
+        "rho_phys_min": float(rho_phys.min()), "E_min_achieved": float(E.min()),
+        "total_cg_iters": total_cg, "warm_start_retries": retries,
+        "max_cg_iters_single_solve": max(s_["cg_iters"] for s_ in solve_log),
+        "max_rel_resid": max(s_["rel_resid"] for s_ in solve_log),
+        "wall_s": time.perf_counter() - t_start,
+        "history": history, "solve_log": solve_log,
+        "rho": rho, "rho_phys": rho_phys,
+    }
+
+
+class FilterFn(torch.autograd.Function):
+    """rho_bar = H x / s with backward H (g / s)."""
+
+    @staticmethod
+    def forward(ctx, x, filt):
+        ctx.filt = filt
+        return filt.forward(x.detach().contiguous())
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.filt.transpose(g.contiguous()), None
+
+
+class ProjectFn(torch.autograd.Function):
+    """Heaviside projection with its analytic derivative (vh_heaviside)."""
+
+    @staticmethod
+    def forward(ctx, rho_bar, beta, eta):
+        rb = rho_bar.detach().contiguous()
+        r, d = torch.empty_like(rb), torch.empty_like(rb)
+        ops.heaviside(rb, beta, eta, rho=r, drho=d)
+        ctx.save_for_backward(d)
+        return r
+
+    @staticmethod
+    def backward(ctx, g):
+        (d,) = ctx.saved_tensors
+        return g * d, None, None
+
+
+def design_compliance(x, problem, filt, beta=0.0, eta=0.5):
+    """c(x) = f . u for the design variable x through filter, projection and the SIMP modulus:
+    differentiable end to end (the adjoint kernel, the projection derivative and the filter
+    transpose in backward)."""
+    return Compliance.apply(ProjectFn.apply(FilterFn.apply(x, filt), float(beta), float(eta)), problem)

@@ -15,12 +15,20 @@ Restates:
                     (experiments/session_i/make_inputs.py:31-50)
   cantilever        the synthetic voxel cantilever of the benchmark configurations (seeded
                     iid densities, x=0 clamp, unit downward line load on the x=nx, z=nz/2 edge)
+  project/dproject, schedule_at, oc_update, run_simp
+                    the donor's projected driver (simp_r2.py:388-433, 467-549, 552-918) with a
+                    direct sparse state solve
 
 Parity status: cone filter pinned by brute force against an explicit sparse H built
 independently (DensityFilter of run_w1b.py:97-126) and by <Hx, y> == <x, Hy>; OC pinned by
 its invariants (bounds, move limit, achieved volume within tolerance) and by a closed-form
 case (uniform sensitivities); sensitivity pinned by central finite differences of the
-compliance; regenerate pinned by the smoke-q4 / optimized-q64 fixtures' mask and force.
+compliance; regenerate pinned by the smoke-q4 / optimized-q64 fixtures' mask and force;
+projection pinned by its closed values (0 -> 0, eta -> eta, 1 -> 1), monotonicity, the
+beta -> 0 limit and central differences of dproject; the projected OC by its invariants
+(box, move limit, achieved physical volume at the bisection's end) and by the closed form of
+uniform sensitivities (the multiplier that keeps a uniform design); run_simp by the volume
+constraint at every step and by the compliance decrease over its first phase.
 """
 from __future__ import annotations
 
@@ -107,6 +115,120 @@ def modulus(rho, penal=3.0, E0=1.0, Emin=1e-9):
 
 def sensitivity(rho, ce, penal=3.0, E0=1.0, Emin=1e-9):
     return -penal * (E0 - Emin) * np.asarray(rho) ** (penal - 1) * ce
+
+
+# ------------------------------------------------------------------ projected SIMP driver
+# simp_r2.py:388-433 (projection, continuation), :467-549 (OC on the physical volume),
+# :552-918 (run_simp), restated in NumPy with a direct sparse state solve.
+DEFAULT_SCHEDULE = [(15, 1.5, 1.0, 0.20, 1.50), (40, 3.5, 4.0, 0.15, 1.35),
+                    (65, 4.5, 16.0, 0.08, 1.25), (120, 4.5, 32.0, 0.05, 1.20)]
+
+
+def schedule_at(it, schedule=None):
+    sch = schedule or DEFAULT_SCHEDULE
+    for until, p, beta, move, rmin in sch:
+        if it <= until:
+            return p, beta, move, rmin, until == sch[-1][0]
+    return (*sch[-1][1:], True)
+
+
+def project(rho_bar, beta, eta=0.5):
+    if beta <= 1e-12:
+        return np.array(rho_bar, dtype=np.float64, copy=True)
+    tb = math.tanh(beta * eta)
+    den = tb + math.tanh(beta * (1.0 - eta))
+    return (tb + np.tanh(beta * (np.asarray(rho_bar) - eta))) / den
+
+
+def dproject(rho_bar, beta, eta=0.5):
+    if beta <= 1e-12:
+        return np.ones_like(np.asarray(rho_bar, dtype=np.float64))
+    tb = math.tanh(beta * eta)
+    den = tb + math.tanh(beta * (1.0 - eta))
+    ch = np.cosh(beta * (np.asarray(rho_bar) - eta))
+    return beta / (den * ch * ch)
+
+
+def oc_update(rho, dc, dv, volfrac, move, filt, beta, eta, rho_lb, bisect_tol=1e-8, max_bisect=80,
+              lam_hint=None):
+    dcn = np.minimum(dc, -1e-30)
+    dvp = np.maximum(dv, 1e-30)
+
+    def design(lmid):
+        step = np.sqrt(-dcn / (lmid * dvp))
+        rn = np.clip(np.clip(rho * step, rho - move, rho + move), rho_lb, 1.0)
+        return rn, float(project(filt.forward(rn), beta, eta).mean())
+
+    probes = 0
+    if lam_hint is not None and lam_hint > 0:
+        l1, l2 = lam_hint / 4.0, lam_hint * 4.0
+        for _ in range(30):
+            _, v1 = design(l1)
+            _, v2 = design(l2)
+            probes += 2
+            if v1 > volfrac and v2 <= volfrac:
+                break
+            if v1 <= volfrac:
+                l1 /= 8.0
+            if v2 > volfrac:
+                l2 *= 8.0
+            if l1 < 1e-30 or l2 > 1e30:
+                l1, l2 = 0.0, 1e9
+                break
+    else:
+        l1, l2 = 0.0, 1e9
+    lam = 0.5 * (l1 + l2)
+    rho_new, achieved = design(lam)
+    probes += 1
+    for _ in range(max_bisect):
+        if (l2 - l1) <= bisect_tol * (l1 + l2 + 1e-30):
+            break
+        lam = 0.5 * (l1 + l2)
+        rho_new, achieved = design(lam)
+        probes += 1
+        if achieved > volfrac:
+            l1 = lam
+        else:
+            l2 = lam
+    return rho_new, achieved, lam, probes
+
+
+def run_simp(shape, ke, mask, force, steps, volfrac=0.30, rmin=1.5, penal=None, E0=1.0, Emin=1e-9,
+             rho_lb=1e-3, eta=0.5, schedule=None):
+    """`steps` outer iterations of the projected SIMP loop with exact state solves; returns the
+    per-step (compliance, physical volume) history and the final design variable."""
+    from scipy.sparse.linalg import spsolve
+    from .hexref import edof_author
+    edof = edof_author(shape)
+    free = np.flatnonzero(np.asarray(mask) != 0)
+    n = int(np.prod(shape))
+    rho = np.full(n, float(volfrac))
+    hist = []
+    lam_prev = None
+    for it in range(1, steps + 1):
+        p_it, beta, move, r_it, _ = schedule_at(it, schedule)
+        if penal is not None:
+            p_it = penal
+        filt = ConeFilter(shape, r_it)
+        rho_bar = filt.forward(rho)
+        rp = project(rho_bar, beta, eta)
+        E = modulus(rp, p_it, E0, Emin)
+        nd = 3 * (shape[0] + 1) * (shape[1] + 1) * (shape[2] + 1)
+        K = sp.coo_matrix(((E[:, None, None] * ke[None]).ravel(),
+                           (np.repeat(edof, 24, axis=1).ravel(), np.tile(edof, (1, 24)).ravel())),
+                          shape=(nd, nd)).tocsr()
+        u = np.zeros(nd)
+        u[free] = spsolve(K[free][:, free].tocsc(), np.asarray(force)[free])
+        c = float(np.asarray(force) @ u)
+        ue = u[edof]
+        ce = np.einsum("ei,ij,ej->e", ue, ke, ue)
+        dc_drp = sensitivity(rp, ce, p_it, E0, Emin)
+        dprj = dproject(rho_bar, beta, eta)
+        dc = filt.transpose(dc_drp * dprj)
+        dv = filt.transpose(dprj / n)
+        rho, vol, lam_prev, _ = oc_update(rho, dc, dv, volfrac, move, filt, beta, eta, rho_lb, lam_hint=lam_prev)
+        hist.append((c, vol))
+    return hist, rho
 
 
 # ------------------------------------------------------------------------- problems

@@ -20,6 +20,8 @@ from oracle import solvers as osv
 
 pytestmark = pytest.mark.gpu
 
+KE = hx.quadrature([1, 1, 1], 0.3)
+
 
 def rel(a, b):
     a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a
@@ -177,3 +179,92 @@ def test_mixed_precision_state_solve_gives_the_fp64_sensitivities():
         out.append((float(c), r.grad.clone()))
     assert out[1][0] == pytest.approx(out[0][0], rel=1e-7)
     assert float((out[1][1] - out[0][1]).norm() / out[0][1].norm()) < 1e-6
+
+
+# ------------------------------------------------------------------ projected SIMP driver
+@pytest.mark.parametrize("beta", [0.0, 1.0, 16.0])
+def test_heaviside_kernel_matches_the_oracle(beta):
+    from voxelhip import ops
+    rng = np.random.default_rng(41)
+    x = rng.uniform(0.0, 1.0, 5000)
+    xd = torch.from_numpy(x).cuda()
+    r, d, E = (torch.empty_like(xd) for _ in range(3))
+    partial = torch.zeros(20, dtype=torch.float64, device="cuda")
+    ops.heaviside(xd, beta, 0.5, 3.5, 1.0, 1e-9, rho=r, drho=d, E=E, partial=partial)
+    rp = sr.project(x, beta)
+    assert np.abs(r.cpu().numpy() - rp).max() < 1e-14
+    assert rel(d, sr.dproject(x, beta)) < 1e-13
+    assert rel(E, sr.modulus(rp, 3.5)) < 1e-13
+    assert float(partial.sum()) == pytest.approx(rp.sum(), rel=1e-13)
+
+
+def test_projected_oc_update_matches_the_oracle():
+    from voxelhip import simp
+    shape = (10, 6, 6)
+    rng = np.random.default_rng(7)
+    n = int(np.prod(shape))
+    rho = rng.uniform(0.05, 0.9, n)
+    dc, dv = -rng.uniform(0.1, 2.0, n), rng.uniform(0.5, 1.5, n) / n
+    filt, ofilt = simp.ConeFilter(shape, 1.35), sr.ConeFilter(shape, 1.35)
+    for beta, hint in ((1.0, None), (8.0, 3.0)):
+        print("case beta", beta, flush=True)
+        want, vol_o, lam_o, _ = sr.oc_update(rho, dc, dv, 0.4, 0.1, ofilt, beta, 0.5, 1e-3, lam_hint=hint)
+        got, vol, lam, _ = simp.oc_update(*(torch.from_numpy(a).cuda() for a in (rho, dc, dv)), 0.4, 0.1, filt, beta,
+                                          0.5, 1e-3, lam_hint=hint)
+        assert lam == pytest.approx(lam_o, rel=1e-7) and vol == pytest.approx(vol_o, abs=1e-9)
+        assert np.abs(got.cpu().numpy() - want).max() < 1e-7
+
+
+def test_warm_started_engine_solve():
+    from voxelhip.engine import GMGSolver
+    shape = (32, 16, 16)
+    _, E, mask, force = sr.cantilever(shape, seed=3, rmin=1.5)
+    f = torch.from_numpy(force).cuda()
+    s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, smoother="chebyshev", degree=3,
+                  precision="mixed", graph=True)
+    x, it, ok, r, _ = s.solve(f, tol=1e-10)
+    cold = x.clone()
+    assert ok
+    x, it0, ok, r0, mv = s.solve(f, tol=1e-8, x0=cold)           # already converged: no iteration
+    assert ok and it0 == 0 and mv == 1 and r0 <= 1e-8 and torch.equal(x, cold)
+    x, itw, ok, rw, _ = s.solve(f, tol=1e-10, x0=0.9 * cold)     # a close start needs fewer iterations
+    assert ok and rw <= 1e-10 and itw < it
+    assert float((x - cold).norm() / cold.norm()) < 1e-8
+
+
+def test_projected_simp_driver_matches_the_oracle():
+    from voxelhip import simp
+    shape = (8, 4, 4)
+    _, _, mask, force = sr.cantilever(shape, seed=1)
+    pb = simp.SimpProblem(shape, KE, mask, force, precision="fp64", max_coarse_x=2, maxiter=300)
+    res = simp.run_simp(pb, fixed_budget=8, cg_tol=1e-11)
+    hist_o, rho_o = sr.run_simp(shape, KE, mask, force, steps=8)
+    c = [h["compliance"] for h in res["history"]]
+    assert np.allclose(c, [h[0] for h in hist_o], rtol=1e-7)
+    assert all(abs(h["vol_phys"] - 0.3) < 1e-6 for h in res["history"])
+    assert np.abs(res["rho"].cpu().numpy() - rho_o).max() < 1e-6
+    assert res["stop_reason"] == "fixed_budget" and res["warm_start_retries"] == 0
+    # warm starts: later solves take fewer iterations than the first
+    its = [s_["cg_iters"] for s_ in res["solve_log"][:-1]]
+    assert min(its[3:]) < its[0]
+
+
+def test_design_gradient_through_filter_and_projection():
+    from voxelhip import simp
+    shape = (10, 5, 5)
+    _, _, mask, force = sr.cantilever(shape, seed=2)
+    pb = simp.SimpProblem(shape, KE, mask, force, precision="fp64", tol=1e-12, max_coarse_x=4, maxiter=400,
+                          penal=3.0)
+    filt = simp.ConeFilter(shape, 1.5)
+    rng = np.random.default_rng(5)
+    x = torch.from_numpy(rng.uniform(0.2, 0.8, int(np.prod(shape)))).cuda().requires_grad_(True)
+    c = simp.design_compliance(x, pb, filt, beta=4.0)
+    c.backward()
+    g = x.grad.clone()
+    for k in range(3):
+        d = torch.from_numpy(rng.standard_normal(x.numel())).cuda()
+        h = 1e-5
+        cp = float(simp.design_compliance((x + h * d).detach(), pb, filt, beta=4.0))
+        cm = float(simp.design_compliance((x - h * d).detach(), pb, filt, beta=4.0))
+        fd = (cp - cm) / (2 * h)
+        assert float(g @ d) == pytest.approx(fd, rel=1e-5)

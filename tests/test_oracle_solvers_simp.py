@@ -143,3 +143,56 @@ def test_cantilever_load_is_unit_and_clamp_is_x0():
     m = mask.reshape(9, 5, 5, 3)
     assert np.all(m[0] == 0) and np.all(m[1:] == 1)
     assert np.all((rho >= 0.01) & (rho <= 1.0))
+
+
+# ------------------------------------------------------------------ projected SIMP driver (oracle)
+@pytest.mark.parametrize("beta", [1.0, 4.0, 32.0])
+def test_projection_closed_values_monotonicity_and_derivative(beta):
+    x = np.linspace(0.0, 1.0, 101)
+    y = sr.project(x, beta)
+    assert abs(y[0]) < 1e-15 and abs(y[-1] - 1.0) < 1e-15 and abs(sr.project(np.array([0.5]), beta)[0] - 0.5) < 1e-15
+    assert np.all(np.diff(y) > 0)
+    h = 1e-6
+    xm = np.linspace(0.05, 0.95, 19)
+    fd = (sr.project(xm + h, beta) - sr.project(xm - h, beta)) / (2 * h)
+    assert np.allclose(sr.dproject(xm, beta), fd, rtol=1e-6, atol=1e-9)
+    # the beta -> 0 limit is the identity
+    assert np.abs(sr.project(x, 1e-6) - x).max() < 1e-12 and np.array_equal(sr.project(x, 0.0), x)
+
+
+def test_continuation_schedule_phases():
+    assert sr.schedule_at(1) == (1.5, 1.0, 0.20, 1.50, False)
+    assert sr.schedule_at(15) == (1.5, 1.0, 0.20, 1.50, False)
+    assert sr.schedule_at(16) == (3.5, 4.0, 0.15, 1.35, False)
+    assert sr.schedule_at(120) == (4.5, 32.0, 0.05, 1.20, True)
+    assert sr.schedule_at(400) == (4.5, 32.0, 0.05, 1.20, True)
+
+
+def test_projected_oc_invariants_and_uniform_case():
+    shape = (10, 6, 6)
+    rng = np.random.default_rng(7)
+    n = int(np.prod(shape))
+    filt = sr.ConeFilter(shape, 1.5)
+    rho = rng.uniform(0.05, 0.9, n)
+    dc, dv = -rng.uniform(0.1, 2.0, n), rng.uniform(0.5, 1.5, n) / n
+    for beta, lam_hint in ((1.0, None), (8.0, 3.0)):
+        new, vol, lam, _ = sr.oc_update(rho, dc, dv, 0.4, 0.1, filt, beta, 0.5, 1e-3, lam_hint=lam_hint)
+        assert new.min() >= 1e-3 and new.max() <= 1.0
+        assert np.all(np.abs(new - rho) <= 0.1 + 1e-15)
+        assert abs(vol - 0.4) < 1e-6
+        assert abs(vol - sr.project(filt.forward(new), beta).mean()) < 1e-15
+    # uniform design and sensitivities: the volume-preserving multiplier keeps the design
+    u = np.full(n, 0.4)
+    new, vol, lam, _ = sr.oc_update(u, np.full(n, -2.0), np.full(n, 1.0 / n), 0.4, 0.2, filt, 0.0, 0.5, 1e-3)
+    assert abs(lam - 2.0 * n) / (2.0 * n) < 1e-6 and np.abs(new - 0.4).max() < 1e-6
+
+
+def test_projected_simp_oracle_keeps_the_volume_and_lowers_the_compliance():
+    shape = (8, 4, 4)
+    ke = hx.quadrature([1, 1, 1], 0.3)
+    _, _, mask, force = sr.cantilever(shape, seed=1)
+    hist, rho = sr.run_simp(shape, ke, mask, force, steps=8)
+    c = [h[0] for h in hist]
+    assert all(abs(v - 0.3) < 1e-6 for _, v in hist)
+    assert all(b < a for a, b in zip(c, c[1:]))
+    assert rho.min() >= 1e-3 and rho.max() <= 1.0
