@@ -86,7 +86,8 @@ typedef struct vh_epilogue {
   double* partial;
   double omega, cheb_a, cheb_c;
   int lx;   /* x planes per workgroup chunk, 0 = automatic */
-  int tile; /* (y, z) tile variant: 0 = automatic, 1 = 16 x 32 nodes, 2 = 8 x 32 nodes */
+  int tile; /* (y, z) tile variant: 0 = automatic, 1 = 16 x 32 nodes, 2 = 8 x 32 nodes, 3 = 4 x 8 nodes (one
+             * wave), 4 = wide (one workgroup per CU: 22 x 32 nodes in FP64, 30 x 32 in FP32) */
   double kdiag[3];
   int flags; /* VH_PREV_ZERO: CHEB3's previous iterate is zero (y is then not read) */
   /* Load-time combination (voxel-cube blocks), the combined input written to u_out at every

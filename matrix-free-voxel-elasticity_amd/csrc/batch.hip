@@ -249,6 +249,7 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
     p.lx = pl.lx;
     p.tiles_y = pl.tiles_y;
     p.tiles_z = pl.tiles_z;
+    p.tile_ty = pl.tile_ty;
     p.chunks_x = pl.chunks_x;
     p.n_tiles = pl.n_tiles;
     prefix[i + 1] = prefix[i] + pl.n_tiles;

@@ -18,9 +18,12 @@ struct Tile {
 constexpr int MV_PREV_ZERO = 1;
 // load-time operations on the input (template parameter LOADOP of the product kernel)
 constexpr int LOAD_NONE = 0, LOAD_AXPY = 1, LOAD_MUL = 2;
-// tile variants: 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads), 3 = 4 x 8 (64 threads: one
-// wave per workgroup, for the latency-bound coarse levels)
-constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2, MV_TILE_4x8 = 3;
+// tile variants (owned nodes in y x z; the y extent is an upper bound, the plan spreads ny evenly
+// over the y tiles): 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads), 3 = 4 x 8
+// (64 threads: one wave per workgroup, for the latency-bound coarse levels), 4 = the wide tile that
+// fills one CU with a single workgroup spread evenly over its four SIMDs: 22 x 32 in FP64 (768
+// threads, three waves per SIMD), 30 x 32 in FP32 (1024 threads, four waves per SIMD)
+constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2, MV_TILE_4x8 = 3, MV_TILE_WIDE = 4;
 
 // Epilogue modes (A = constrained operator M K M + I - M when a mask is given, else K).
 enum Mode : int {
@@ -62,6 +65,7 @@ struct MatvecParams {
   int flags;              // MV_PREV_ZERO: CHEB3's previous iterate is zero (y not read)
   int fixed_zero;         // with a mask: the output is 0 at fixed DOFs (M K M u, (b - A u) M)
   int tiles_y, tiles_z, chunks_x, n_tiles;
+  int tile_ty;            // owned node rows per y tile (at most the tile variant's TY)
   T omega, cheb_a, cheb_c, load_scale;
   T kdi[3];               // 1 / (diagonal of ke per component, the same at every corner of a brick):
                           // with dinv == nullptr, D^-1 = kdi[c] / (sum of the incident moduli)
@@ -71,7 +75,7 @@ struct MatvecParams {
 // Launch plan of the product for a grid: tile variant, x chunking and grid size, chosen from
 // the CU count and the resident workgroups per CU of the kernel (hint values > 0 override).
 struct MatvecPlan {
-  int tile, lx, tiles_y, tiles_z, chunks_x, n_tiles, grid;
+  int tile, lx, tiles_y, tiles_z, chunks_x, n_tiles, grid, tile_ty;
   double cost;
 };
 MatvecPlan matvec_plan_f64(bool cube, int nx, int ny, int nz, int lx_hint, int tile_hint);

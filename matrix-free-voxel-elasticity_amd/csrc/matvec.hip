@@ -33,6 +33,8 @@
 // This file is compiled twice, through matvec_f64.hip (VH_MATVEC_F64: the FP64 instantiations)
 // and matvec_f32.hip (VH_MATVEC_F32: FP32 with FP32 or BF16 storage), so the two halves build
 // in parallel.
+#include <type_traits>
+
 #include "common.h"
 #include "matvec.h"
 
@@ -133,7 +135,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ny1 = ny + 1, nz1 = nz + 1;
-  const int oy0 = ty * TL::TY, oy1 = (ty == a.tiles_y - 1) ? ny1 : oy0 + TL::TY;
+  const int tye = a.tile_ty;                       // owned node rows of this launch's y tiles
+  const int oy0 = ty * tye, oy1 = (ty == a.tiles_y - 1) ? ny1 : oy0 + tye;
   const int oz0 = tz * TL::TZ, oz1 = (tz == a.tiles_z - 1) ? nz1 : oz0 + TL::TZ;
   const int ox0 = cx * a.lx, ox1 = (cx == a.chunks_x - 1) ? nx + 1 : ox0 + a.lx;
   const int ybase = oy0 - 1, zbase = oz0 - 1;      // local node j <-> global base + j
@@ -141,7 +144,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   const int tid = threadIdx.x;
   const int ly = tid / EZ, lz = tid - (tid / EZ) * EZ;
   const int ey = ybase + ly, ez = zbase + lz;
-  const bool col_in_tile = tid < TL::NCOL;
+  const bool col_in_tile = tid < TL::NCOL && ly <= tye;
   const bool col_ok = col_in_tile && ey >= 0 && ey < ny && ez >= 0 && ez < nz;
   const int iy_own = oy0 + ly, iz_own = oz0 + lz;  // node (ly+1, lz+1) in local coordinates
   const bool node_own = col_in_tile && iy_own < oy1 && iz_own < oz1;
@@ -171,7 +174,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     const int col = k - row * ROWLEN;
     const int iy = ybase + row;
     const int iz = zbase + col / 3;
-    const bool ok = k < NLOAD && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
+    const bool ok = k < (tye + 2) * ROWLEN && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
     ld_okbits |= unsigned(ok) << j;
     ld_off[j] = ok ? 3 * (iy * nz1 + zbase) + col : 0;
   }
@@ -470,6 +473,10 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecPar
 using Tile16 = Tile<16, 32>;
 using Tile8 = Tile<8, 32>;
 using Tile4 = Tile<4, 8>;
+// one workgroup per CU, its waves spread evenly over the four SIMDs: FP64 (<= 168 VGPRs) takes
+// three waves per SIMD (23 x 33 = 759 element columns), FP32 (<= 128 VGPRs) four (31 x 33 = 1023)
+template <typename T>
+using TileWide = std::conditional_t<sizeof(T) == 8, Tile<22, 32>, Tile<30, 32>>;
 
 static int device_cus() {
   static int cus = 0;
@@ -523,6 +530,9 @@ template <typename TL>
 static void plan_tile(int nx, int ny, int nz, int lx_hint, int occ, MatvecPlan* pl) {
   const int ty = (ny + TL::TY - 1) / TL::TY;
   const int tz = (nz + TL::TZ - 1) / TL::TZ;
+  // the ny element rows spread evenly over the y tiles: each owns tile_ty <= TY node rows (the
+  // last one tile_ty + 1 at most, the extra node row at y = ny)
+  pl->tile_ty = (ny + ty - 1) / ty;
   int lx = 0, cx = 0;
   plan_chunks(nx, ty * tz, device_cus() * occ, lx_hint, &lx, &cx);
   pl->tiles_y = ty;
@@ -531,10 +541,13 @@ static void plan_tile(int nx, int ny, int nz, int lx_hint, int occ, MatvecPlan* 
   pl->lx = lx;
   pl->n_tiles = ty * tz * cx;
   pl->grid = ((pl->n_tiles + 7) / 8) * 8;
-  // relative cost: rounds x steps x element columns x resident workgroups
+  // relative cost: rounds x plane steps x waves on the busiest SIMD of a CU.  The workgroups of a
+  // CU step in lockstep through their barriers, so a step lasts as long as the SIMD holding the
+  // most waves takes to issue them (9 waves: 3 + 2 + 2 + 2, the SIMD with 3 sets the pace).
   const long slots = long(device_cus()) * occ;
   const long rounds = (pl->n_tiles + slots - 1) / slots;
-  pl->cost = double(rounds) * (lx + 1) * TL::NCOL * occ;
+  const int waves = occ * (TL::BLOCK / 64);
+  pl->cost = double(rounds) * (lx + 1) * ((waves + 3) / 4);
 }
 
 template <typename T, typename S, bool CUBE>
@@ -552,7 +565,12 @@ static MatvecPlan plan_typed(int nx, int ny, int nz, int lx_hint, int tile_hint)
     c.tile = MV_TILE_4x8;
     return c;
   }
-  return (b.cost < a.cost) ? b : a;
+  MatvecPlan c{};
+  plan_tile<TileWide<T>>(nx, ny, nz, lx_hint, resident_per_cu<T, S, TileWide<T>, CUBE>(), &c);
+  c.tile = MV_TILE_WIDE;
+  if (tile_hint == MV_TILE_WIDE) return c;
+  MatvecPlan best = (b.cost < a.cost) ? b : a;
+  return c.cost < best.cost ? c : best;
 }
 
 #if defined(VH_MATVEC_F64)
@@ -576,6 +594,9 @@ static void launch_tiles(const MatvecParams<T>& p, int grid, hipStream_t st) {
     hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile8, CUBE, LOP>), dim3(grid), dim3(Tile8::BLOCK), 0, st, p);
   else if (p.tile == MV_TILE_4x8)
     hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile4, CUBE, LOP>), dim3(grid), dim3(Tile4::BLOCK), 0, st, p);
+  else if (p.tile == MV_TILE_WIDE)
+    hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, TileWide<T>, CUBE, LOP>), dim3(grid), dim3(TileWide<T>::BLOCK), 0,
+                       st, p);
   else
     hipLaunchKernelGGL((parity_xmarch<T, S, MODE, MASK, Tile16, CUBE, LOP>), dim3(grid), dim3(Tile16::BLOCK), 0, st, p);
 }
@@ -630,6 +651,7 @@ int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipSt
   p.lx = pl.lx;
   p.tiles_y = pl.tiles_y;
   p.tiles_z = pl.tiles_z;
+  p.tile_ty = pl.tile_ty;
   p.chunks_x = pl.chunks_x;
   p.n_tiles = pl.n_tiles;
   if (storage == BF16) {

@@ -1,13 +1,13 @@
 """Empirical launch tuning of the parity product.
 
-The product kernel takes two launch parameters: the node tile of a workgroup (16 x 32 or
-8 x 32 nodes in y, z) and the x-chunk length lx (0 lets the occupancy model choose).  Their
-best values depend on the grid, the precision and the fused pass (a smoothing pass streams two
-more vectors than a plain product), and the occupancy model does not rank them reliably.
-`LaunchTuner` times every candidate with HIP events on the operands of the pass being tuned and
-keeps the fastest; results are cached per process and, when a path is given (or the
-VOXELHIP_TUNE_CACHE environment variable names one), in a JSON file keyed by device name, grid,
-precision, mask kind and pass.
+The product kernel takes two launch parameters: the node tile of a workgroup (16 x 32, 8 x 32,
+4 x 8, or the wide tile of one workgroup per CU: 22 x 32 nodes in FP64, 30 x 32 in FP32, in y, z)
+and the x-chunk length lx (0 lets the occupancy model choose).  Their best values depend on the
+grid, the precision and the fused pass (a smoothing pass streams two more vectors than a plain
+product), and the occupancy model does not rank them reliably.  `LaunchTuner` times every
+candidate with HIP events on the operands of the pass being tuned and keeps the fastest; results
+are cached per process and, when a path is given (or the VOXELHIP_TUNE_CACHE environment variable
+names one), in a JSON file keyed by device name, grid, precision, mask kind and pass.
 
 Tuning is a set-up step: it launches each candidate a few times on scratch vectors and
 leaves the level's own vectors untouched.
@@ -23,7 +23,7 @@ import torch
 from . import _lib as L
 from . import ops
 
-TILES = (L.TILE_16x32, L.TILE_8x32)
+TILES = (L.TILE_16x32, L.TILE_8x32, L.TILE_WIDE)
 LX_CANDIDATES = (0, 1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
 # one-wave 4 x 8 tiles, for grids small enough to be latency-bound (elements)
 SMALL_GRID = 64 * 32 * 32

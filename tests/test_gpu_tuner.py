@@ -14,19 +14,19 @@ pytestmark = pytest.mark.gpu
 KE = 2 * hx.quadrature([1, 1, 1], 0.3)
 
 
-@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_every_launch_plan_gives_the_same_product(dtype):
+@pytest.mark.parametrize("dtype,shape", [(torch.float64, (24, 20, 36)), (torch.float32, (24, 20, 36)),
+                                         (torch.float64, (10, 35, 101)), (torch.float32, (6, 67, 40))])
+def test_every_launch_plan_gives_the_same_product(dtype, shape):
     import voxel_kernels as vk
     from voxelhip import ops
     from voxelhip.tuner import _candidates
-    shape = (24, 20, 36)
     g = torch.Generator(device="cuda").manual_seed(3)
     E = torch.rand(int(np.prod(shape)), generator=g, device="cuda", dtype=torch.float64).to(dtype) + 0.01
     u = torch.randn(3 * ops.n_nodes(shape), generator=g, device="cuda", dtype=torch.float64).to(dtype)
     blocks = vk.Modal(KE).blocks
     ref = ops.matvec(blocks, E, u, torch.empty_like(u), shape)
     for t, lx in _candidates(shape[0], int(np.prod(shape))):
-        print("case", dtype, t, lx, flush=True)
+        print("case", dtype, shape, t, lx, flush=True)
         y = ops.matvec(blocks, E, u, torch.empty_like(u), shape, tile=t, lx=lx)
         assert torch.equal(y, ref), (t, lx)
 
