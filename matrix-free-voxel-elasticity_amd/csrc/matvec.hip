@@ -20,10 +20,11 @@
 //    + 12 FP64 operations (forward, blocks, modulus, inverse, carry) instead of 240 with an
 //    8-lane shuffle butterfly.
 //  * Node planes are staged through LDS in their HBM order (rows of (x, y, z) triples: lane k
-//    loads and stores flat element k, so both the loads and the LDS stores are unit-stride and
-//    free of bank conflicts; the face reads then stride by 3 words, which is conflict-free too)
-//    with a one-plane register prefetch; the four corner contributions of each node are
-//    gathered from LDS in a fixed order, so results are bitwise reproducible.
+//    loads and stores flat element k, so both the loads and the LDS stores are unit-stride)
+//    with a one-plane register prefetch.  Each element column stores its four corner
+//    contributions in LDS slots numbered by its thread, and the owner of a node gathers them
+//    from its four incident columns in a fixed order (unit-stride writes and reads across a
+//    wave; results are bitwise reproducible).
 //  * The epilogue fuses the constrained operator (M K M + I - M), the residual, a damped
 //    Jacobi sweep, a Chebyshev step or a p.Ap partial sum into the single pass over y.
 //  * Workgroup ids are remapped so that each XCD (8 per MI355X, own L2) receives a contiguous
@@ -119,8 +120,11 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
   constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
   __shared__ T su[2][3 * PLANE];      // staged node planes ix & 1 (input), AoS rows as in HBM
-  __shared__ T sc[4][3][PLANE];       // corner contributions, indexed by node
-  __shared__ T se[PLANE];             // per element column: E(ex-1) + E(ex), for D^-1 on the fly
+  // corner contributions and column modulus sums are indexed by the writing thread (unit stride
+  // for the writes and for the owner's four reads), plus one zero pad slot past every writer
+  constexpr int SCN = BLOCK + EZ + 2;
+  __shared__ T sc[4][3][SCN];         // corner contributions of each element column
+  __shared__ T se[SCN];               // per element column: E(ex-1) + E(ex), for D^-1 on the fly
   __shared__ T sE[BLOCK];             // this thread's prefetched modulus, parked across the barrier
   __shared__ double red[BLOCK / 64];
   constexpr bool NEEDS_DINV = MODE == MODE_JACOBI || MODE == MODE_CHEB || MODE == MODE_CHEB3 || MODE == MODE_CHEB3D;
@@ -152,9 +156,9 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // zero the contribution slots once: slots without a writer column must read as zero
   {
     T* flat = &sc[0][0][0];
-    for (int i = tid; i < 12 * PLANE; i += BLOCK) flat[i] = T(0);
+    for (int i = tid; i < 12 * SCN; i += BLOCK) flat[i] = T(0);
     if (dinv_from_E)
-      for (int i = tid; i < PLANE; i += BLOCK) se[i] = T(0);
+      for (int i = tid; i < SCN; i += BLOCK) se[i] = T(0);
   }
 
   const S* __restrict__ u = reinterpret_cast<const S*>(a.u);
@@ -275,6 +279,10 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 #pragma unroll
   for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = ep_z[c] = T(0);
   const int jn_own = (ly + 1) * NZ + (lz + 1);
+  // the element columns (ly+1, lz+1), (ly+1, lz), (ly, lz+1), (ly, lz) around the owned node, as
+  // thread numbers; past the last column of a row (lz + 1 == EZ) there is none: the zero pad
+  const bool zlast = lz + 1 >= EZ;
+  const int w0 = zlast ? SCN - 1 : tid + EZ + 1, w1 = tid + EZ, w2 = zlast ? SCN - 1 : tid + 1, w3 = tid;
   auto fetch_epilogue = [&](int ix) {
     const bool use = node_own && ix >= ox0 && ix < ox1;
     const size_t node = use ? (size_t(ix) * ny1 + iy_own) * nz1 + iz_own : 0;
@@ -345,13 +353,12 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         const T g0 = Gc[0][c] + Gl[0][c], g1 = Gc[1][c] + Gl[1][c];
         const T g2 = Gc[2][c] + Gl[2][c], g3 = Gc[3][c] + Gl[3][c];
         const T A0 = g0 + g1, A1 = g0 - g1, B0 = g2 + g3, B1 = g2 - g3;
-        const int i00 = ly * NZ + lz;
-        sc[0][c][i00] = A0 + B0;                       // corner (b=0, c=0)
-        sc[1][c][i00 + 1] = A0 - B0;                   // corner (0, 1)
-        sc[2][c][i00 + NZ] = A1 + B1;                  // corner (1, 0)
-        sc[3][c][i00 + NZ + 1] = A1 - B1;              // corner (1, 1)
+        sc[0][c][tid] = A0 + B0;                       // corner (b=0, c=0): node (ly, lz)
+        sc[1][c][tid] = A0 - B0;                       // corner (0, 1): node (ly, lz+1)
+        sc[2][c][tid] = A1 + B1;                       // corner (1, 0): node (ly+1, lz)
+        sc[3][c][tid] = A1 - B1;                       // corner (1, 1): node (ly+1, lz+1)
       }
-      if (dinv_from_E) se[ly * NZ + lz] = Eprev + Ee;
+      if (dinv_from_E) se[tid] = Eprev + Ee;
     }
     // unmasked input at the owned node of plane ex, read before the barrier that lets the next
     // step overwrite this plane buffer
@@ -361,16 +368,18 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     }
     __syncthreads();
     if (finalize && node_own) {
-      const int jn = jn_own;
       const size_t node = (size_t(ex) * ny1 + iy_own) * nz1 + iz_own;
-      T part[3][4];
+      T part[3][4];      // corner k of the node comes from column w_k
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) part[c][k] = sc[k][c][jn];
+      for (int c = 0; c < 3; ++c) {
+        part[c][0] = sc[0][c][w0];
+        part[c][1] = sc[1][c][w1];
+        part[c][2] = sc[2][c][w2];
+        part[c][3] = sc[3][c][w3];
+      }
       T rsum = T(0);     // 1 / (sum of the moduli of the node's incident elements)
-      if (dinv_from_E) { // incident columns (jy-1|jy, jz-1|jz) of node (jy, jz) = (ly+1, lz+1)
-        const T esum = ((se[jn - NZ - 1] + se[jn - 1]) + se[jn - NZ]) + se[jn];
+      if (dinv_from_E) { // incident columns (ly|ly+1, lz|lz+1) of the owned node (ly+1, lz+1)
+        const T esum = ((se[w3] + se[w1]) + se[w2]) + se[w0];
         rsum = reciprocal(esum);
       }
 #pragma unroll
