@@ -170,6 +170,26 @@ int vh_assemble_dense(const double* ke, const double* E, int nx, int ny, int nz,
 int vh_galerkin_scatter(const double* G, int nx, int ny, int nz, int mask_kind, const uint8_t* mask, double* A, int ld,
                         void* stream);
 
+/* Galerkin coarse operator of a non-coarsest level as a symmetric 27-point node-block stencil
+ * (dtype F64 or F32, device arrays; (nx, ny, nz) the COARSE element grid, nn its node count).
+ * G is the 81 x 3nn probe matrix of vh_galerkin_scatter (FP64: R A_f P v_p, A_f the finer
+ * level's constrained operator, R and P unmasked).  U (14 x 9 x nn) receives the blocks
+ * K[I, I + d] of the upper half of the offsets d in {-1,0,1}^3 (offset index
+ * k = ((dx+1)*3+(dy+1))*3+(dz+1) >= 13, slot k - 13), coefficient-major:
+ * U[((slot*3 + r)*3 + c)*nn + I] = K[3I + r, 3(I + d) + c], 0 where I + d is outside the grid.
+ * K is the unmasked P^T A_f P; the coarse mask is applied by vh_stencil_apply.
+ * Replaces: the Galerkin coarse operators of donor_yang_gmg multigrid_v4.py:132-270 (level 1,
+ * CSR from element triple products) and :747-772 (deeper levels, sparse R K P). */
+int vh_stencil_scatter(int dtype, const double* G, void* U, int nx, int ny, int nz, void* stream);
+/* y = M K M u + (I - M) u with the stencil K of vh_stencil_scatter (one thread per node), and
+ * the epilogues of vh_matvec_fused's RESID (with y32), JACOBI and CHEB3 (prev, VH_PREV_ZERO)
+ * passes (fixed_zero as there).  The smoothing passes take dinv (kdiag is not used); ep's
+ * d, partial, w, zold and u_out must be NULL.
+ * Replaces: the CSR SpMV of the coarse Chebyshev smoothing and residuals of donor_yang_gmg
+ * multigrid_v4.py:934-1008. */
+int vh_stencil_apply(int dtype, const void* U, const void* u, void* y, int nx, int ny, int nz, int mask_kind,
+                     const uint8_t* mask, const vh_epilogue* ep, void* stream);
+
 /* y = mask * (A r) for a dense row-major n x n matrix with leading dimension ld (ld >= n,
  * ld % 4 == 0, A 16-byte aligned; the padding columns are not read): the coarsest solve with the
  * explicit inverse of the coarsest operator.  a_dtype is the matrix type, v_dtype the type of r

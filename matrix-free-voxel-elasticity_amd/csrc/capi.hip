@@ -8,6 +8,7 @@
 #include "voxelhip.h"
 #include "common.h"
 #include "matvec.h"
+#include "stencil.h"
 #include "grid_ops.h"
 #include "blas.h"
 #include "simp.h"
@@ -181,6 +182,31 @@ static GridArgs<T> grid_args(int nx, int ny, int nz, int mask_kind, const uint8_
   return g;
 }
 
+template <typename T>
+static int stencil_t(const void* U, const void* u, void* y, int nx, int ny, int nz, int mask_kind, const uint8_t* mask,
+                     const vh_epilogue* ep, hipStream_t st) {
+  StencilParams<T> p{};
+  p.U = static_cast<const T*>(U);
+  p.u = static_cast<const T*>(u);
+  p.y = static_cast<T*>(y);
+  p.mask = mask;
+  p.nx = nx; p.ny = ny; p.nz = nz;
+  int mode = MODE_RAW;
+  if (ep) {
+    mode = ep->mode;
+    p.b = static_cast<const T*>(ep->b);
+    p.dinv = static_cast<const T*>(ep->dinv);
+    p.prev = static_cast<const T*>(ep->prev);
+    p.y32 = ep->y32;
+    p.fixed_zero = ep->fixed_zero;
+    p.flags = ep->flags;
+    p.omega = T(ep->omega);
+    p.cheb_a = T(ep->cheb_a);
+    p.cheb_c = T(ep->cheb_c);
+  }
+  return stencil_apply_launch<T>(p, mode, mask_kind, st);
+}
+
 extern "C" {
 
 int vh_abi_version(void) { return VH_ABI_VERSION; }
@@ -352,6 +378,42 @@ int vh_galerkin_scatter(const double* G, int nx, int ny, int nz, int mask_kind, 
   VH_REQUIRE(81.0 * n < 2.0e9 && ld >= n, "vh_galerkin_scatter: %.0f DOFs or ld %d out of range", n, ld);
   VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_galerkin_scatter: mask missing");
   return galerkin_scatter_launch(G, nx, ny, nz, mask_kind, mask, A, ld, static_cast<hipStream_t>(stream));
+}
+
+int vh_stencil_scatter(int dtype, const double* G, void* U, int nx, int ny, int nz, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype), "vh_stencil_scatter: dtype %d", dtype);
+  VH_REQUIRE(G && U && static_cast<const void*>(G) != U && dims_ok(nx, ny, nz), "vh_stencil_scatter: bad arguments");
+  const double nn = (nx + 1.0) * (ny + 1.0) * (nz + 1.0);
+  VH_REQUIRE(126.0 * nn < 2.0e9, "vh_stencil_scatter: %.0f nodes out of range", nn);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64) return stencil_scatter_launch<double>(G, static_cast<double*>(U), nx, ny, nz, st);
+  return stencil_scatter_launch<float>(G, static_cast<float*>(U), nx, ny, nz, st);
+}
+
+int vh_stencil_apply(int dtype, const void* U, const void* u, void* y, int nx, int ny, int nz, int mask_kind,
+                     const uint8_t* mask, const vh_epilogue* ep, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype), "vh_stencil_apply: dtype %d", dtype);
+  VH_REQUIRE(U && u && (y || (ep && ep->mode == MODE_RESID && ep->y32)), "vh_stencil_apply: null pointer");
+  VH_REQUIRE(u != y && U != y, "vh_stencil_apply: y must not alias u or U");
+  VH_REQUIRE(dims_ok(nx, ny, nz) && 126.0 * (nx + 1.0) * (ny + 1.0) * (nz + 1.0) < 2.0e9,
+             "vh_stencil_apply: bad grid %d x %d x %d", nx, ny, nz);
+  VH_REQUIRE(mask_kind >= 0 && mask_kind <= 2, "vh_stencil_apply: bad mask kind %d", mask_kind);
+  VH_REQUIRE(mask_kind != MASK_ARRAY || mask, "vh_stencil_apply: mask array missing");
+  if (ep) {
+    const int m = ep->mode;
+    VH_REQUIRE(m == MODE_RAW || m == MODE_RESID || m == MODE_JACOBI || m == MODE_CHEB3,
+               "vh_stencil_apply: mode %d (RAW, RESID, JACOBI and CHEB3 are available)", m);
+    VH_REQUIRE(!ep->d && !ep->partial && !ep->w && !ep->zold && !ep->u_out,
+               "vh_stencil_apply: d, partial, w, zold and u_out are parity-product fields");
+    VH_REQUIRE(m == MODE_RAW || ep->b, "vh_stencil_apply: b missing");
+    VH_REQUIRE(!(m == MODE_JACOBI || m == MODE_CHEB3) || ep->dinv, "vh_stencil_apply: dinv missing");
+    VH_REQUIRE(!ep->prev || (m == MODE_CHEB3 && ep->prev != y), "vh_stencil_apply: prev is a CHEB3 input");
+    VH_REQUIRE(!ep->y32 || m == MODE_RESID, "vh_stencil_apply: y32 is an output of the RESID pass");
+    VH_REQUIRE(!y || (ep->b != y && ep->dinv != y), "vh_stencil_apply: y must not alias an input");
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64) return stencil_t<double>(U, u, y, nx, ny, nz, mask_kind, mask, ep, st);
+  return stencil_t<float>(U, u, y, nx, ny, nz, mask_kind, mask, ep, st);
 }
 
 int vh_dense_mv(int a_dtype, int v_dtype, int n, int ld, const void* A, const void* r, void* y, const uint8_t* mask,

@@ -87,6 +87,44 @@ class HipProduct:
         return ops.diagonal(self.ke, E, d, self.shape, mask=mask, invert=invert)
 
 
+class StencilProduct:
+    """Assembled Galerkin operator K = P^T A_f P of one coarse level as the symmetric 27-point
+    node-block stencil U of vh_stencil_scatter (14 x 9 x nn coefficients), applied by
+    vh_stencil_apply; it takes the place of HipProduct on the non-coarsest levels of a Galerkin
+    hierarchy (donor_yang_gmg multigrid_v4.py:132-270,747-772).  `raw` ignores E."""
+
+    cube = False
+    kdiag = None
+    storage = None
+
+    def __init__(self, shape, U):
+        self.shape = tuple(int(s) for s in shape)
+        self.ndof = 3 * ops.n_nodes(self.shape)
+        if U.numel() != 42 * self.ndof:
+            raise ValueError("stencil size does not match the grid")
+        self.U = U.contiguous()
+        self.dtype = U.dtype
+
+    def astype(self, dtype):
+        return self if _tdt(dtype) == self.dtype else StencilProduct(self.shape, self.U.to(_tdt(dtype)))
+
+    def raw(self, u, E=None, out=None, mask=None, fixed_zero=False):
+        y = out if out is not None else torch.empty(self.ndof, dtype=self.dtype, device=u.device)
+        return ops.stencil_apply(self.U, u, y, self.shape, mask=mask, fixed_zero=fixed_zero)
+
+    def diagonal(self, E=None, out=None, mask=None, invert=False):
+        """Diagonal of K (1 at fixed DOFs under a mask), or its inverse."""
+        nn = self.ndof // 3
+        d = self.U.view(14, 9, nn)[0, [0, 4, 8], :].t().reshape(-1)
+        if mask is not None and mask.kind != L.MASK_NONE:
+            d = torch.where(mask.values(self.shape, self.dtype, d.device) != 0, d, torch.ones_like(d))
+        d = 1.0 / d if invert else d.clone()
+        if out is not None:
+            out.copy_(d)
+            return out
+        return d
+
+
 def assemble_dense(shape, ke, E, mask):
     """Dense M K M + (I - M) in FP64 on the host (src/voxel_mg.py:61-70); setup only."""
     import scipy.sparse as sp
@@ -148,7 +186,8 @@ class Level:
         self.mask = self.maskobj.values(self.shape, self.dtype, dev)
         self.fixed = 1 - self.mask
         self.n = 3 * ops.n_nodes(self.shape)
-        self.fused = hasattr(product, "blocks")
+        self.stencil = isinstance(product, StencilProduct)
+        self.fused = hasattr(product, "blocks") or self.stencil
         if self.fused:
             self.inv_diag = product.diagonal(self.E, mask=self.maskobj, invert=True)
         else:
@@ -157,6 +196,8 @@ class Level:
 
     def apply(self, u, out=None):
         y = out if out is not None else torch.empty(self.n, dtype=self.dtype, device=u.device)
+        if self.stencil:
+            return self.product.raw(u, out=y, mask=self.maskobj)
         if self.fused:
             return ops.matvec(self.product.blocks, self.E, u, y, self.shape, mask=self.maskobj,
                               storage=self.product.storage)
@@ -172,13 +213,18 @@ class Level:
     def tiles(self, mode=L.MODE_DOT, combine=False):
         """Number of per-tile partial sums a DOT (or RESID) pass of this level writes, under the
         launch plan that pass uses (tuned or automatic)."""
+        if self.stencil:
+            raise TypeError("a stencil level has no per-tile partial sums")
         t, lx = self._plan(mode, combine) or (0, 0)
         return ops.matvec_tiles(self.shape, lx=lx, tile=t, dtype=self.dtype, blocks=self.product.blocks,
                                 storage=self.product.storage)[0]
 
     def matvec(self, u, y, **kw):
         if not self.fused:
-            raise TypeError("fused passes need a HipProduct")
+            raise TypeError("fused passes need a HipProduct or a StencilProduct")
+        if self.stencil:
+            kw.pop("kdiag", None)
+            return ops.stencil_apply(self.product.U, u, y, self.shape, mask=self.maskobj, **kw)
         if "tile" not in kw and "lx" not in kw:
             plan = self._plan(kw.get("mode", L.MODE_RAW), kw.get("w") is not None)
             if plan is not None:
@@ -188,7 +234,7 @@ class Level:
 
     def tune(self, tuner, passes):
         """Empirical launch plans for the given (mode, combine) pass kinds (voxelhip.tuner)."""
-        if not self.fused:
+        if not self.fused or self.stencil:
             return
         self.launch = getattr(self, "launch", {})
         for mode, combine in passes:

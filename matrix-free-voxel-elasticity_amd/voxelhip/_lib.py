@@ -69,6 +69,8 @@ SIGNATURES = {
     "vh_heaviside": [_sz, _vp, _d, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i, _vp],
     "vh_oc_design": [_sz, _vp, _vp, _vp, _d, _d, _d, _vp, _vp],
     "vh_galerkin_scatter": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp],
+    "vh_stencil_scatter": [_i, _vp, _vp, _i, _i, _i, _vp],
+    "vh_stencil_apply": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Epilogue), _vp],
     "vh_reduce_parts": [_sz],
     "vh_reduce": [_i, _i, _sz, _vp, _vp, _vp, _vp, _vp],
     "vh_finish": [_vp, _i, _i, _vp, _i, _i, _vp],

@@ -18,6 +18,13 @@ precision:
                   with GMG in place of Jacobi): FP64 only for the outer residual b - A x, the
                   correction update x += |r| d, the dot accumulation and the coarsest solve;
                   the inner solve of A d = r / |r| stops at max(inner_tol, 0.5 tol / rel).
+
+coarse:
+  "rediscretize"  every coarse level rediscretised (ke * 2^l with averaged moduli; src/voxel_mg.py)
+  "galerkin"      rediscretised levels, the coarsest one the Galerkin P^T A P of the level above
+  "galerkin-all"  every coarse operator Galerkin, A_l = M (P^T A_(l-1) P) M + (I - M)
+                  (donor_yang_gmg multigrid_v4.py:132-270,747-772): the non-coarsest levels as
+                  assembled 27-point stencils (StencilProduct), the coarsest dense
 """
 from __future__ import annotations
 
@@ -56,6 +63,8 @@ class GMGSolver:
             from .tuner import LaunchTuner
             tune = LaunchTuner()
         self.tuner = tune or None
+        if coarse not in ("rediscretize", "galerkin", "galerkin-all"):
+            raise ValueError(f"coarse: {coarse!r}")
         self.coarse = coarse
         self.graph = graph
         self.lower_frac = lower_frac
@@ -73,11 +82,15 @@ class GMGSolver:
         E64 = E.to(self.dev, torch.float64).contiguous() if isinstance(E, torch.Tensor) else \
             torch.from_numpy(np.ascontiguousarray(E, dtype=np.float64)).to(self.dev)
         galerkin = self._galerkin_coarsest(E64) if (self.coarse == "galerkin" and len(self.shapes) > 1) else None
+        stencils = None
+        if self.coarse == "galerkin-all" and len(self.shapes) > 1:
+            stencils, galerkin = galerkin_hierarchy(self.shapes, self.ke, self.mask, E64)
         self.fine, self.M = build_hierarchy(self.shape, self.ke, self.mask, E64, self.shapes, self.pdtype,
                                             smoother=self.smoother, degree=self.degree, pre=self.pre,
                                             post=self.post, lower_frac=self.lower_frac,
                                             omega_iterations=self.omega_iterations,
-                                            coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin)
+                                            coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin,
+                                            stencils=stencils)
         # the last post-smoothing step of the V-cycle forms the Polak-Ribiere dots itself (and
         # writes z in the V-cycle precision, which the DOT pass reads directly)
         self.fused_pr = (self.smoother == "chebyshev" and len(self.shapes) > 1 and self.degree >= 1
@@ -347,13 +360,19 @@ class GMGSolver:
 
 
 def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev", degree=3, pre=4, post=4,
-                    lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None):
+                    lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None,
+                    stencils=None):
     """The FP64 fine Level and the Multigrid (levels in pdtype, rediscretised ke * 2^l, 2x2x2
     modulus averaging, injected masks, explicit inverse of the coarsest operator, spectral
     omegas / lambdas) of one problem.  coarse_matrix: a precomputed coarsest operator (Galerkin);
-    otherwise the rediscretised one is assembled."""
+    otherwise the rediscretised one is assembled.  stencils: assembled Galerkin operators
+    (StencilProduct) of levels 1 .. L-2, which then replace the rediscretised ones."""
     dev = E64.device
     prods = [mg.HipProduct(s, ke * 2 ** i, pdtype) for i, s in enumerate(shapes)]
+    if stencils is not None:
+        if len(stencils) != len(shapes) - 2:
+            raise ValueError("one stencil per level between the finest and the coarsest")
+        prods[1:-1] = [sp.astype(pdtype) for sp in stencils]
     fine = mg.Level(None, shape, mg.HipProduct(shape, ke, torch.float64), E64, mask, torch.float64)
     if coarse_matrix is None:
         Ec, m = E64, mask
@@ -377,34 +396,69 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
     return fine, M
 
 
-def galerkin_coarsest(fshape, ke, Ef, mask):
-    """Dense FP64 Galerkin operator M_c (P^T A_f P) M_c + (I - M_c) of the grid below `fshape`,
-    on the device (multigrid_v4.py:248-270,747-772 build it with host sparse triple products).
-    A_f = M K(Ef) M + (I - M) is the level's constrained operator (ke: that level's element
-    matrix).  P^T A_f P couples coarse nodes at most one apart, so 81 probes (the 27 node
-    residues mod 3 times 3 components) give every column: probe p = sum of e_(J, c) over the
-    nodes J of residue class p, pushed through prolongation, the level product and restriction,
-    and scattered into A_c by vh_galerkin_scatter."""
-    dev = Ef.device
+def galerkin_probes(fshape, apply, dev):
+    """G (81 x 3nn_c, FP64): R A_f P v_p for the 81 probes v_p of the grid below `fshape`, with
+    apply(f, y) the level's constrained operator y = A_f f.  P^T A_f P couples coarse nodes at
+    most one apart, so 81 probes (the 27 node residues mod 3 times 3 components) give every
+    column: probe p = sum of e_(J, c) over the nodes J of residue class p, pushed through
+    prolongation, the level operator and restriction (P and R unmasked)."""
     cshape = tuple(int(s) // 2 for s in fshape)
     cx, cy, cz = (s + 1 for s in cshape)
     nn, nf = cx * cy * cz, 3 * ops.n_nodes(fshape)
     node = torch.arange(nn, device=dev)
     iz, iy, ix = node % cz, (node // cz) % cy, node // (cz * cy)
     cls = ((ix % 3) * 3 + iy % 3) * 3 + iz % 3
-    V = torch.zeros((81, nn, 3), dtype=torch.float64, device=dev)
-    for c in range(3):
-        V[3 * cls + c, node, c] = 1.0
-    V = V.view(81, 3 * nn)
-    G = torch.empty_like(V)
-    blocks = mg.HipProduct(fshape, ke, torch.float64).blocks
-    mf = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, fshape, dev)
+    G = torch.empty((81, 3 * nn), dtype=torch.float64, device=dev)
+    v = torch.zeros((nn, 3), dtype=torch.float64, device=dev)
     f, y = (torch.empty(nf, dtype=torch.float64, device=dev) for _ in range(2))
     for p in range(81):
-        ops.prolong(V[p], f, fshape, assign=True)
-        ops.matvec(blocks, Ef, f, y, fshape, mask=mf)
+        v.zero_()
+        v[cls == p // 3, p % 3] = 1.0
+        ops.prolong(v.view(-1), f, fshape, assign=True)
+        apply(f, y)
         ops.restrict(y, G[p], fshape)
-    A = torch.zeros((3 * nn, 3 * nn), dtype=torch.float64, device=dev)
+    return G
+
+
+def galerkin_coarsest(fshape, ke, Ef, mask):
+    """Dense FP64 Galerkin operator M_c (P^T A_f P) M_c + (I - M_c) of the grid below `fshape`,
+    on the device (multigrid_v4.py:248-270,747-772 build it with host sparse triple products).
+    A_f = M K(Ef) M + (I - M) is the level's constrained operator (ke: that level's element
+    matrix); its 81 probe products (galerkin_probes) are scattered into A_c by
+    vh_galerkin_scatter."""
+    dev = Ef.device
+    cshape = tuple(int(s) // 2 for s in fshape)
+    blocks = mg.HipProduct(fshape, ke, torch.float64).blocks
+    mf = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, fshape, dev)
+    G = galerkin_probes(fshape, lambda f, y: ops.matvec(blocks, Ef, f, y, fshape, mask=mf), dev)
+    n = 3 * ops.n_nodes(cshape)
+    A = torch.zeros((n, n), dtype=torch.float64, device=dev)
     return ops.galerkin_scatter(G, A, cshape, mask=mf.coarsen(fshape))
 
 
+def galerkin_hierarchy(shapes, ke, mask, E64):
+    """Full Galerkin hierarchy A_l = M_l (P^T A_(l-1) P) M_l + (I - M_l) from the fine
+    constrained operator A_0 = M K(E) M + (I - M), every operator assembled on the device
+    (donor_yang_gmg multigrid_v4.py:132-270 for level 1 and :747-772 below, there as host-built
+    CSR triple products): levels 1 .. L-2 as FP64 stencils (StencilProduct, from the 81 probe
+    products of the level above through vh_stencil_scatter), the coarsest as the dense matrix of
+    vh_galerkin_scatter.  Returns (stencil products of levels 1 .. L-2, dense coarsest)."""
+    dev = E64.device
+    masks = [mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, shapes[0], dev)]
+    for i in range(1, len(shapes)):
+        masks.append(masks[-1].coarsen(shapes[i - 1]))
+    blocks = mg.HipProduct(shapes[0], ke, torch.float64).blocks
+    apply = lambda f, y: ops.matvec(blocks, E64, f, y, shapes[0], mask=masks[0])  # noqa: E731
+    stencils = []
+    for lvl in range(1, len(shapes) - 1):
+        G = galerkin_probes(shapes[lvl - 1], apply, dev)
+        U = torch.empty(126 * ops.n_nodes(shapes[lvl]), dtype=torch.float64, device=dev)
+        ops.stencil_scatter(G, U, shapes[lvl])
+        del G
+        sp = mg.StencilProduct(shapes[lvl], U)
+        stencils.append(sp)
+        apply = (lambda f, y, sp=sp, m=masks[lvl]: sp.raw(f, out=y, mask=m))  # noqa: E731
+    G = galerkin_probes(shapes[-2], apply, dev)
+    n = 3 * ops.n_nodes(shapes[-1])
+    A = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    return stencils, ops.galerkin_scatter(G, A, shapes[-1], mask=masks[-1])

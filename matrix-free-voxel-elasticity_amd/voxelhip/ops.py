@@ -261,6 +261,37 @@ def galerkin_scatter(G, A, coarse_shape, mask=None):
     return A
 
 
+def stencil_scatter(G, U, coarse_shape):
+    """Symmetric 27-point stencil U (14 x 9 x nn, float64 or float32) of the unmasked Galerkin
+    operator P^T A_f P from the 81 probe rows G (see include/voxelhip.h, vh_stencil_scatter)."""
+    _dev(G, "G"); _dev(U, "U")
+    nn = n_nodes(coarse_shape)
+    if G.shape != (81, 3 * nn) or G.dtype != torch.float64 or U.numel() != 126 * nn or not U.is_contiguous():
+        raise ValueError("stencil_scatter: G must be 81 x 3nn float64 and U contiguous with 126 nn entries")
+    L.call("vh_stencil_scatter", dtype_code(U), _p(G), _p(U), *[int(s) for s in coarse_shape], _stream())
+    return U
+
+
+def stencil_apply(U, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None, omega=0.0, cheb_a=0.0,
+                  cheb_c=0.0, fixed_zero=False, prev=None, prev_zero=False, y32=None):
+    """One pass of the stencil operator M K M + I - M with the RAW, RESID, JACOBI or CHEB3
+    epilogue (see include/voxelhip.h, vh_stencil_apply)."""
+    _dev(U, "U"); _dev(u, "u")
+    if y is None and not (mode == L.MODE_RESID and y32 is not None):
+        raise ValueError("only a RESID pass writing y32 may omit y")
+    nd = 3 * n_nodes(shape)
+    if u.numel() != nd or (y is not None and (y.numel() != nd or y.dtype != u.dtype)) or U.dtype != u.dtype \
+            or U.numel() != 42 * nd:
+        raise ValueError(f"stencil_apply: sizes or dtypes do not match grid {shape}")
+    m = mask or Mask.none()
+    ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), None, None, float(omega), float(cheb_a),
+                    float(cheb_c), 0, 0, (ctypes.c_double * 3)(0.0, 0.0, 0.0), int(bool(prev_zero)),
+                    None, None, 0, None, _p(y32), 1.0, _p(prev), None)
+    L.call("vh_stencil_apply", dtype_code(u), _p(U), _p(u), _p(y), *[int(s) for s in shape], m.kind, m.ptr,
+           ctypes.byref(ep), _stream())
+    return y
+
+
 def padded_matrix(A, dtype=None):
     """Copy of the square matrix A with its rows padded to a multiple of 4 entries (16-byte
     aligned rows for the vector loads of dense_mv); the padding is zero."""

@@ -12,6 +12,9 @@ Parity status:
                      P^T K_f P == K_c for uniform E (tests/test_mg_cpu.py:53-67)
   galerkin_dense     equals the rediscretised operator for uniform E, with and without the
                      clamped face; symmetric; fixed rows are identity rows
+  galerkin_hierarchy level 1 equals galerkin_dense; for uniform E and the clamp every level
+                     equals the rediscretised constrained operator; symmetric positive definite
+                     for random moduli and masks
   Multigrid/fcg      reference-held known answers (smoke-q4 iterations, compliance, solution
                      npz of results/session-d-20260923), golden fixtures of one V-cycle and of
                      spectral_omegas written from the reference in this container
@@ -86,6 +89,35 @@ def galerkin_dense(fine_shape, ke, E, mask):
     Ac = P.T @ Af @ P
     mc = coarsen_mask(mask, fine_shape).astype(np.float64)
     return mc[:, None] * Ac * mc[None, :] + np.diag(1.0 - mc)
+
+
+def galerkin_hierarchy(shapes, ke, E, mask):
+    """Full Galerkin hierarchy (multigrid_v4.py:132-270 level 1, :747-772 deeper levels, there
+    R K P in CSR): A_0 = M K(E) M + (I - M) (assemble_dense), K_l = P^T A_(l-1) P and
+    A_l = M_l K_l M_l + (I - M_l) with the injected masks.  Dense; small grids only.
+    Returns (K_1 .. K_(L-1) unmasked, A_0 .. A_(L-1) constrained)."""
+    A = [assemble_dense(shapes[0], ke, E, mask)]
+    K, m = [], np.asarray(mask, dtype=np.float64)
+    for lvl in range(1, len(shapes)):
+        P = prolongation_matrix(shapes[lvl])
+        K.append(P.T @ A[-1] @ P)
+        m = coarsen_mask(m, shapes[lvl - 1]).astype(np.float64)
+        A.append(m[:, None] * K[-1] * m[None, :] + np.diag(1.0 - m))
+    return K, A
+
+
+class DenseProduct:
+    """An assembled (unmasked) level operator K in the product interface of Level:
+    raw(u, E) = K u and diagonal(E) = diag K (E is not used)."""
+
+    def __init__(self, K, dtype=np.float64):
+        self.K, self.dtype = np.asarray(K, dtype=np.float64), dtype
+
+    def raw(self, u, E):
+        return (self.K @ np.asarray(u, dtype=np.float64)).astype(self.dtype)
+
+    def diagonal(self, E):
+        return np.diag(self.K).astype(self.dtype)
 
 
 def coarsen_mask(mask, shape):

@@ -235,3 +235,137 @@ def test_galerkin_coarsest_solve(KE):
     # same solution; the Galerkin coarsest is the better coarse operator for varying moduli
     assert float((out["galerkin"][0] - out["rediscretize"][0]).norm() / out["rediscretize"][0].norm()) < 1e-7
     assert out["galerkin"][1] <= out["rediscretize"][1] + 1
+
+
+# ------------------------------------------------------------------ full Galerkin hierarchy
+def _galerkin_case(kind, shapes, seed=12):
+    rng = np.random.default_rng(seed)
+    n0 = 3 * hx.n_nodes(shapes[0])
+    E = rng.uniform(0.01, 1.0, int(np.prod(shapes[0])))
+    mask = np.ones(n0)
+    if kind == "x0":
+        mask[: 3 * (shapes[0][1] + 1) * (shapes[0][2] + 1)] = 0
+    elif kind == "array":
+        mask = (rng.random(n0) > 0.15).astype(float)
+    return E, mask
+
+
+@pytest.mark.parametrize("kind", ["none", "x0", "array"])
+def test_galerkin_stencils_match_the_oracle_hierarchy(KE, kind):
+    """Levels 1 and 2 as assembled stencils and the dense coarsest of level 3 against the dense
+    recursion A_l = M (P^T A_(l-1) P) M + (I - M) of the oracle."""
+    from voxelhip import ops
+    from voxelhip.engine import galerkin_hierarchy
+    shapes = [(16, 8, 8), (8, 4, 4), (4, 2, 2), (2, 1, 1)]
+    E, mask = _galerkin_case(kind, shapes)
+    print("case", kind, flush=True)
+    K, A = mg_ref.galerkin_hierarchy(shapes, KE, E, mask)
+    stencils, Ac = galerkin_hierarchy(shapes, KE, ops.Mask.from_values(mask, shapes[0]), torch.from_numpy(E).cuda())
+    assert len(stencils) == 2
+    rng = np.random.default_rng(3)
+    m = mask
+    for lvl, sp in ((1, stencils[0]), (2, stencils[1])):
+        m = mg_ref.coarsen_mask(m, shapes[lvl - 1])
+        mo = ops.Mask.from_values(m, shapes[lvl])
+        u = rng.standard_normal(A[lvl].shape[0])
+        y = sp.raw(torch.from_numpy(u).cuda(), mask=mo)
+        assert rel(y, A[lvl] @ u) < 1e-13, lvl
+        # the unmasked operator, and its diagonal
+        assert rel(sp.raw(torch.from_numpy(u).cuda()), K[lvl - 1] @ u) < 1e-13, lvl
+        assert rel(sp.diagonal(), np.diag(K[lvl - 1])) < 1e-14
+    assert np.abs(Ac.cpu().numpy() - A[3]).max() <= 1e-12 * np.abs(A[3]).max()
+
+
+@pytest.mark.parametrize("kind", ["x0", "array"])
+def test_stencil_epilogues_match_their_formulas(KE, kind):
+    import voxel_mg as mg
+    from voxelhip import _lib as L
+    from voxelhip import ops
+    from voxelhip.engine import galerkin_hierarchy
+    shapes = [(12, 8, 8), (6, 4, 4), (3, 2, 2)]
+    E, mask = _galerkin_case(kind, shapes, seed=4)
+    K, A = mg_ref.galerkin_hierarchy(shapes, KE, E, mask)
+    (sp,), _ = galerkin_hierarchy(shapes, KE, ops.Mask.from_values(mask, shapes[0]), torch.from_numpy(E).cuda())
+    m = mg_ref.coarsen_mask(mask, shapes[0])
+    mo = ops.Mask.from_values(m, shapes[1])
+    A1, K1 = A[1], K[0]
+    n = A1.shape[0]
+    rng = np.random.default_rng(8)
+    u, b, pv = (rng.standard_normal(n) for _ in range(3))
+    dinv = np.where(m != 0, 1.0 / np.diag(K1), 1.0)
+    assert rel(sp.diagonal(mask=mo, invert=True), dinv) < 1e-15
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()  # noqa: E731
+    Au = A1 @ u
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    y32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    print("case", kind, "resid", flush=True)
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_RESID, b=t(b), y32=y32)
+    assert rel(y, b - Au) < 1e-13 and rel(y32.double(), b - Au) < 1e-7
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_RESID, b=t(b), fixed_zero=True)
+    assert rel(y, (b - Au) * m) < 1e-13
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, fixed_zero=True)
+    assert rel(y, m * (K1 @ (m * u))) < 1e-13
+    print("case", kind, "jacobi/cheb3", flush=True)
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_JACOBI, b=t(b), dinv=t(dinv), omega=0.6)
+    assert rel(y, u + 0.6 * dinv * (b - Au)) < 1e-13
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_CHEB3, b=t(b), dinv=t(dinv), cheb_a=0.7,
+                      cheb_c=0.2, prev=t(pv))
+    assert rel(y, u + 0.7 * (dinv * (b - Au) + 0.2 * (u - pv))) < 1e-13
+    y.copy_(t(pv))                    # y holds the previous iterate on entry
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_CHEB3, b=t(b), dinv=t(dinv), cheb_a=0.7,
+                      cheb_c=0.2)
+    assert rel(y, u + 0.7 * (dinv * (b - Au) + 0.2 * (u - pv))) < 1e-13
+    ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_CHEB3, b=t(b), dinv=t(dinv), cheb_a=0.7,
+                      cheb_c=0.2, prev_zero=True)
+    assert rel(y, u + 0.7 * (dinv * (b - Au) + 0.2 * u)) < 1e-13
+    # the FP32 copy of the stencil
+    s32 = sp.astype(torch.float32)
+    y32b = s32.raw(t(u).float(), mask=mo)
+    assert rel(y32b.double(), Au) < 1e-5
+    with pytest.raises(Exception):
+        ops.stencil_apply(sp.U, t(u), y, shapes[1], mask=mo, mode=L.MODE_DOT)
+    assert isinstance(mg.StencilProduct(shapes[1], sp.U), mg.StencilProduct)
+
+
+def test_galerkin_hierarchy_vcycle_matches_the_oracle(KE):
+    """One Chebyshev V-cycle over Galerkin levels against the oracle's Multigrid with the dense
+    Galerkin operators (same lambdas)."""
+    import voxel_mg as mg
+    from voxelhip import ops
+    from voxelhip.engine import build_hierarchy, galerkin_hierarchy
+    shapes = [(16, 8, 8), (8, 4, 4), (4, 2, 2), (2, 1, 1)]
+    E, mask = _galerkin_case("x0", shapes, seed=9)
+    K, A = mg_ref.galerkin_hierarchy(shapes, KE, E, mask)
+    E64 = torch.from_numpy(E).cuda()
+    mo = ops.Mask.from_values(mask, shapes[0])
+    stencils, Ac = galerkin_hierarchy(shapes, KE, mo, E64)
+    _, M = build_hierarchy(shapes[0], KE, mo, E64, shapes, torch.float64, smoother="chebyshev", degree=3,
+                           coarse_matrix=Ac, stencils=stencils)
+    prods = [hx.NumpyProduct(shapes[0], KE)] + [mg_ref.DenseProduct(k) for k in K[:-1]] + \
+        [hx.NumpyProduct(shapes[-1], KE * 8)]
+    Mo = mg_ref.Multigrid(shapes, prods, E, mask, np.linalg.cholesky(A[-1]), np.float64, pre=3, post=3,
+                          smoother="chebyshev", lambdas=M.lambdas)
+    lam_o = mg_ref.chebyshev_bounds(Mo.levels)
+    assert np.allclose(M.lambdas, lam_o, rtol=1e-10)
+    r = np.random.default_rng(5).standard_normal(mask.size) * mask
+    assert rel(M(torch.from_numpy(r).cuda()), Mo(r)) < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_full_galerkin_solve(KE, precision):
+    from voxelhip.engine import GMGSolver
+    shape = (32, 16, 16)
+    rho, E, mask, force = sr.cantilever(shape, seed=6, rmin=1.5)
+    # a high-contrast design: void below rho = 0.5
+    E = np.where(rho > 0.5, E, 1e-6)
+    f = torch.from_numpy(force).cuda()
+    out = {}
+    for coarse in ("rediscretize", "galerkin-all"):
+        print("case", precision, coarse, flush=True)
+        s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=4, smoother="chebyshev", degree=3,
+                      precision=precision, coarse=coarse)
+        x, it, ok, r, _ = s.solve(f, tol=1e-8, maxiter=400)
+        assert ok and r <= 1e-8, (coarse, it, r)
+        out[coarse] = (x, it)
+    assert float((out["galerkin-all"][0] - out["rediscretize"][0]).norm() / out["rediscretize"][0].norm()) < 1e-6
+    assert out["galerkin-all"][1] <= out["rediscretize"][1], (out["galerkin-all"][1], out["rediscretize"][1])

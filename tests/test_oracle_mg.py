@@ -90,6 +90,40 @@ def test_galerkin_operator_of_random_moduli_is_spd_with_identity_rows(KE):
     assert np.linalg.eigvalsh(Ac).min() > 0
 
 
+def test_galerkin_hierarchy_level_one_is_galerkin_dense(KE):
+    shapes = [(8, 4, 4), (4, 2, 2), (2, 1, 1)]
+    rng = np.random.default_rng(11)
+    mask = (rng.random(3 * hx.n_nodes(shapes[0])) > 0.15).astype(float)
+    E = rng.uniform(0.01, 1, 128)
+    K, A = mg.galerkin_hierarchy(shapes, KE, E, mask)
+    ref = mg.galerkin_dense(shapes[0], KE, E, mask)
+    assert np.abs(A[1] - ref).max() < 1e-13 * np.abs(ref).max()
+    for a in A[1:]:
+        assert np.abs(a - a.T).max() < 1e-13 * np.abs(a).max()
+        assert np.linalg.eigvalsh(a).min() > 0
+
+
+@pytest.mark.parametrize("clamped", [False, True])
+def test_galerkin_hierarchy_equals_rediscretisation_for_uniform_modulus(KE, clamped):
+    """P^T K_h P = K_2h for trilinear P and uniform moduli, level after level; the clamp's fixed
+    fine nodes interpolate only from fixed coarse nodes, so the masks nest."""
+    shapes = [(8, 4, 4), (4, 2, 2), (2, 1, 1)]
+    mask = np.ones(3 * hx.n_nodes(shapes[0]))
+    if clamped:
+        mask[: 3 * 5 * 5] = 0
+    _, A = mg.galerkin_hierarchy(shapes, KE, np.full(128, 0.7), mask)
+    m = mask
+    for lvl in range(1, 3):
+        m = mg.coarsen_mask(m, shapes[lvl - 1])
+        ne = int(np.prod(shapes[lvl]))
+        R = mg.assemble_dense(shapes[lvl], KE * 2 ** lvl, np.full(ne, 0.7), m)
+        assert np.abs(A[lvl] - R).max() < 1e-12 * np.abs(R).max(), lvl
+    # a plausible slip - restricting with P instead of P^T scaled to the coarse size - is caught
+    P = mg.prolongation_matrix(shapes[1])
+    wrong = (P / 8.0).T @ A[0] @ P
+    assert np.abs(mg.coarsen_mask(mask, shapes[0])[:, None] * wrong - A[1]).max() > 1e-3 * np.abs(A[1]).max()
+
+
 def test_vcycle_is_symmetric(golden, KE):
     shape, E, mask, _ = smoke(golden)
     _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2)
