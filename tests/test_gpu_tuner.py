@@ -15,7 +15,8 @@ KE = 2 * hx.quadrature([1, 1, 1], 0.3)
 
 
 @pytest.mark.parametrize("dtype,shape", [(torch.float64, (24, 20, 36)), (torch.float32, (24, 20, 36)),
-                                         (torch.float64, (10, 35, 101)), (torch.float32, (6, 67, 40))])
+                                         (torch.float64, (10, 35, 101)), (torch.float32, (6, 67, 40)),
+                                         (torch.float64, (8, 20, 70))])
 def test_every_launch_plan_gives_the_same_product(dtype, shape):
     import voxel_kernels as vk
     from voxelhip import ops
