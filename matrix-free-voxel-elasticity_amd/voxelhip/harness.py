@@ -1,0 +1,302 @@
+"""Study harness of the product and the solvers on one MI355X: inputs, correctness gates, paired
+randomized timing with a bootstrap decision, the W2b solve protocol and result.json records.
+
+It restates the reference's harness (code/hex_modal_cuda/common.py:57-117 and run_gpu.py,
+experiments/session_i/make_inputs.py:31-49 and run_gpu.py:90-104,255-330) for the paths of this
+library:
+
+- inputs: `regenerate(shape)` (every DOF of the x = 0 face clamped; a consistent line load of total
+  1 in -z along y at x = nx, z = 0), `upsample(data, factor)` (nearest-neighbour refinement of a
+  design, the boundary data regenerated), `to_author` (natural (z, y, x[, 3]) -> author layout);
+- correctness: `product_check` (relative and per-sqrt(DOF) absolute L2 limits), `exactness` (zero in,
+  zero out; identity rows at fixed DOFs), `physical_check` (true residual recomputed by the dense
+  node kernel, an arithmetic independent of the parity product, and the energy / force-work check);
+- timing: `paired_rounds` (each round runs every path once, in a seeded random order, HIP events
+  around a batch of calls) and `timing_decision` (per baseline the paired median ratio and the 5 %
+  bootstrap lower bound of the median; a primary case passes at median >= 1.15 and bound > 1.05);
+- `solve_case`: the W2b solve of the scale study through the drop-in voxel_mg (voxel hierarchy
+  halved to x <= 16, damped Jacobi V(4,4) with omega_l = 4/(3 lambda_l) from 30 seeded power steps,
+  dense Cholesky coarsest, FP64 flexible CG to 1e-8 with the true residual every iteration);
+- `write_result`: result.json written atomically, NaN and infinities rejected; `check_record`
+  validates a solve record against RESULT_SCHEMA.
+
+    python -m voxelhip.harness --inputs DIR --case q64x2 --out result.json
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import ops
+
+# the scale study's declared protocol (experiments/session_i/protocol.json: multigrid, solver,
+# material, order_seed) and its input cases (make_inputs.py:23-29: base design, upsampling factor)
+PROTOCOL = {
+    "multigrid": {"max_coarse_x": 16, "pre": 4, "post": 4, "power_iterations": 30},
+    "solver": {"requested_rtol": 1e-8, "true_residual_limit": 1e-7, "energy_force_relative_limit": 1e-6,
+               "maximum_iterations": 200, "check_every": 1},
+    "material": {"Emin": 1e-6, "Emax": 1.0, "penal": 3.0, "nu": 0.3},
+    "order_seed": 2026092621,
+    "correctness": {"relative_l2": 1e-11, "absolute_per_sqrt_dof": 1e-12},
+    "timing": {"rounds": 5, "calls_per_batch": 10, "bootstrap_seed": 20260923, "bootstrap_resamples": 10000,
+               "median_ratio_gate": 1.15, "lower_bound_gate": 1.05},
+}
+CASES = {"q4x2": ("smoke-q4.npz", 2), "q96x1": ("optimized-q96.npz", 1), "q64x2": ("optimized-q64.npz", 2),
+         "q96x2": ("optimized-q96.npz", 2), "q96x3": ("optimized-q96.npz", 3)}
+
+# fields of a solve record and their types (result.json "solves" rows)
+RESULT_SCHEMA = {
+    "case": str, "shape": list, "ndof": int, "iterations": int, "native_converged": bool,
+    "native_true_residual": float, "independent_true_residual": float, "compliance": float,
+    "force_work": float, "energy_force_relative_error": float, "solution_norm": float,
+    "solution_sha256": str, "omegas": list, "lambdas": list, "setup_seconds": float,
+    "solver_seconds": float, "physical_passed": bool,
+}
+
+
+# ---------------------------------------------------------------------------------- inputs
+def regenerate(shape):
+    """Mask and force of the cantilever in the natural layout (z, y, x, 3) at any resolution."""
+    nx, ny, nz = (int(n) for n in shape)
+    mask = np.ones((nz + 1, ny + 1, nx + 1, 3))
+    mask[:, :, 0, :] = 0.0
+    force = np.zeros((nz + 1, ny + 1, nx + 1, 3))
+    line = np.full(ny + 1, -1.0 / ny)
+    line[[0, -1]] = -0.5 / ny
+    force[0, :, nx, 2] = line
+    return mask.ravel(), force.ravel()
+
+
+def upsample(data, factor):
+    """Each element repeated `factor` times per axis; boundary data regenerated at the new size."""
+    nx, ny, nz = (int(n) for n in data["shape"])
+    rho = np.asarray(data["rho"]).reshape(nz, ny, nx)
+    for axis in range(3):
+        rho = np.repeat(rho, int(factor), axis=axis)
+    shape = np.array([nx * factor, ny * factor, nz * factor])
+    mask, force = regenerate(shape)
+    return {"shape": shape, "rho": np.ascontiguousarray(rho.ravel()), "mask": mask, "force": force,
+            "ke": np.asarray(data["ke"])}
+
+
+def to_author(a, shape, nodal=False):
+    """Natural (z, y, x[, 3]) order -> author (x, y, z[, 3]) order."""
+    x, y, z = (int(s) for s in shape)
+    sizes = (z + 1, y + 1, x + 1, 3) if nodal else (z, y, x)
+    axes = (2, 1, 0, 3) if nodal else (2, 1, 0)
+    return np.ascontiguousarray(np.asarray(a).reshape(sizes).transpose(axes)).ravel()
+
+
+def load_case(inputs, case):
+    """The named study case from the base designs in directory `inputs`."""
+    base, factor = CASES[case]
+    with np.load(Path(inputs) / base) as f:
+        data = {k: np.asarray(f[k]) for k in f.files}
+    return upsample(data, factor)
+
+
+# ----------------------------------------------------------------------------- correctness
+def relative(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def product_check(actual, expected, limits=None):
+    """L2 error against relative_l2 * |expected| + absolute_per_sqrt_dof * sqrt(n)."""
+    c = limits or PROTOCOL["correctness"]
+    a = actual.detach().cpu().numpy() if isinstance(actual, torch.Tensor) else np.asarray(actual)
+    e = expected.detach().cpu().numpy() if isinstance(expected, torch.Tensor) else np.asarray(expected)
+    err = float(np.linalg.norm(a - e))
+    limit = c["relative_l2"] * float(np.linalg.norm(e)) + c["absolute_per_sqrt_dof"] * math.sqrt(e.size)
+    return {"relative_l2": relative(a, e), "absolute_l2": err, "limit_l2": limit,
+            "passed": bool(np.isfinite(a).all() and err <= limit)}
+
+
+def exactness(apply, u, mask):
+    """Zero input gives an exactly zero output; fixed DOFs return the input exactly."""
+    z = apply(torch.zeros_like(u))
+    y = apply(u)
+    fixed = mask == 0
+    return {"zero_exact": bool(torch.count_nonzero(z) == 0), "fixed_identity_exact": bool(torch.equal(y[fixed], u[fixed]))}
+
+
+def constrained_dense(ke, E, shape, mask):
+    """A u = M K M u + (I - M) u with the dense 24x24 node kernel (the independent arithmetic)."""
+    def apply(u):
+        y = torch.empty_like(u)
+        ops.dense_product(ke, E, u * mask, y, shape)
+        return y * mask + u * (1 - mask)
+    return apply
+
+
+def physical_check(shape, ke, E, mask, force, u, native_ok, native_rel, limits=None):
+    """True residual by the dense kernel, compliance sum E_e u_e^T ke u_e against the force work."""
+    s = limits or PROTOCOL["solver"]
+    import voxel_kernels as vk
+    blocks = vk.Modal(ke).blocks
+    independent = relative(constrained_dense(ke, E, shape, mask)(u), force)
+    ce = torch.empty_like(E)
+    ops.element_energy(blocks, u * mask, ce, shape)
+    compliance = float(E @ ce)
+    work = float(force @ u)
+    err = abs(compliance / work - 1.0) if work != 0 else float("inf")
+    return {"native_converged": bool(native_ok), "native_true_residual": float(native_rel),
+            "independent_true_residual": independent, "compliance": compliance, "force_work": work,
+            "energy_force_relative_error": err,
+            "physical_passed": bool(native_ok and native_rel <= s["requested_rtol"]
+                                    and independent <= s["true_residual_limit"]
+                                    and err <= s["energy_force_relative_limit"])}
+
+
+# ---------------------------------------------------------------------------------- timing
+def paired_rounds(paths, rounds=None, calls=None, seed=None, case=""):
+    """paths: name -> zero-argument callable launching one product.  Every round runs every path
+    once, in an order drawn from the seed; each entry times `calls` launches between HIP events."""
+    t = PROTOCOL["timing"]
+    rounds = t["rounds"] if rounds is None else rounds
+    calls = t["calls_per_batch"] if calls is None else calls
+    rng = np.random.default_rng(PROTOCOL["order_seed"] if seed is None else seed)
+    names = list(paths)
+    for fn in paths.values():          # warm-up
+        fn()
+    torch.cuda.synchronize()
+    rows = []
+    for r in range(rounds):
+        for pos, i in enumerate(rng.permutation(len(names))):
+            name = names[int(i)]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(calls):
+                paths[name]()
+            e1.record()
+            e1.synchronize()
+            rows.append({"id": case, "round": r, "position": pos, "path": name, "calls": calls,
+                         "gpu_ms_per_call": e0.elapsed_time(e1) / calls})
+    return rows
+
+
+def timing_decision(timings, candidate, primary=True, seed=None, resamples=None):
+    """Per baseline path: the median over rounds of baseline / candidate time and the 5 % quantile
+    of the bootstrap distribution of that median (rounds resampled with replacement)."""
+    t = PROTOCOL["timing"]
+    rng = np.random.default_rng(t["bootstrap_seed"] if seed is None else seed)
+    resamples = t["bootstrap_resamples"] if resamples is None else resamples
+    by_path = {}
+    for row in sorted(timings, key=lambda r: r["round"]):
+        by_path.setdefault(row["path"], []).append(row["gpu_ms_per_call"])
+    cand = np.asarray(by_path[candidate])
+    if not (cand > 0).all() or not np.isfinite(cand).all():
+        raise ValueError("candidate timings must be positive and finite")
+    out = []
+    for name, vals in by_path.items():
+        if name == candidate:
+            continue
+        base = np.asarray(vals)
+        if base.shape != cand.shape:
+            raise ValueError("every path needs one timing per round")
+        ratios = base / cand
+        idx = rng.integers(0, len(ratios), size=(resamples, len(ratios)))
+        medians = np.median(ratios[idx], axis=1)
+        med, lower = float(np.median(ratios)), float(np.quantile(medians, 0.05))
+        out.append({"baseline": name, "candidate_median_ms": float(np.median(cand)),
+                    "baseline_median_ms": float(np.median(base)), "paired_median_ratio": med,
+                    "paired_bootstrap_lower_95": lower, "primary": bool(primary),
+                    "passed": med >= t["median_ratio_gate"] and lower > t["lower_bound_gate"]})
+    return {"comparisons": out, "timing_gate_passed": all(c["passed"] for c in out if c["primary"])}
+
+
+# ---------------------------------------------------------------------------------- solves
+def solve_case(data, case="", device=None):
+    """The W2b FP64 multigrid solve of one case through the drop-in voxel_mg; returns the record."""
+    import voxel_mg as mg
+    import voxelhip
+    p = PROTOCOL
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    shape = tuple(int(s) for s in data["shape"])
+    ke = np.asarray(data["ke"], dtype=np.float64)
+    emin = p["material"]["Emin"]
+    t0 = time.perf_counter()
+    E_h = to_author(emin + (1 - emin) * np.asarray(data["rho"]) ** p["material"]["penal"], shape)
+    m_h = to_author(data["mask"], shape, True)
+    f_h = to_author(data["force"], shape, True)
+    E = torch.from_numpy(E_h).to(dev)
+    mask = torch.from_numpy(m_h).to(dev)
+    force = torch.from_numpy(f_h).to(dev)
+    shapes = mg.hierarchy_shapes(shape, p["multigrid"]["max_coarse_x"])
+    prods = [mg.HipProduct(s, ke * 2 ** i, np.float64) for i, s in enumerate(shapes)]
+    Ec, mc = E_h, m_h
+    for i in range(1, len(shapes)):
+        Ec = np.asarray(Ec).reshape(shapes[i - 1][0] // 2, 2, shapes[i - 1][1] // 2, 2,
+                                    shapes[i - 1][2] // 2, 2).mean(axis=(1, 3, 5)).ravel()
+        mc = np.asarray(mc).reshape(shapes[i - 1][0] + 1, shapes[i - 1][1] + 1, shapes[i - 1][2] + 1,
+                                    3)[::2, ::2, ::2, :].ravel()
+    factor = np.linalg.cholesky(mg.assemble_dense(shapes[-1], ke * 2 ** (len(shapes) - 1), Ec, mc))
+    xp = voxelhip.xp
+    M = mg.Multigrid(xp, shapes, prods, E, mask, factor, np.float64, pre=p["multigrid"]["pre"],
+                     post=p["multigrid"]["post"])
+    omegas, lambdas = mg.spectral_omegas(xp, M.levels, p["multigrid"]["power_iterations"], p["order_seed"])
+    M.omegas = omegas
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    fine = M.levels[0]
+    t1 = time.perf_counter()
+    x, it, ok, rel, _ = mg.fcg(xp, fine.apply, force, M, p["solver"]["requested_rtol"],
+                               p["solver"]["maximum_iterations"], check_every=p["solver"]["check_every"])
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t1
+    rec = {"case": case, "shape": list(shape), "ndof": int(force.numel()), "iterations": int(it),
+           "omegas": [float(v) for v in omegas], "lambdas": [float(v) for v in lambdas],
+           "setup_seconds": setup, "solver_seconds": seconds,
+           "solution_norm": float(x.norm()),
+           "solution_sha256": hashlib.sha256(x.detach().cpu().numpy().tobytes()).hexdigest()}
+    rec.update(physical_check(shape, ke, E, mask, force, x, ok, rel))
+    return rec, x
+
+
+# ---------------------------------------------------------------------------------- records
+def check_record(rec):
+    """Raises if a solve record misses a field of RESULT_SCHEMA or has the wrong type."""
+    for key, typ in RESULT_SCHEMA.items():
+        if key not in rec:
+            raise ValueError(f"record misses {key}")
+        if typ is float and isinstance(rec[key], (int, float)) and not isinstance(rec[key], bool):
+            continue
+        if not isinstance(rec[key], typ):
+            raise ValueError(f"{key}: {type(rec[key]).__name__}, expected {typ.__name__}")
+    return True
+
+
+def write_result(path, result):
+    """Atomic JSON write (a temporary file renamed over the target); NaN and infinities raise."""
+    path = Path(path)
+    tmp = path.with_name(path.name + ".tmp")
+    tmp.write_text(json.dumps(result, indent=2, allow_nan=False) + "\n")
+    tmp.replace(path)
+    return path
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--inputs", type=Path, required=True, help="directory holding the base designs (npz)")
+    ap.add_argument("--case", choices=sorted(CASES), required=True)
+    ap.add_argument("--out", type=Path, required=True)
+    args = ap.parse_args(argv)
+    data = load_case(args.inputs, args.case)
+    rec, _ = solve_case(data, args.case)
+    check_record(rec)
+    write_result(args.out, {"status": "completed", "protocol": PROTOCOL, "solves": [rec]})
+    print(json.dumps({k: rec[k] for k in ("case", "iterations", "native_true_residual", "compliance",
+                                          "physical_passed")}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
