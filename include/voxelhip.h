@@ -121,7 +121,10 @@ int vh_blocks_are_cube(const double* blocks72);
 int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int ny, int nz, int lx, int tile,
                     int* n_tiles, int* n_workgroups);
 
-/* Node-owned product with the dense 24x24 ke (host pointer): the dense8 / node control.
+/* Node-owned product with the dense 24x24 ke (host pointer): the dense8 / node control.  The
+ * library keeps a device copy per distinct ke (per device and dtype, for the life of the process),
+ * so repeated calls with a matrix seen before copy nothing and may be captured in a HIP graph; a
+ * new ke inside a capture is an error.
  * Replaces: dense8 code/hex_modal_cuda/cuda.py:47-79 and node_matvec_fp64
  * experiments/session_a/donor/src/gpu_fem/cuda_operators.py:204-285. */
 int vh_dense_product(int dtype, const double* ke576, const void* E, const void* u, void* y, int nx, int ny, int nz,

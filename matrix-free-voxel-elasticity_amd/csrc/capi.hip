@@ -4,6 +4,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "voxelhip.h"
 #include "common.h"
@@ -207,6 +209,56 @@ static int stencil_t(const void* U, const void* u, void* y, int nx, int ny, int 
   return stencil_apply_launch<T>(p, mode, mask_kind, st);
 }
 
+// Device copies of 24x24 element matrices for the dense node kernel, one per (device, dtype,
+// contents), kept for the life of the process: a call with a matrix seen before launches without
+// a copy (and can be captured in a HIP graph); a new matrix is copied on the call's stream.
+struct KeEntry {
+  int device, dtype;
+  double ke[576];
+  void* ptr;
+};
+static std::vector<KeEntry>& ke_cache() {
+  static std::vector<KeEntry> c;
+  return c;
+}
+static std::mutex& ke_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+static int device_ke(int dtype, const double* ke576, hipStream_t st, const void** out) {
+  int dev = 0;
+  if (int r = hip_status(hipGetDevice(&dev), "hipGetDevice")) return r;
+  std::lock_guard<std::mutex> lock(ke_mutex());
+  for (const KeEntry& e : ke_cache())
+    if (e.device == dev && e.dtype == dtype && std::memcmp(e.ke, ke576, sizeof(e.ke)) == 0) {
+      *out = e.ptr;
+      return 0;
+    }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (int r = hip_status(hipStreamIsCapturing(st, &cap), "hipStreamIsCapturing")) return r;
+  if (cap != hipStreamCaptureStatusNone)
+    return fail(-2, "vh_dense_product: a new ke inside a graph capture (call once before capturing)");
+  KeEntry e{};
+  e.device = dev;
+  e.dtype = dtype;
+  std::memcpy(e.ke, ke576, sizeof(e.ke));
+  const size_t bytes = 576 * (dtype == F64 ? sizeof(double) : sizeof(float));
+  if (hipMalloc(&e.ptr, bytes) != hipSuccess) return fail(-1, "vh_dense_product: hipMalloc ke");
+  float k32[576];
+  for (int i = 0; i < 576; ++i) k32[i] = float(ke576[i]);
+  int r = hip_status(hipMemcpyAsync(e.ptr, dtype == F64 ? static_cast<const void*>(ke576) : k32, bytes,
+                                    hipMemcpyHostToDevice, st), "ke copy");
+  if (!r) r = hip_status(hipStreamSynchronize(st), "ke copy");       // the host arrays may go away
+  if (r) {
+    (void)hipFree(e.ptr);
+    return r;
+  }
+  ke_cache().push_back(e);
+  *out = e.ptr;
+  return 0;
+}
+
 extern "C" {
 
 int vh_abi_version(void) { return VH_ABI_VERSION; }
@@ -283,21 +335,13 @@ int vh_dense_product(int dtype, const double* ke576, const void* E, const void* 
   VH_REQUIRE(ke576 && E && u && y && u != y, "vh_dense_product: bad pointers");
   VH_REQUIRE(dims_ok(nx, ny, nz), "vh_dense_product: bad grid");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  // ke goes to the device through a small persistent buffer per dtype
-  static double* dk64 = nullptr;
-  static float* dk32 = nullptr;
-  if (dtype == F64) {
-    if (!dk64 && hipMalloc(&dk64, 576 * sizeof(double)) != hipSuccess) return fail(-1, "hipMalloc ke");
-    if (int r = hip_status(hipMemcpy(dk64, ke576, 576 * sizeof(double), hipMemcpyHostToDevice), "ke copy")) return r;
-    return dense_launch<double>(static_cast<const double*>(u), static_cast<const double*>(E), dk64,
+  const void* dk = nullptr;
+  if (int r = device_ke(dtype, ke576, st, &dk)) return r;
+  if (dtype == F64)
+    return dense_launch<double>(static_cast<const double*>(u), static_cast<const double*>(E), static_cast<const double*>(dk),
                                 static_cast<double*>(y), nx, ny, nz, st);
-  }
-  float k32[576];
-  for (int i = 0; i < 576; ++i) k32[i] = float(ke576[i]);
-  if (!dk32 && hipMalloc(&dk32, 576 * sizeof(float)) != hipSuccess) return fail(-1, "hipMalloc ke");
-  if (int r = hip_status(hipMemcpy(dk32, k32, sizeof(k32), hipMemcpyHostToDevice), "ke copy")) return r;
-  return dense_launch<float>(static_cast<const float*>(u), static_cast<const float*>(E), dk32, static_cast<float*>(y),
-                             nx, ny, nz, st);
+  return dense_launch<float>(static_cast<const float*>(u), static_cast<const float*>(E), static_cast<const float*>(dk),
+                             static_cast<float*>(y), nx, ny, nz, st);
 }
 
 

@@ -247,6 +247,38 @@ def test_full_size_product_properties():
     assert float(yl.abs().max()) < 1e-12 * float(rot.abs().max())
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_dense_control_is_cached_and_graph_capturable(dtype):
+    """The node-owned dense control keeps its ke on the device per distinct matrix: after one eager
+    call it replays inside a HIP graph (every replay equal to the eager product); a second ke is its
+    own entry; a new ke inside a capture is refused."""
+    from voxelhip import ops
+    shape = (12, 6, 10)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    ne, nd = int(np.prod(shape)), 3 * hx.n_nodes(shape)
+    E = (torch.rand(ne, generator=g, device="cuda", dtype=torch.float64) + 0.1).to(dtype)
+    u = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64).to(dtype)
+    ref = ops.dense_product(KE, E, u, torch.empty_like(u), shape).clone()
+    ref2 = ops.dense_product(2 * KE, E, u, torch.empty_like(u), shape).clone()
+    assert torch.allclose(ref2, 2 * ref, rtol=1e-12 if dtype == torch.float64 else 1e-5, atol=0)
+    y = torch.empty_like(u)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        ops.dense_product(KE, E, u, y, shape)
+    for _ in range(3):
+        y.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref)
+    del graph
+    with pytest.raises(Exception):
+        g2 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g2):
+            ops.dense_product(3.5 * KE, E, u, y, shape)
+    torch.cuda.synchronize()
+
+
 def test_product_and_fused_passes_are_reproducible_under_repetition():
     """Regression for LDS ordering: many launches of every epilogue must give bitwise-identical
     output (a missing barrier between the staged-plane reads and the next plane's writes shows
