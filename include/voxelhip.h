@@ -130,6 +130,13 @@ int vh_matvec_tiles(int dtype, int storage, const double* blocks72, int nx, int 
 int vh_dense_product(int dtype, const double* ke576, const void* E, const void* u, void* y, int nx, int ny, int nz,
                      void* stream);
 
+/* Element-per-lanes layout control: each element worked by 24 lanes (node x component) of a
+ * 32-lane half of a wave64, transforms by lane shuffles, y = K(E) u (unconstrained; y is zeroed
+ * first and the contributions are added with atomics, so the summation order is not fixed).
+ * Replaces: modal32 code/hex_modal_layout_v2/variants.py:95-117. */
+int vh_lanes24_product(int dtype, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny,
+                       int nz, void* stream);
+
 /* Jacobi diagonal (invert = 0) or inverse diagonal with 1 at fixed DOFs (invert = 1).
  * Replaces: jacobi_diag_fp64 cuda_operators.py:371-421 / OperatorSuite.diagonal :763-792;
  * NumpyProduct.diagonal src/voxel_mg.py:55-58 and Level.inv_diag :126-127. */

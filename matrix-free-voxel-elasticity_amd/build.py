@@ -20,7 +20,7 @@ OUT = PKG / "voxelhip" / "libvoxelhip.so"
 # matvec.hip is compiled through matvec_{f64,f32}.hip and matvec_batch_{f64,f32}.hip (one
 # translation unit each, built in parallel)
 SOURCES = ["capi.hip", "matvec_f64.hip", "matvec_f32.hip", "matvec_batch_f64.hip", "matvec_batch_f32.hip",
-           "grid_ops.hip", "blas.hip", "simp.hip", "batch.hip", "stencil.hip"]
+           "grid_ops.hip", "blas.hip", "simp.hip", "batch.hip", "stencil.hip", "lanes.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VOXELHIP_ARCH", "gfx950")
 

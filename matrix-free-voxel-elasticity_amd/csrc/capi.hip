@@ -11,6 +11,7 @@
 #include "common.h"
 #include "matvec.h"
 #include "stencil.h"
+#include "lanes.h"
 #include "grid_ops.h"
 #include "blas.h"
 #include "simp.h"
@@ -344,6 +345,19 @@ int vh_dense_product(int dtype, const double* ke576, const void* E, const void* 
                              static_cast<float*>(y), nx, ny, nz, st);
 }
 
+
+int vh_lanes24_product(int dtype, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny,
+                       int nz, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype), "vh_lanes24_product: dtype %d", dtype);
+  VH_REQUIRE(blocks72 && E && u && y && u != y, "vh_lanes24_product: bad pointers");
+  VH_REQUIRE(dims_ok(nx, ny, nz) && 32.0 * nx * ny * nz < 2147483648.0, "vh_lanes24_product: bad grid");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return lanes24_launch<double>(blocks72, static_cast<const double*>(E), static_cast<const double*>(u),
+                                  static_cast<double*>(y), nx, ny, nz, st);
+  return lanes24_launch<float>(blocks72, static_cast<const float*>(E), static_cast<const float*>(u),
+                               static_cast<float*>(y), nx, ny, nz, st);
+}
 
 int vh_diagonal(int dtype, const double* ke576, const void* E, void* out, int nx, int ny, int nz, int mask_kind,
                 const uint8_t* mask, int invert, void* stream) {

@@ -59,6 +59,7 @@ SIGNATURES = {
     "vh_matvec_fused": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Epilogue), _vp],
     "vh_matvec_tiles": [_i, _i, _vp, _i, _i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "vh_dense_product": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp],
+    "vh_lanes24_product": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp],
     "vh_diagonal": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp],
     "vh_element_energy": [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _d, _d, _vp],
     "vh_restrict": [_i, _vp, _vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],

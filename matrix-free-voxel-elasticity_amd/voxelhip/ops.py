@@ -175,6 +175,17 @@ def dense_product(ke, E, u, y, shape):
     return y
 
 
+def lanes24_product(blocks, E, u, y, shape):
+    """y = K(E) u with the element-per-24-lanes layout control (see include/voxelhip.h)."""
+    _dev(E, "E"); _dev(u, "u"); _dev(y, "y")
+    hb = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(72))
+    if u.dtype != E.dtype or y.dtype != E.dtype or u.numel() != y.numel() or u.numel() != 3 * n_nodes(shape):
+        raise ValueError("lanes24_product: dtypes or sizes do not match")
+    L.call("vh_lanes24_product", dtype_code(y), hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
+           *[int(s) for s in shape], _stream())
+    return y
+
+
 def diagonal(ke, E, out, shape, mask=None, invert=False):
     _dev(E, "E"); _dev(out, "out")
     k = np.ascontiguousarray(np.asarray(ke, dtype=np.float64).reshape(576))

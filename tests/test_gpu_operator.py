@@ -248,6 +248,27 @@ def test_full_size_product_properties():
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 3, 7), (16, 9, 12)])
+def test_lanes24_control_matches_the_product(dtype, shape):
+    """The element-per-24-lanes layout control (atomics, two elements per wave64) against the
+    plane-sweep product and the oracle."""
+    from voxelhip import ops
+    print("case", dtype, shape, flush=True)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    ne, nd = int(np.prod(shape)), 3 * hx.n_nodes(shape)
+    E = torch.rand(ne, generator=g, device="cuda", dtype=torch.float64) + 0.05
+    u = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64)
+    ref = hx.NumpyProduct(shape, KE).raw(u.cpu().numpy(), E.cpu().numpy())
+    blocks = hx.Modal(KE).blocks
+    y = torch.full((nd,), 7.0, dtype=dtype, device="cuda")          # not zero on entry
+    ops.lanes24_product(blocks, E.to(dtype), u.to(dtype), y, shape)
+    yp = ops.matvec(blocks, E.to(dtype), u.to(dtype), torch.empty_like(y), shape)
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    assert np.linalg.norm(y.double().cpu().numpy() - ref) / np.linalg.norm(ref) < tol
+    assert float((y - yp).double().norm() / yp.double().norm()) < tol
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_dense_control_is_cached_and_graph_capturable(dtype):
     """The node-owned dense control keeps its ke on the device per distinct matrix: after one eager
     call it replays inside a HIP graph (every replay equal to the eager product); a second ke is its
