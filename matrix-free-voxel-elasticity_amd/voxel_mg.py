@@ -255,8 +255,11 @@ class Multigrid:
 
     def __init__(self, xp, shapes, products, E_fine, mask_fine, coarse_factor, dtype,
                  pre=4, post=4, omegas=None, smoother="jacobi", lambdas=None,
-                 lower_frac=1.0 / 30.0, cheb_variant="saad", coarse_dtype=None):
+                 lower_frac=1.0 / 30.0, cheb_variant="saad", coarse_dtype=None, coarse_sweeps=None, coarse_from=2):
         self.xp, self.pre, self.post = xp, pre, post
+        # levels >= coarse_from smooth with coarse_sweeps steps instead of pre / post (the donor's
+        # separate coarse smoothing count, coarse_smooth_iters of multigrid_v4.py:466,934-967)
+        self.coarse_sweeps, self.coarse_from = coarse_sweeps, int(coarse_from)
         self.dtype = _tdt(dtype)
         dev = voxelhip.device()
         self.levels = []
@@ -372,6 +375,13 @@ class Multigrid:
             x, other = other, x
         return x
 
+    def _sweeps(self, level):
+        """(pre, post) smoothing steps of a level: `pre`/`post`, or `coarse_sweeps` (when set) on
+        the levels from `coarse_from` down."""
+        if self.coarse_sweeps is not None and level >= self.coarse_from:
+            return self.coarse_sweeps, self.coarse_sweeps
+        return self.pre, self.post
+
     def coarse_solve(self, r, out=None):
         y = out if out is not None else torch.empty_like(r)
         return ops.dense_mv(self.coarse_inverse, r, y, mask=self._coarse_mask, n=r.numel())
@@ -382,14 +392,15 @@ class Multigrid:
         Lv, w = self.levels[level], self._work[level]
         if level == len(self.levels) - 1:
             return self.coarse_solve(r, w["x"] if out is None else out)
-        x = self._smooth(level, w["x"], r, self.pre, True)
+        pre, post = self._sweeps(level)
+        x = self._smooth(level, w["x"], r, pre, True)
         Lv.matvec(x, w["res"], mode=L.MODE_RESID, b=r, fixed_zero=True)      # (r - A x) M
         nxt = self.levels[level + 1]
         rc = self._work[level + 1]["rhs"]
         ops.restrict(w["res"], rc, Lv.shape, cmask=nxt.maskobj)
         ec = self.vcycle(rc, level + 1)
         ops.prolong(ec, x, Lv.shape, mask=Lv.maskobj)                          # x += P ec M
-        return self._smooth(level, x, r, self.post, False, out=out, dots=dots)
+        return self._smooth(level, x, r, post, False, out=out, dots=dots)
 
     def __call__(self, r, out=None):
         w0 = self._work[0]

@@ -19,6 +19,9 @@ precision:
                   correction update x += |r| d, the dot accumulation and the coarsest solve;
                   the inner solve of A d = r / |r| stops at max(inner_tol, 0.5 tol / rel).
 
+coarse_degree / coarse_level: smoothing steps on the levels from coarse_level down (the donor's
+  coarse_smooth_iters, multigrid_v4.py:466,934-967); None keeps degree / pre / post there.
+
 coarse:
   "rediscretize"  every coarse level rediscretised (ke * 2^l with averaged moduli; src/voxel_mg.py)
   "galerkin"      rediscretised levels, the coarsest one the Galerkin P^T A P of the level above
@@ -40,7 +43,8 @@ import voxel_mg as mg
 class GMGSolver:
     def __init__(self, shape, ke, mask, E=None, *, max_coarse_x=16, n_levels=None, smoother="jacobi",
                  degree=3, pre=4, post=4, precision="fp64", coarse="rediscretize", graph=False,
-                 lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False):
+                 lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False,
+                 coarse_degree=None, coarse_level=2):
         self.shape = tuple(int(s) for s in shape)
         self.ke = np.asarray(ke, dtype=np.float64)
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -49,6 +53,8 @@ class GMGSolver:
             shapes = shapes[:n_levels]
         self.shapes = shapes
         self.smoother, self.degree, self.pre, self.post = smoother, degree, pre, post
+        # smoothing steps on the levels from coarse_level down (None: degree / pre / post there too)
+        self.coarse_degree, self.coarse_level = coarse_degree, coarse_level
         if precision not in ("fp64", "mixed", "ir"):
             raise ValueError(precision)
         self.precision = precision
@@ -90,7 +96,8 @@ class GMGSolver:
                                             post=self.post, lower_frac=self.lower_frac,
                                             omega_iterations=self.omega_iterations,
                                             coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin,
-                                            stencils=stencils)
+                                            stencils=stencils, coarse_degree=self.coarse_degree,
+                                            coarse_level=self.coarse_level)
         # the last post-smoothing step of the V-cycle forms the Polak-Ribiere dots itself (and
         # writes z in the V-cycle precision, which the DOT pass reads directly)
         self.fused_pr = (self.smoother == "chebyshev" and len(self.shapes) > 1 and self.degree >= 1
@@ -361,7 +368,7 @@ class GMGSolver:
 
 def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev", degree=3, pre=4, post=4,
                     lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None,
-                    stencils=None):
+                    stencils=None, coarse_degree=None, coarse_level=2):
     """The FP64 fine Level and the Multigrid (levels in pdtype, rediscretised ke * 2^l, 2x2x2
     modulus averaging, injected masks, explicit inverse of the coarsest operator, spectral
     omegas / lambdas) of one problem.  coarse_matrix: a precomputed coarsest operator (Galerkin);
@@ -387,6 +394,7 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
         kw = dict(smoother="chebyshev", pre=degree, post=degree, lower_frac=lower_frac)
     else:
         kw = dict(pre=pre, post=post)
+    kw.update(coarse_sweeps=coarse_degree, coarse_from=coarse_level)
     M = mg.Multigrid(None, shapes, prods, E64, mask, Lc, pdtype, coarse_dtype=coarse_dtype, **kw)
     # Jacobi: the reference's spectral omegas; Chebyshev: safe upper bounds of the spectrum
     if smoother == "chebyshev":

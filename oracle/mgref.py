@@ -236,12 +236,16 @@ class Multigrid:
         L = self.levels[level]
         if level == len(self.levels) - 1:
             return self.coarse_solve(r) * L.mask
-        x = self._smooth(level, None, r, self.pre, True)
+        pre, post = self.pre, self.post
+        cs = getattr(self, "coarse_sweeps", None)    # the donor's coarse smoothing count
+        if cs is not None and level >= getattr(self, "coarse_from", 2):
+            pre = post = cs
+        x = self._smooth(level, None, r, pre, True)
         res = (r - L.apply(x)) * L.mask
         rc = restrict(res, L.shape) * self.levels[level + 1].mask
         ec = self.vcycle(rc, level + 1)
         x = x + prolong(ec, self.levels[level + 1].shape) * L.mask
-        x = self._smooth(level, x, r, self.post, False)
+        x = self._smooth(level, x, r, post, False)
         return x
 
     def __call__(self, r):

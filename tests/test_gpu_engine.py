@@ -136,3 +136,18 @@ def test_iteration_limit_and_queued_iterations(problem, precision):
     y, it3, ok3, r3, _ = s.solve(f, tol=1e-3, maxiter=it2)
     assert ok3 and it3 == it2 and torch.equal(y, xl) and r3 == r2
     assert rel(x_full, xo) < (1e-9 if precision == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_coarse_degree_keeps_the_solution(problem, precision):
+    """Degree 2 on the levels from 2 down: the same converged solution within the tolerance."""
+    from voxelhip.engine import GMGSolver
+    shape, E, mask, force, xo, ito = problem
+    print("case coarse_degree", precision, flush=True)
+    s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=2, smoother="chebyshev", degree=3,
+                  precision=precision, coarse_degree=2, coarse_level=2, graph=True)
+    assert len(s.shapes) >= 3 and s.M.coarse_sweeps == 2
+    x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
+    assert ok and r <= 1e-8 and it <= ito + 2
+    assert rel(x, xo) < 1e-7
+

@@ -115,6 +115,25 @@ def test_fixed_iterations_work_accounting(golden, KE):
     assert it == 7 and mv == 8
 
 
+def test_coarse_smoothing_degree_matches_the_oracle(golden, KE):
+    """Fewer smoothing steps on the levels from coarse_from down (the donor's coarse_smooth_iters):
+    one V-cycle against the oracle with the same per-level counts."""
+    g = np.load(golden / "mg_golden.npz")
+    shape = tuple(int(s) for s in g["q8s3/shape"])
+    E, mask = g["q8s3/E"], g["q8s3/mask"]
+    _, M, lam = build(KE, shape, E, mask, smoother="chebyshev", degree=3)
+    _, Mo = mg_ref.build_hierarchy(shape, KE, E, mask, max_coarse_x=2, smoother="chebyshev", degree=3)
+    assert len(M.levels) >= 3
+    r = g["q8s3/vcycle_in"]
+    base = M(torch.from_numpy(r).cuda()).cpu().numpy()
+    for cf, cs in ((1, 2), (2, 1)):
+        M.coarse_sweeps, M.coarse_from = cs, cf
+        Mo.coarse_sweeps, Mo.coarse_from = cs, cf
+        y = M(torch.from_numpy(r).cuda())
+        assert rel(y, Mo(r)) < 1e-10
+        assert rel(y, base) > 1e-6                     # the count does change the cycle
+
+
 def test_chebyshev_vcycle_matches_the_oracle(golden, KE):
     g = np.load(golden / "mg_golden.npz")
     shape = tuple(int(s) for s in g["q8s3/shape"])
