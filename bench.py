@@ -7,8 +7,8 @@ Workload (one "step" = one complete solve from x = 0 to the FP64 true relative r
   Hex8 voxel cantilever 256 x 128 x 128 elements (12,830,211 DOFs), E0 = 1, Emin = 1e-9,
   SIMP p = 3, densities iid U[0.01, 1] (seed 3) smoothed by the radius-1.5 cone filter, x = 0
   face clamped, unit downward line load on the x = nx, z = nz/2 edge; GMG-preconditioned
-  flexible CG (5 levels, 2x coarsening, Chebyshev-Jacobi degree 3, degree 2 on the levels from
-  the third down as the donor's coarse_smooth_iters, dense coarsest), mixed
+  flexible CG (5 levels, 2x coarsening, Chebyshev-Jacobi degree 3 on [lambda_max / 15, lambda_max],
+  degree 2 on the levels from the third down as the donor's coarse_smooth_iters, dense coarsest), mixed
   precision: FP32 V-cycle inside the FP64 outer CG with FP64 true residual and FP64 dot
   accumulation.  The plain FP64 solve of the same system is timed beside it.
 
@@ -157,8 +157,7 @@ def simp_leg(ke, shape=(384, 192, 192), steps=100, seed=5):
     from voxelhip.simp import SimpProblem, simp_run
     _, _, force, mask, _ = problem(shape, seed=seed, rmin=0)
     t0 = time.perf_counter()
-    pb = SimpProblem(shape, ke, mask, force, precision="mixed", tol=1e-8, max_coarse_x=16, degree=3,
-                     coarse_degree=2, coarse_level=2)
+    pb = SimpProblem(shape, ke, mask, force, precision="mixed", tol=1e-8, max_coarse_x=16, degree=3)
     hist, rho = simp_run(pb, steps=steps, volfrac=0.3, rmin=1.5, move=0.2)
     total = time.perf_counter() - t0
     its = [h["iterations"] for h in hist]
@@ -298,7 +297,7 @@ def main():
     precs = ("mixed", "fp64") if args.precision == "both" else (args.precision,)
     for prec in precs:
         s = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision=prec, graph=bool(args.graph),
-                      tune=tuner, coarse_degree=2, coarse_level=2)
+                      tune=tuner, lower_frac=1.0 / 15.0, coarse_degree=2, coarse_level=2)
         solvers[prec] = s
         dt, st = timed_solves(s, force, args.steps, args.warmup)
         its = sorted({x[0] for x in st})
@@ -361,8 +360,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 V-cycle in f64 outer CG",
         "data": "synthetic (seeded densities; no dataset)",
-        "config": {"workload": "cantilever 256x128x128 (12,830,211 DOF), GMG-FCG 5 levels Chebyshev deg 3 "
-                               "(deg 2 on levels 2-3), FP32 V-cycle in FP64 outer CG, rtol 1e-8 (FP64 true residual)",
+        "config": {"workload": "cantilever 256x128x128 (12,830,211 DOF), GMG-FCG 5 levels Chebyshev deg 3 on "
+                               "[lmax/15, lmax] (deg 2 on levels 2-3), FP32 V-cycle in FP64 outer CG, rtol 1e-8 "
+                               "(FP64 true residual)",
                    "ndof": ndof, "parallelism": "single GPU"},
         "fp64_solves_per_s": out["fp64"]["solves_per_s"],
         "mixed_speedup_vs_fp64": out["fp64"]["seconds_per_solve"] / out["mixed"]["seconds_per_solve"],
