@@ -1,6 +1,6 @@
 """Writes the golden fixtures under tests/golden/ from the reference checkout at /root/reference.
 
-Run in the build container only (the GPU box has no reference):  python tests/golden/make_golden.py
+Run only where that checkout exists (the tests read the committed fixtures):  python tests/golden/make_golden.py
 
 1. Data files the reference itself holds (copied byte-for-byte as data, no code):
    smoke-q4.npz, optimized-q64.npz            experiments/session_a/inputs/

@@ -1,5 +1,5 @@
 """Writes tests/golden/session_i_known_answers.json and copies optimized-q96.npz from the reference
-checkout at /root/reference (build container only; the GPU box has no reference).
+checkout at /root/reference (only where that checkout exists; the tests read the committed fixtures).
 
     python tests/golden/make_session_i.py
 

@@ -1,9 +1,11 @@
 """GPU parity of the operator kernels against the oracle, through the C-ABI.
 
-Tolerances: FP64 relative L2 <= 1e-12 (the bar of the north-star contract; the reference's own
+Tolerances: FP64 relative L2 <= 1e-12 (the parity bar of this operator; the reference's own
 gate is 1e-11, experiments/session_a/protocol.json:127-133); FP32 <= 1e-5 (same gate).
 Index/mask work (fixed DOFs) is checked exactly.
 """
+import warnings
+
 import numpy as np
 import pytest
 import torch
@@ -293,10 +295,12 @@ def test_dense_control_is_cached_and_graph_capturable(dtype):
         torch.cuda.synchronize()
         assert torch.equal(y, ref)
     del graph
-    with pytest.raises(Exception):
-        g2 = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g2):
-            ops.dense_product(3.5 * KE, E, u, y, shape)
+    with warnings.catch_warnings():
+        warnings.filterwarnings("ignore", message="The CUDA Graph is empty")   # the refused capture records nothing
+        with pytest.raises(Exception):
+            g2 = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g2):
+                ops.dense_product(3.5 * KE, E, u, y, shape)
     torch.cuda.synchronize()
 
 
