@@ -45,8 +45,7 @@ static const int kNativeLane[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 
 static int modal_blocks(const double* ke, double tol, double* blocks, double* discarded) {
   // full = T ke T^T / 64 with T[3p+c][3n+c] = (-1)^popcount(p & lane(n))
-  static double full[24][24];
-  double tmp[24][24];
+  double full[24][24], tmp[24][24];
   auto h = [](int p, int l) { return (__builtin_popcount(p & l) & 1) ? -1.0 : 1.0; };
   for (int i = 0; i < 24; ++i)      // tmp = T ke
     for (int j = 0; j < 24; ++j) {

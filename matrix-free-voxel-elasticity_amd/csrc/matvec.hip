@@ -487,29 +487,28 @@ using Tile4 = Tile<4, 8>;
 template <typename T>
 using TileWide = std::conditional_t<sizeof(T) == 8, Tile<22, 32>, Tile<30, 32>>;
 
+// (the cached values are initialised once, thread-safely, on the first call)
 static int device_cus() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
+  static const int cus = [] {
+    int dev = 0, n = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-      cus = prop.multiProcessorCount;
-    if (cus <= 0) cus = 256;
-  }
+      n = prop.multiProcessorCount;
+    return n > 0 ? n : 256;
+  }();
   return cus;
 }
 
 // Resident workgroups per CU of one kernel instantiation (cached).
 template <typename T, typename S, typename TL, bool CUBE>
 static int resident_per_cu() {
-  static int occ = 0;
-  if (occ == 0) {
+  static const int occ = [] {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch<T, S, MODE_RAW, MASK_NONE, TL, CUBE>,
                                                     TL::BLOCK, 0) != hipSuccess || n <= 0)
       n = 1;
-    occ = n;
-  }
+    return n;
+  }();
   return occ;
 }
 
@@ -739,14 +738,13 @@ int matvec_batch_launch(const MatvecBatch& bt, hipStream_t st) {
 // units)
 template <typename T, typename TL>
 static int resident_batch() {
-  static int occ = 0;
-  if (occ == 0) {
+  static const int occ = [] {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch_batch<T, T, MODE_RAW, MASK_X0, TL, true, LOAD_NONE>,
                                                     TL::BLOCK, 0) != hipSuccess || n <= 0)
       n = 1;
-    occ = n;
-  }
+    return n;
+  }();
   return occ;
 }
 

@@ -37,6 +37,22 @@ def test_modal_blocks_match_the_oracle():
         assert disc == pytest.approx(ref.discarded_relative, abs=1e-16)
 
 
+def test_modal_blocks_are_safe_from_concurrent_threads():
+    """ctypes releases the GIL around the call: eight threads factorising different bricks at once
+    must each get exactly their single-threaded result (no shared scratch in the library)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import hexref as hx
+    from voxelhip import ops
+    kes = [hx.quadrature([1.0, 1.0 + 0.25 * i, 2.0 - 0.125 * i], 0.2 + 0.02 * i) for i in range(8)]
+    want = [ops.modal_blocks(ke)[0] for ke in kes]
+
+    def run(i):
+        return all(np.array_equal(ops.modal_blocks(kes[i])[0], want[i]) for _ in range(300))
+
+    with ThreadPoolExecutor(8) as ex:
+        assert all(ex.map(run, range(8)))
+
+
 def test_modal_blocks_reject_a_non_brick():
     from voxelhip import ops
     a = np.random.default_rng(1).standard_normal((24, 24))
