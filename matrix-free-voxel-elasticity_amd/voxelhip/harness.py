@@ -233,13 +233,14 @@ def solve_case(data, case="", device=None):
     force = torch.from_numpy(f_h).to(dev)
     shapes = mg.hierarchy_shapes(shape, p["multigrid"]["max_coarse_x"])
     prods = [mg.HipProduct(s, ke * 2 ** i, np.float64) for i, s in enumerate(shapes)]
-    Ec, mc = E_h, m_h
+    # coarsest operator: moduli averaged and mask injected level by level, assembled and factored
+    # on the device (the reference assembles on the host and calls np.linalg.cholesky)
+    Ec, mc = E, mask
     for i in range(1, len(shapes)):
-        Ec = np.asarray(Ec).reshape(shapes[i - 1][0] // 2, 2, shapes[i - 1][1] // 2, 2,
-                                    shapes[i - 1][2] // 2, 2).mean(axis=(1, 3, 5)).ravel()
-        mc = np.asarray(mc).reshape(shapes[i - 1][0] + 1, shapes[i - 1][1] + 1, shapes[i - 1][2] + 1,
-                                    3)[::2, ::2, ::2, :].ravel()
-    factor = np.linalg.cholesky(mg.assemble_dense(shapes[-1], ke * 2 ** (len(shapes) - 1), Ec, mc))
+        Ec = mg.coarsen_modulus(None, Ec, shapes[i - 1])
+        mc = mg.coarsen_mask(None, mc, shapes[i - 1])
+    kc = torch.from_numpy(np.ascontiguousarray(ke * 2 ** (len(shapes) - 1))).to(dev)
+    factor = torch.linalg.cholesky(ops.assemble_dense(kc, Ec, shapes[-1], mask=ops.Mask.from_values(mc, shapes[-1])))
     xp = voxelhip.xp
     M = mg.Multigrid(xp, shapes, prods, E, mask, factor, np.float64, pre=p["multigrid"]["pre"],
                      post=p["multigrid"]["post"])
