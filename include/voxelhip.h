@@ -89,7 +89,9 @@ typedef struct vh_epilogue {
   int tile; /* (y, z) tile variant: 0 = automatic, 1 = 16 x 32 nodes, 2 = 8 x 32 nodes, 3 = 4 x 8 nodes (one
              * wave), 4 = wide (one workgroup per CU: 22 x 32 nodes in FP64, 30 x 32 in FP32) */
   double kdiag[3];
-  int flags; /* VH_PREV_ZERO: CHEB3's previous iterate is zero (y is then not read) */
+  int flags; /* VH_PREV_ZERO: CHEB3's previous iterate is zero (y is then not read);
+             * VH_W_F32: w is stored in FP32 (FP64 RESID pass with the load-time combination only:
+             * the mixed-precision Krylov loop forms its search direction in FP32) */
   /* Load-time combination (voxel-cube blocks), the combined input written to u_out at every
    * DOF:  DOT / RESID read u + s w with s = sdev[sidx] (device FP64 scalar);  CHEB3 reads
    * load_scale * (u (.) w) (elementwise product: with u = b, w = dinv and load_scale = 1/theta
@@ -109,6 +111,7 @@ typedef struct vh_epilogue {
   const void* zold;
 } vh_epilogue;
 #define VH_PREV_ZERO 1
+#define VH_W_F32 2
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
                     int nx, int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep,
                     void* stream);

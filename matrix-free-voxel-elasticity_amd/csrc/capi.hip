@@ -315,6 +315,9 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
                           ep->u_out != ep->b),
                "vh_matvec: the load-time combination needs a distinct u_out");
     VH_REQUIRE(!ep->y32 || m == MODE_RESID, "vh_matvec: y32 is an output of the RESID pass");
+    VH_REQUIRE((ep->flags & ~(MV_PREV_ZERO | MV_W_F32)) == 0, "vh_matvec: unknown flags %d", ep->flags);
+    VH_REQUIRE(!(ep->flags & MV_W_F32) || (dtype == F64 && storage == F64 && m == MODE_RESID && ep->w),
+               "vh_matvec: an FP32 w is read by the FP64 RESID pass with the load-time combination only");
     VH_REQUIRE(!y || (ep->b != y && ep->dinv != y), "vh_matvec: y must not alias an input");
     if (smoother && ep->dinv == nullptr)   // D^-1 from the moduli and ke's diagonal
       VH_REQUIRE(ep->kdiag[0] > 0 && ep->kdiag[1] > 0 && ep->kdiag[2] > 0, "vh_matvec: kdiag missing");

@@ -16,8 +16,11 @@ struct Tile {
   static constexpr int LOADS = (NY * 3 * NZ + BLOCK - 1) / BLOCK;
 };
 constexpr int MV_PREV_ZERO = 1;
-// load-time operations on the input (template parameter LOADOP of the product kernel)
-constexpr int LOAD_NONE = 0, LOAD_AXPY = 1, LOAD_MUL = 2;
+constexpr int MV_W_F32 = 2;       // the AXPY operand w of an FP64 pass is stored in FP32
+// load-time operations on the input (template parameter LOADOP of the product kernel); AXPY32 is
+// AXPY with an FP32 w in an FP64 pass (the RESID pass of the mixed-precision Krylov loop, whose
+// search direction is formed in FP32)
+constexpr int LOAD_NONE = 0, LOAD_AXPY = 1, LOAD_MUL = 2, LOAD_AXPY32 = 3;
 // tile variants (owned nodes in y x z; the y extent is an upper bound, the plan spreads ny evenly
 // over the y tiles): 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads), 3 = 4 x 8
 // (64 threads: one wave per workgroup, for the latency-bound coarse levels), 4 = the wide tile that
@@ -62,7 +65,7 @@ struct MatvecParams {
   int lx;                 // x chunk length (0: automatic)
   int tile;               // tile variant (MV_TILE_*)
   int cube;               // coef holds the 8 cube-pattern values instead of 72 block entries
-  int flags;              // MV_PREV_ZERO: CHEB3's previous iterate is zero (y not read)
+  int flags;              // MV_PREV_ZERO: CHEB3's previous iterate is zero (y not read); MV_W_F32
   int fixed_zero;         // with a mask: the output is 0 at fixed DOFs (M K M u, (b - A u) M)
   int tiles_y, tiles_z, chunks_x, n_tiles;
   int tile_ty;            // owned node rows per y tile (at most the tile variant's TY)

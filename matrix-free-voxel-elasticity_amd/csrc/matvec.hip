@@ -163,9 +163,12 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 
   const S* __restrict__ u = reinterpret_cast<const S*>(a.u);
   const T* __restrict__ wv = reinterpret_cast<const T*>(a.w);     // combination operand, compute type
+  const float* __restrict__ wf = reinterpret_cast<const float*>(a.w);   // (AXPY32: stored in FP32)
   const T* __restrict__ E = a.E;
   constexpr bool COMBINE = LOADOP != LOAD_NONE;
-  const T sw = LOADOP == LOAD_AXPY ? T(a.sdev[a.sidx]) : (LOADOP == LOAD_MUL ? a.load_scale : T(0));
+  constexpr bool AXPY = LOADOP == LOAD_AXPY || LOADOP == LOAD_AXPY32;
+  auto w_at = [&](size_t dof) -> T { return LOADOP == LOAD_AXPY32 ? T(wf[dof]) : wv[dof]; };
+  const T sw = AXPY ? T(a.sdev[a.sidx]) : (LOADOP == LOAD_MUL ? a.load_scale : T(0));
 
   // ---- loader geometry (plane independent): this thread's LOADS slots of the flat plane
   // image (NY rows of 3*NZ contiguous values), their in-plane DOF offsets and validity.
@@ -206,7 +209,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int j = 0; j < LOADS; ++j) {
       const size_t dof = base + ld_off[j];
       pre[j] = load_as<S, T>(u, dof);
-      if (COMBINE) pre_w[j] = wv[dof];
+      if (COMBINE) pre_w[j] = w_at(dof);
       if (MASK == MASK_ARRAY) pre_m[j] = a.mask[dof];
       keep |= unsigned(plane_ok && ((ld_okbits >> j) & 1u)) << j;
     }
@@ -226,7 +229,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int j = 0; j < LOADS; ++j) {
       bool keep = (pre_keep >> j) & 1u;
       if (MASK == MASK_ARRAY) keep = keep && pre_m[j] != 0;
-      const T v = LOADOP == LOAD_AXPY ? pre[j] + sw * pre_w[j]
+      const T v = AXPY ? pre[j] + sw * pre_w[j]
                   : (LOADOP == LOAD_MUL ? sw * (pre[j] * pre_w[j]) : pre[j]);
       if (tid + j * BLOCK < NLOAD) flat[tid + j * BLOCK] = keep ? v : T(0);
     }
@@ -290,7 +293,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     for (int c = 0; c < 3; ++c) {
       if (NEED_U && MASK == MASK_ARRAY) {       // the staged plane is masked: fetch the input
         ep_u[c] = load_as<S, T>(u, 3 * node + c);
-        if (LOADOP == LOAD_AXPY) ep_u[c] += sw * wv[3 * node + c];
+        if (AXPY) ep_u[c] += sw * w_at(3 * node + c);
         if (LOADOP == LOAD_MUL) ep_u[c] = sw * (ep_u[c] * wv[3 * node + c]);
       }
       if (NEED_B) ep_b[c] = a.b[3 * node + c];
@@ -616,6 +619,13 @@ static void launch_t(const MatvecParams<T>& p, int grid, hipStream_t st) {
   constexpr int LOP = (MODE == MODE_DOT || MODE == MODE_RESID) ? LOAD_AXPY : (MODE == MODE_CHEB3 ? LOAD_MUL : LOAD_NONE);
   if constexpr (LOP != LOAD_NONE) {
     if (p.w) {
+      // an FP32 w in the FP64 RESID pass (checked by the caller)
+      if constexpr (sizeof(T) == 8 && MODE == MODE_RESID) {
+        if (p.flags & MV_W_F32) {
+          launch_tiles<T, S, MODE, MASK, true, LOAD_AXPY32>(p, grid, st);
+          return;
+        }
+      }
       launch_tiles<T, S, MODE, MASK, true, LOP>(p, grid, st);
       return;
     }

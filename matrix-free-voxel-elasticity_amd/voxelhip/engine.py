@@ -12,7 +12,12 @@ every iteration) and replayed; the host only reads the residual norm at the chec
 precision:
   "fp64"          FP64 everywhere (the reference's plain FP64 GMG-FCG path)
   "mixed"         FP64 outer flexible CG and true residual, FP32 V-cycle (matvec, smoothing,
-                  transfers, coarsest solve in FP32), FP64 dot accumulation
+                  transfers, coarsest solve in FP32), FP64 dot accumulation.  With
+                  fp32_direction (the default) the search direction p = z + beta p_old and the
+                  product p.Ap that gives alpha are formed in FP32 as well (z is already an FP32
+                  V-cycle output); the iterate x, the residual b - A x formed every iteration and
+                  the convergence test stay FP64, so an FP32 rounding of p or alpha only changes
+                  the step taken, never the residual it is judged by.
   "ir"            FP64 iterative refinement around an FP32 flexible CG with the FP32 V-cycle
                   (the scheme of pcg_ir, experiments/session_a/donor/src/gpu_fem/simp_r2.py:298-385,
                   with GMG in place of Jacobi): FP64 only for the outer residual b - A x, the
@@ -44,7 +49,7 @@ class GMGSolver:
     def __init__(self, shape, ke, mask, E=None, *, max_coarse_x=16, n_levels=None, smoother="jacobi",
                  degree=3, pre=4, post=4, precision="fp64", coarse="rediscretize", graph=False,
                  lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False,
-                 coarse_degree=None, coarse_level=2):
+                 coarse_degree=None, coarse_level=2, fp32_direction=True):
         self.shape = tuple(int(s) for s in shape)
         self.ke = np.asarray(ke, dtype=np.float64)
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -58,6 +63,7 @@ class GMGSolver:
         if precision not in ("fp64", "mixed", "ir"):
             raise ValueError(precision)
         self.precision = precision
+        self.fp32_direction = bool(fp32_direction)
         self.pdtype = torch.float64 if precision == "fp64" else torch.float32
         self.inner_tol = inner_tol
         # the FP64 coarsest solve inside the FP32 V-cycle of the refinement scheme
@@ -102,11 +108,14 @@ class GMGSolver:
         # writes z in the V-cycle precision, which the DOT pass reads directly)
         self.fused_pr = (self.smoother == "chebyshev" and len(self.shapes) > 1 and self.degree >= 1
                          and self.precision != "ir")
+        # FP32 search direction (mixed, with z in FP32 from the fused V-cycle)
+        self.p32 = self.fp32_direction and self.precision == "mixed" and self.fused_pr
         if self.tuner is not None:
             self.M.tune(self.tuner)
-            self.fine.tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True), (L.MODE_RESID, False)])
+            self.fine.tune(self.tuner, ([] if self.p32 else [(L.MODE_DOT, True)]) +
+                           [(L.MODE_RESID, True), (L.MODE_RESID, False)])
             if self.fused_pr:
-                self.M.levels[0].tune(self.tuner, [(L.MODE_CHEB3D, False)])
+                self.M.levels[0].tune(self.tuner, [(L.MODE_CHEB3D, False)] + ([(L.MODE_DOT, True)] if self.p32 else []))
             if self.precision == "ir":
                 self.M.levels[0].tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True)])
         self._alloc()
@@ -129,7 +138,8 @@ class GMGSolver:
         self.parts = ops.reduce_parts(n)
         # per-tile partial counts of the fused passes (they depend on each pass's launch plan)
         f, f32 = self.fine, self.M.levels[0]
-        self.t_dot, self.t_res = f.tiles(L.MODE_DOT, True), f.tiles(L.MODE_RESID, True)
+        self.t_dot = (f32 if self.p32 else f).tiles(L.MODE_DOT, True)
+        self.t_res = f.tiles(L.MODE_RESID, True)
         self.t_res0 = f.tiles(L.MODE_RESID, False)
         if self.precision == "ir":
             self.t_dot32, self.t_res32 = f32.tiles(L.MODE_DOT, True), f32.tiles(L.MODE_RESID, True)
@@ -149,21 +159,24 @@ class GMGSolver:
             self.ds, self.ps, self.zs = [z(f32), z(f32)], [z(f32), z(f32)], [z(f32), z(f32)]
         else:
             zdt = self.pdtype if self.fused_pr else torch.float64
-            self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(zdt), z(zdt)]
+            pdt = torch.float32 if self.p32 else torch.float64
+            self.xs, self.ps, self.zs = [z(), z()], [z(pdt), z(pdt)], [z(zdt), z(zdt)]
 
     # ------------------------------------------------------------------ one iteration
     def _iteration(self, i):
         """Launch flexible-CG iteration with buffer parity i (no host synchronisation).
 
         DOT pass:   p_i = z_i + beta p_o (read on the fly), p_i.A p_i (A p_i not stored) -> alpha
-        RESID pass: x_i = x_o + alpha p_i (on the fly), r = b - A x_i, |r|^2, r -> fp32
+                    (FP32 with fp32_direction, FP64 otherwise)
+        RESID pass: x_i = x_o + alpha p_i (on the fly), r = b - A x_i, |r|^2, r -> fp32 (FP64)
         V-cycle on r into z_o; its last smoothing step forms r.z_o and r.(z_o - z_i) -> beta, r.z
         (z in the V-cycle precision; the Jacobi smoother converts and dots in a separate pass)
         """
         o = 1 - i
         f, L_ = self.fine, L
-        f.matvec(self.zs[i], None, mode=L_.MODE_DOT, partial=self.partial,
-                 w=self.ps[o], sdev=self.scal, sidx=L_.S_BETA, u_out=self.ps[i])
+        (self.M.levels[0] if self.p32 else f).matvec(self.zs[i], None, mode=L_.MODE_DOT, partial=self.partial,
+                                                     w=self.ps[o], sdev=self.scal, sidx=L_.S_BETA,
+                                                     u_out=self.ps[i])
         ops.finish(self.partial, self.t_dot, 1, self.scal, L_.S_PQ, L_.OP_ALPHA)
         # (with the fused dots and an FP32 V-cycle nothing reads the FP64 residual: only its FP32
         #  copy, the V-cycle right-hand side, is written)

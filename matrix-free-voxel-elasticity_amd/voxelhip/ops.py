@@ -122,7 +122,8 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
            kdiag=None, prev_zero=False, w=None, sdev=None, sidx=0, u_out=None, y32=None, load_scale=1.0,
            prev=None, zold=None):
     """One pass of the parity product with a fused epilogue (see include/voxelhip.h).  An FP32
-    u with FP64 moduli selects the FP32-input DOT pass of the mixed-precision Krylov loop."""
+    u with FP64 moduli selects the FP32-input DOT pass of the mixed-precision Krylov loop; an
+    FP32 w with FP64 moduli, the RESID pass that adds an FP32 search direction to the FP64 x."""
     _dev(E, "E"); _dev(u, "u")
     if y is None and not (mode == L.MODE_DOT or (mode == L.MODE_RESID and y32 is not None)):
         raise ValueError("only a DOT pass, or a RESID pass writing y32, may omit y")
@@ -148,8 +149,11 @@ def matvec(blocks, E, u, y, shape, mask=None, mode=L.MODE_RAW, b=None, dinv=None
     kd = (ctypes.c_double * 3)(*([0.0] * 3 if kdiag is None else [float(v) for v in kdiag]))
     if mode in (L.MODE_JACOBI, L.MODE_CHEB, L.MODE_CHEB3, L.MODE_CHEB3D) and dinv is None and kdiag is None:
         raise ValueError("smoothing passes need dinv or kdiag")
+    flags = L.PREV_ZERO if prev_zero else 0
+    if w is not None and w.dtype == torch.float32 and E.dtype == torch.float64:
+        flags |= L.W_F32                # FP32 search direction read by the FP64 RESID pass
     ep = L.Epilogue(mode, int(bool(fixed_zero)), _p(b), _p(dinv), _p(d), _p(partial),
-                    float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, int(bool(prev_zero)),
+                    float(omega), float(cheb_a), float(cheb_c), int(lx), int(tile), kd, flags,
                     _p(w), _p(sdev), int(sidx), _p(u_out), _p(y32), float(load_scale), _p(prev), _p(zold))
     L.call("vh_matvec_fused", dt, st, hb.ctypes.data_as(ctypes.c_void_p), _p(E), _p(u), _p(y),
            nx, ny, nz, m.kind, m.ptr, ctypes.byref(ep), _stream())

@@ -63,12 +63,14 @@ def test_graph_replay_equals_eager(problem):
     assert torch.equal(sols[0], sols[1])
 
 
-@pytest.mark.parametrize("precision", ["mixed", "ir"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed-p64", "ir"])
 def test_mixed_precision_reaches_fp64_tolerance(problem, precision):
+    """mixed: FP32 search direction (the default); mixed-p64: the direction and p.Ap in FP64."""
     from voxelhip.engine import GMGSolver
     shape, E, mask, force, xo, ito = problem
     s = GMGSolver(shape, KE, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, smoother="chebyshev", degree=3,
-                  precision=precision, graph=True)
+                  precision=precision.split("-")[0], graph=True, fp32_direction=precision == "mixed")
+    assert s.p32 == (precision == "mixed")
     print("case", precision, flush=True)
     x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
     assert ok and r <= 1e-8, (it, r)
@@ -78,6 +80,8 @@ def test_mixed_precision_reaches_fp64_tolerance(problem, precision):
     xn = x.cpu().numpy()
     res = force - (fine.raw(xn * mask, E) * mask + xn * (1 - mask))
     assert np.linalg.norm(res) / np.linalg.norm(force) <= 1.01e-8
+    if precision.startswith("mixed"):
+        assert it <= ito + 1, (it, ito)
 
 
 def test_refinement_graph_replay_equals_eager(problem):

@@ -375,6 +375,48 @@ def test_fp32_product_takes_the_cube_path_and_matches_fp64_blocks():
     assert rel(y, hx.NumpyProduct(shape, KE).raw(u, E)) < 1e-5
 
 
+@pytest.mark.parametrize("kind", ["none", "x0", "array"])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+def test_resid_pass_reads_an_fp32_direction(kind, tile):
+    """FP64 RESID pass with the AXPY load reading an FP32 w (the mixed-precision Krylov loop's
+    search direction): bitwise the pass fed the FP64 image of the same values, and the oracle's
+    x + alpha p and b - A x; an FP32 w anywhere else is refused."""
+    from voxelhip import _lib as L, ops
+    shape = (20, 18, 36)
+    rng = np.random.default_rng(31)
+    E, x_old = case(shape, 12)
+    m = np.ones(3 * hx.n_nodes(shape)) if kind == "none" else _masks(shape, rng)[kind]
+    lvl = mg_ref.Level(shape, hx.NumpyProduct(shape, KE), E, m, np.float64)
+    p32 = rng.standard_normal(x_old.size).astype(np.float32)
+    b = rng.standard_normal(x_old.size)
+    alpha = -0.8125
+    scal = torch.zeros(16, dtype=torch.float64, device="cuda")
+    scal[L.S_ALPHA] = alpha
+    mask = ops.Mask.from_values(m, shape)
+    blocks = hx.Modal(KE).blocks
+    Ed, xd, bd = dev(E), dev(x_old), dev(b)
+    nt, _ = ops.matvec_tiles(shape, blocks=blocks, tile=tile)
+    outs = []
+    print("case", kind, tile, flush=True)
+    for w in (dev(p32, torch.float32), dev(p32.astype(np.float64))):
+        part = torch.zeros(nt, dtype=torch.float64, device="cuda")
+        r, xn = torch.empty_like(xd), torch.empty_like(xd)
+        r32 = torch.empty(x_old.size, dtype=torch.float32, device="cuda")
+        ops.matvec(blocks, Ed, xd, r, shape, mask=mask, mode=L.MODE_RESID, b=bd, partial=part, w=w, sdev=scal,
+                   sidx=L.S_ALPHA, u_out=xn, y32=r32, tile=tile)
+        outs.append((r, xn, r32, part))
+    for a, c in zip(outs[0], outs[1]):
+        assert torch.equal(a, c)
+    r, xn, r32, part = outs[0]
+    x_ref = x_old + alpha * p32.astype(np.float64)
+    r_ref = b - lvl.apply(x_ref)
+    assert rel(xn, x_ref) < 1e-15 and rel(r, r_ref) < 1e-12
+    assert float(part.sum()) == pytest.approx(float(r_ref @ r_ref), rel=1e-12)
+    with pytest.raises(RuntimeError, match="FP32 w"):   # an FP32 w in the FP64 DOT pass
+        ops.matvec(blocks, Ed, xd, None, shape, mask=mask, mode=L.MODE_DOT, partial=part,
+                   w=dev(p32, torch.float32), sdev=scal, sidx=L.S_BETA, u_out=torch.empty_like(xd))
+
+
 def test_fused_krylov_passes():
     """DOT pass with p = z + beta p_old formed on load, RESID pass with x = x_old + alpha p formed
     on load (+ FP32 copy and |r|^2 partials), and the convert + Polak-Ribiere-dots kernel."""
