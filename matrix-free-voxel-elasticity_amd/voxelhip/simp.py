@@ -139,7 +139,7 @@ class SimpProblem:
     residual of the last solve are kept in last_iterations / last_rel."""
 
     def __init__(self, shape, ke, mask, force, *, precision="mixed", tol=1e-8, maxiter=200, penal=3.0, E0=1.0,
-                 Emin=1e-9, graph=False, **solver_kw):
+                 Emin=1e-9, graph=False, warm_start=False, **solver_kw):
         from .engine import GMGSolver
         import voxel_kernels as vk
         self.shape = tuple(int(s) for s in shape)
@@ -151,15 +151,28 @@ class SimpProblem:
         self.mask = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape)
         solver_kw.setdefault("smoother", "chebyshev")
         self.solver = GMGSolver(self.shape, ke, self.mask, None, precision=precision, graph=graph, **solver_kw)
-        self.last_iterations, self.last_rel = None, None
+        # warm start: each solve starts from the previous state (the tolerance stays relative to
+        # |f|), with the fail-closed retry from zero of simp_r2.py:680-708
+        self.warm_start = bool(warm_start)
+        self._u = None
+        self.last_iterations, self.last_rel, self.last_retried = None, None, False
 
     def solve(self, E):
         self.solver.update(E)
-        x, it, ok, rel, _ = self.solver.solve(self.force, tol=self.tol, maxiter=self.maxiter)
+        x0 = self._u if self.warm_start else None
+        x, it, ok, rel, _ = self.solver.solve(self.force, tol=self.tol, maxiter=self.maxiter, x0=x0)
+        self.last_retried = False
+        if not ok and x0 is not None:
+            self.last_retried = True
+            x, it2, ok, rel, _ = self.solver.solve(self.force, tol=self.tol, maxiter=self.maxiter)
+            it += it2
         if not ok:
             raise RuntimeError(f"state solve did not reach {self.tol:g} in {it} iterations (rel {rel:.3e})")
         self.last_iterations, self.last_rel = it, rel
-        return x.clone()
+        u = x.clone()
+        if self.warm_start:
+            self._u = u.clone()
+        return u
 
 
 def simp_run(problem, steps, volfrac=0.3, rmin=1.5, move=0.2, stop_change=None):

@@ -163,6 +163,39 @@ def test_simp_loop_runs_and_keeps_the_volume():
     assert float(rho.min()) >= 0.0 and float(rho.max()) <= 1.0
 
 
+def test_warm_started_simp_loop_follows_the_cold_one():
+    """State solves started from the previous state (tolerance relative to |f| either way): the
+    compliance and volume histories of the cold run within 1e-7, fewer iterations after the first
+    step; a warm start that cannot converge falls back to the solve from zero."""
+    from voxelhip.simp import SimpProblem, simp_run
+    shape = (16, 8, 8)
+    ke = 2 * hx.quadrature([1, 1, 1], 0.3)
+    _, _, mask, force = sr.cantilever(shape, seed=1)
+    runs = []
+    for warm in (False, True):
+        print("case warm", warm, flush=True)
+        pb = SimpProblem(shape, ke, mask, force, precision="mixed", tol=1e-8, max_coarse_x=4, degree=3,
+                         warm_start=warm)
+        runs.append((pb, simp_run(pb, steps=6, volfrac=0.3, rmin=1.5, move=0.2)[0]))
+    (_, cold), (pb, warm) = runs
+    for a, b in zip(cold, warm):
+        assert b["compliance"] == pytest.approx(a["compliance"], rel=1e-7)
+        assert b["volume"] == pytest.approx(a["volume"], abs=1e-7)
+    assert warm[0]["iterations"] == cold[0]["iterations"]
+    assert sum(h["iterations"] for h in warm[1:]) < sum(h["iterations"] for h in cold[1:])
+    # fail-closed: a warm start that misses the iteration limit is retried from zero
+    E = torch.full((int(np.prod(shape)),), 0.5, dtype=torch.float64, device="cuda")
+    pb.warm_start = False
+    u0 = pb.solve(E)
+    n0 = pb.last_iterations
+    assert not pb.last_retried
+    pb.warm_start, pb.maxiter = True, n0 + 1
+    pb._u = torch.full_like(u0, 1e6)          # far from the solution: needs many more than n0 + 1
+    u = pb.solve(E)
+    assert pb.last_retried and pb.last_iterations == n0 + 1 + n0 and pb.last_rel <= 1e-8
+    assert float((u - u0).norm() / u0.norm()) < 1e-12    # the retry is the cold solve
+
+
 def test_mixed_precision_state_solve_gives_the_fp64_sensitivities():
     """The SIMP step on the mixed solver (FP32 V-cycle, FP64 outer CG to 1e-8): compliance and
     dc/drho within 1e-6 of the FP64 solve at 1e-12."""
