@@ -535,7 +535,7 @@ def spectral_omegas(xp, levels, iterations=30, seed=20260923):
     return omegas, lambdas
 
 
-def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923):
+def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923, start=None, keep=None):
     """Upper bounds of the spectrum of D^-1 A per smoothing level for the Chebyshev smoother:
     safety x the D-weighted Rayleigh quotient v.Av / v.Dv after `iterations` power steps
     v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from the seeded
@@ -543,11 +543,13 @@ def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923):
     approaches lambda_max from below slowly on smooth moduli - 30 steps give 2.81 for 3.09 on a
     uniform 256 x 128 x 128 grid - and a Chebyshev interval that ends below the spectrum
     amplifies the top modes; the Rayleigh quotient converges twice as fast and the 10 % margin
-    covers the rest.)  The normalisation stays on the device: one host read per level."""
+    covers the rest.)  The normalisation stays on the device: one host read per level.
+    start: per-level vectors to start from instead of the seeded one (a previous call's `keep`,
+    for an operator that changed little); keep: a list that receives each level's last iterate."""
     lams = []
-    for Lv in levels[:-1]:
+    for li, Lv in enumerate(levels[:-1]):
         d = Lv.product.diagonal(Lv.E, mask=Lv.maskobj)               # D, 1 at fixed DOFs
-        w = _start_vector(Lv.n, seed, Lv.dtype).clone()
+        w = (start[li] if start is not None else _start_vector(Lv.n, seed, Lv.dtype)).clone()
         w = ops.axpby(1.0, w, 0.0, w, w=Lv.mask)
         red = _Reducer(Lv.n, w.device)
         v, t = torch.empty_like(w), torch.empty_like(w)
@@ -564,4 +566,6 @@ def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923):
         red.dot(v, t, 9)
         s89 = red.scal[8:10].cpu()
         lams.append(safety * float(s89[0]) / float(s89[1]))
+        if keep is not None:
+            keep.append(v)
     return lams

@@ -27,6 +27,10 @@ precision:
 coarse_degree / coarse_level: smoothing steps on the levels from coarse_level down (the donor's
   coarse_smooth_iters, multigrid_v4.py:466,934-967); None keeps degree / pre / post there.
 
+bound_warm_iterations: for a sequence of rebuilds with slowly changing moduli (a SIMP run), the
+  Chebyshev bounds of each rebuild after the first continue the previous rebuild's power iteration
+  for this many steps instead of 30 steps from the seeded vector (None: always from the seed).
+
 coarse:
   "rediscretize"  every coarse level rediscretised (ke * 2^l with averaged moduli; src/voxel_mg.py)
   "galerkin"      rediscretised levels, the coarsest one the Galerkin P^T A P of the level above
@@ -49,7 +53,7 @@ class GMGSolver:
     def __init__(self, shape, ke, mask, E=None, *, max_coarse_x=16, n_levels=None, smoother="jacobi",
                  degree=3, pre=4, post=4, precision="fp64", coarse="rediscretize", graph=False,
                  lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False,
-                 coarse_degree=None, coarse_level=2, fp32_direction=True):
+                 coarse_degree=None, coarse_level=2, fp32_direction=True, bound_warm_iterations=None):
         self.shape = tuple(int(s) for s in shape)
         self.ke = np.asarray(ke, dtype=np.float64)
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -64,6 +68,10 @@ class GMGSolver:
             raise ValueError(precision)
         self.precision = precision
         self.fp32_direction = bool(fp32_direction)
+        # Chebyshev bounds of a rebuild continue the previous rebuild's power iteration for this many
+        # steps (None: 30 steps from the seeded vector every time)
+        self.bound_warm_iterations = bound_warm_iterations
+        self._bound_vectors = None
         self.pdtype = torch.float64 if precision == "fp64" else torch.float32
         self.inner_tol = inner_tol
         # the FP64 coarsest solve inside the FP32 V-cycle of the refinement scheme
@@ -103,7 +111,7 @@ class GMGSolver:
                                             omega_iterations=self.omega_iterations,
                                             coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin,
                                             stencils=stencils, coarse_degree=self.coarse_degree,
-                                            coarse_level=self.coarse_level)
+                                            coarse_level=self.coarse_level, **self._bound_args())
         # the last post-smoothing step of the V-cycle forms the Polak-Ribiere dots itself (and
         # writes z in the V-cycle precision, which the DOT pass reads directly)
         self.fused_pr = (self.smoother == "chebyshev" and len(self.shapes) > 1 and self.degree >= 1
@@ -120,6 +128,16 @@ class GMGSolver:
                 self.M.levels[0].tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True)])
         self._alloc()
         self._graphs = None
+
+    def _bound_args(self):
+        if self.bound_warm_iterations is None or self.smoother != "chebyshev":
+            return {}
+        warm = self._bound_vectors is not None
+        keep = []
+        args = dict(bound_start=self._bound_vectors, bound_keep=keep,
+                    bound_iterations=int(self.bound_warm_iterations) if warm else 30)
+        self._bound_vectors = keep
+        return args
 
     def _galerkin_coarsest(self, E64):
         """A_c = P^T A_{L-2} P with the coarse mask re-imposed, assembled on the device."""
@@ -381,12 +399,15 @@ class GMGSolver:
 
 def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev", degree=3, pre=4, post=4,
                     lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None,
-                    stencils=None, coarse_degree=None, coarse_level=2):
+                    stencils=None, coarse_degree=None, coarse_level=2, bound_start=None, bound_keep=None,
+                    bound_iterations=30):
     """The FP64 fine Level and the Multigrid (levels in pdtype, rediscretised ke * 2^l, 2x2x2
     modulus averaging, injected masks, explicit inverse of the coarsest operator, spectral
     omegas / lambdas) of one problem.  coarse_matrix: a precomputed coarsest operator (Galerkin);
     otherwise the rediscretised one is assembled.  stencils: assembled Galerkin operators
-    (StencilProduct) of levels 1 .. L-2, which then replace the rediscretised ones."""
+    (StencilProduct) of levels 1 .. L-2, which then replace the rediscretised ones.
+    bound_start / bound_keep / bound_iterations: the power iteration of the Chebyshev bounds
+    (mg.chebyshev_bounds start, keep and iterations)."""
     dev = E64.device
     prods = [mg.HipProduct(s, ke * 2 ** i, pdtype) for i, s in enumerate(shapes)]
     if stencils is not None:
@@ -411,7 +432,8 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
     M = mg.Multigrid(None, shapes, prods, E64, mask, Lc, pdtype, coarse_dtype=coarse_dtype, **kw)
     # Jacobi: the reference's spectral omegas; Chebyshev: safe upper bounds of the spectrum
     if smoother == "chebyshev":
-        M.lambdas = mg.chebyshev_bounds(None, M.levels)
+        M.lambdas = mg.chebyshev_bounds(None, M.levels, iterations=bound_iterations, start=bound_start,
+                                        keep=bound_keep)
     else:
         M.omegas, M.lambdas = mg.spectral_omegas(None, M.levels, iterations=omega_iterations)
     return fine, M

@@ -155,3 +155,23 @@ def test_coarse_degree_keeps_the_solution(problem, precision):
     assert ok and r <= 1e-8 and it <= ito + 2
     assert rel(x, xo) < 1e-7
 
+
+
+def test_continued_power_iteration_bounds(problem):
+    """bound_warm_iterations: the first build's Chebyshev bounds are the default ones; a rebuild
+    continues the power iteration (the D-weighted Rayleigh quotient of a self-adjoint operator
+    does not decrease along it, up to FP32 rounding) and the solve still converges."""
+    from voxelhip.engine import GMGSolver
+    shape, E, mask, force, xo, ito = problem
+    Ed = torch.from_numpy(E).cuda()
+    kw = dict(max_coarse_x=8, smoother="chebyshev", degree=3, precision="mixed")
+    s = GMGSolver(shape, KE, mask, Ed, bound_warm_iterations=10, **kw)
+    lam0 = list(s.M.lambdas)
+    assert lam0 == list(GMGSolver(shape, KE, mask, Ed, **kw).M.lambdas)
+    s.update(Ed)
+    lam1 = list(s.M.lambdas)
+    for a, b in zip(lam0, lam1):
+        assert a * (1 - 1e-5) <= b <= a * 1.05, (lam0, lam1)
+    x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
+    assert ok and r <= 1e-8 and it <= ito + 1
+    assert rel(x, xo) < 1e-7

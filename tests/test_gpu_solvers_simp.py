@@ -172,18 +172,24 @@ def test_warm_started_simp_loop_follows_the_cold_one():
     ke = 2 * hx.quadrature([1, 1, 1], 0.3)
     _, _, mask, force = sr.cantilever(shape, seed=1)
     runs = []
-    for warm in (False, True):
-        print("case warm", warm, flush=True)
+    # cold; warm start alone (fixed spectral bounds procedure); warm start with the bounds'
+    # power iteration continued from the previous state (the SimpProblem default when warm)
+    for warm, bw in ((False, None), (True, None), (True, "default")):
+        print("case warm", warm, bw, flush=True)
+        kw = {} if bw == "default" else dict(bound_warm_iterations=bw)
         pb = SimpProblem(shape, ke, mask, force, precision="mixed", tol=1e-8, max_coarse_x=4, degree=3,
-                         warm_start=warm)
+                         warm_start=warm, **kw)
         runs.append((pb, simp_run(pb, steps=6, volfrac=0.3, rmin=1.5, move=0.2)[0]))
-    (_, cold), (pb, warm) = runs
-    for a, b in zip(cold, warm):
-        assert b["compliance"] == pytest.approx(a["compliance"], rel=1e-7)
-        assert b["volume"] == pytest.approx(a["volume"], abs=1e-7)
-    assert warm[0]["iterations"] == cold[0]["iterations"]
+    (_, cold), (pb, warm), (pbw, warm_bw) = runs
+    assert pbw.solver.bound_warm_iterations == 10
+    for run in (warm, warm_bw):
+        for a, b in zip(cold, run):
+            assert b["compliance"] == pytest.approx(a["compliance"], rel=1e-7)
+            assert b["volume"] == pytest.approx(a["volume"], abs=1e-7)
+        assert run[0]["iterations"] == cold[0]["iterations"]
     assert sum(h["iterations"] for h in warm[1:]) < sum(h["iterations"] for h in cold[1:])
-    # fail-closed: a warm start that misses the iteration limit is retried from zero
+    # fail-closed: a warm start that misses the iteration limit is retried from zero (on the
+    # problem whose rebuilds repeat the same bounds, so the retry is the cold solve itself)
     E = torch.full((int(np.prod(shape)),), 0.5, dtype=torch.float64, device="cuda")
     pb.warm_start = False
     u0 = pb.solve(E)
