@@ -320,7 +320,7 @@ def main():
     shape2 = (128, 64, 64)
     _, E2, f2, m2, _ = problem(shape2, seed=2)
     s2 = GMGSolver(shape2, ke, m2, E2, smoother="chebyshev", degree=3, precision="fp64", coarse="galerkin",
-                   graph=bool(args.graph), tune=tuner)
+                   graph=bool(args.graph), tune=tuner, lower_frac=1.0 / 15.0)
     dt2, st2 = timed_solves(s2, f2, args.steps, args.warmup)
     out["c2_fp64"] = {"shape": list(shape2), "seconds_per_solve": dt2, "solves_per_s": 1 / dt2,
                       "iterations": sorted({x[0] for x in st2}), "converged": all(x[1] for x in st2)}
