@@ -108,7 +108,7 @@ def timed_batch(problems, ke, steps, warmup, graph=True, sequential=True):
     from voxelhip.batch import BatchedGMGSolver, coarse_dof_shapes
     from voxelhip.engine import GMGSolver
     t0 = time.perf_counter()
-    bs = BatchedGMGSolver(problems, ke, precision="mixed", graph=graph)
+    bs = BatchedGMGSolver(problems, ke, precision="mixed", graph=graph, lower_frac=1.0 / 15.0)
     setup = time.perf_counter() - t0
     for _ in range(warmup):
         bs.solve(tol=1e-8, maxiter=200)
@@ -132,7 +132,7 @@ def timed_batch(problems, ke, steps, warmup, graph=True, sequential=True):
         for p in problems:
             shapes = coarse_dof_shapes(p.shape)
             solvers.append(GMGSolver(p.shape, ke, p.mask, p.E, smoother="chebyshev", degree=3, precision="mixed",
-                                     graph=graph, max_coarse_x=0, n_levels=len(shapes)))
+                                     graph=graph, max_coarse_x=0, n_levels=len(shapes), lower_frac=1.0 / 15.0))
         loads = [[torch.from_numpy(f).cuda() for f in p.loads] for p in problems]
         for s, fs in zip(solvers, loads):
             for f in fs:
