@@ -360,11 +360,11 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": out["mixed"]["seconds_per_solve"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 V-cycle in f64 outer CG",
+        "dtype": "fp32 V-cycle and search direction in f64 outer CG",
         "data": "synthetic (seeded densities; no dataset)",
         "config": {"workload": "cantilever 256x128x128 (12,830,211 DOF), GMG-FCG 5 levels Chebyshev deg 3 on "
-                               "[lmax/15, lmax] (deg 2 on levels 2-3), FP32 V-cycle in FP64 outer CG, rtol 1e-8 "
-                               "(FP64 true residual)",
+                               "[lmax/15, lmax] (deg 2 on levels 2-3), FP32 V-cycle and search direction in FP64 outer CG "
+                               "(FP64 iterate and true residual), rtol 1e-8 on the FP64 true residual",
                    "ndof": ndof, "parallelism": "single GPU"},
         "fp64_solves_per_s": out["fp64"]["solves_per_s"],
         "mixed_speedup_vs_fp64": out["fp64"]["seconds_per_solve"] / out["mixed"]["seconds_per_solve"],
