@@ -112,6 +112,8 @@ class GMGSolver:
                                             coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin,
                                             stencils=stencils, coarse_degree=self.coarse_degree,
                                             coarse_level=self.coarse_level, **self._bound_args())
+        if getattr(self, "_pending_bounds", None) is not None:     # the rebuild completed
+            self._bound_vectors, self._pending_bounds = self._pending_bounds, None
         # the last post-smoothing step of the V-cycle forms the Polak-Ribiere dots itself (and
         # writes z in the V-cycle precision, which the DOT pass reads directly)
         self.fused_pr = (self.smoother == "chebyshev" and len(self.shapes) > 1 and self.degree >= 1
@@ -132,12 +134,12 @@ class GMGSolver:
     def _bound_args(self):
         if self.bound_warm_iterations is None or self.smoother != "chebyshev":
             return {}
-        warm = self._bound_vectors is not None
+        prev, self._bound_vectors = self._bound_vectors, None    # (a rebuild that fails keeps none)
+        warm = prev is not None and len(prev) == len(self.shapes) - 1
         keep = []
-        args = dict(bound_start=self._bound_vectors, bound_keep=keep,
+        self._pending_bounds = keep
+        return dict(bound_start=prev if warm else None, bound_keep=keep,
                     bound_iterations=int(self.bound_warm_iterations) if warm else 30)
-        self._bound_vectors = keep
-        return args
 
     def _galerkin_coarsest(self, E64):
         """A_c = P^T A_{L-2} P with the coarse mask re-imposed, assembled on the device."""
