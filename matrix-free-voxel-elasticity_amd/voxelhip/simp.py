@@ -151,7 +151,7 @@ class SimpProblem:
         self.mask = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape)
         solver_kw.setdefault("smoother", "chebyshev")
         if warm_start:       # the spectral bounds also continue from the previous state's
-            solver_kw.setdefault("bound_warm_iterations", 10)
+            solver_kw.setdefault("bound_warm_iterations", 5)
         self.solver = GMGSolver(self.shape, ke, self.mask, None, precision=precision, graph=graph, **solver_kw)
         # warm start: each solve starts from the previous state (the tolerance stays relative to
         # |f|), with the fail-closed retry from zero of simp_r2.py:680-708

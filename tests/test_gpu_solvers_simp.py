@@ -181,7 +181,7 @@ def test_warm_started_simp_loop_follows_the_cold_one():
                          warm_start=warm, **kw)
         runs.append((pb, simp_run(pb, steps=6, volfrac=0.3, rmin=1.5, move=0.2)[0]))
     (_, cold), (pb, warm), (pbw, warm_bw) = runs
-    assert pbw.solver.bound_warm_iterations == 10
+    assert pbw.solver.bound_warm_iterations == 5
     for run in (warm, warm_bw):
         for a, b in zip(cold, run):
             assert b["compliance"] == pytest.approx(a["compliance"], rel=1e-7)
