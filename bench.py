@@ -67,38 +67,8 @@ def problem(shape, seed, rmin=1.5):
 
 
 def c4_problems(seed=4, n=48):
-    """Batched workload: n cantilever-type problems with grids drawn from the seed between
-    16 x 16 x 16 and 96 x 48 x 48 elements (every extent a multiple of 8), densities iid
-    U[0.01, 1] smoothed by the radius-1.5 cone filter, x = 0 face clamped, and 1-4 load cases of
-    one unit point load each at a random node of a random face other than x = 0."""
-    import torch
-    from voxelhip.batch import BatchProblem
-    from voxelhip.simp import ConeFilter, modulus
-    rng = np.random.default_rng(seed)
-    dev = torch.device("cuda", 0)
-    out = []
-    for _ in range(n):
-        nx = 8 * int(rng.integers(2, 13))
-        ny = 8 * int(rng.integers(2, min(6, nx // 8) + 1))
-        nz = 8 * int(rng.integers(2, min(6, nx // 8) + 1))
-        shape = (nx, ny, nz)
-        rho = torch.from_numpy(rng.uniform(0.01, 1.0, nx * ny * nz)).to(dev)
-        E = modulus(ConeFilter(shape, 1.5, device=dev).forward(rho), 3.0, 1.0, 1e-9)
-        mask = np.ones((nx + 1, ny + 1, nz + 1, 3))
-        mask[0] = 0
-        loads = []
-        for _ in range(int(rng.integers(1, 5))):
-            f = np.zeros((nx + 1, ny + 1, nz + 1, 3))
-            face = int(rng.integers(0, 5))
-            ix, iy, iz = int(rng.integers(1, nx + 1)), int(rng.integers(0, ny + 1)), int(rng.integers(0, nz + 1))
-            ix = nx if face == 0 else ix
-            iy = 0 if face == 1 else (ny if face == 2 else iy)
-            iz = 0 if face == 3 else (nz if face == 4 else iz)
-            d = rng.normal(size=3)
-            f[ix, iy, iz] = d / np.linalg.norm(d)
-            loads.append(f.reshape(-1))
-        out.append(BatchProblem(shape, E, mask.reshape(-1), loads))
-    return out
+    from voxelhip.batch import random_cantilevers
+    return random_cantilevers(seed=seed, n=n)
 
 
 def timed_batch(problems, ke, steps, warmup, graph=True, sequential=True):

@@ -395,7 +395,7 @@ class BatchedGMGSolver:
         self.dot_h[i][0].launch()
         self._finish(self.dot_offs[i], 1, L.S_PQ, L.OP_ALPHA)
         self.res_h[i][0].launch()
-        self._finish(self.res_offs[i], 1, 8, L.OP_NONE)
+        self._finish(self.res_offs[i], 1, 8, L.OP_RESNORM)
         zl = self.vcycle()
         L.call("vh_seg_dotdiff", self.segs.h, _DT[zl.dtype], L.F64, zl.data_ptr(), self.r.data_ptr(),
                self.zs[i].data_ptr(), self.zs[o].data_ptr(), self.partial.data_ptr(), ops._stream())
@@ -417,20 +417,45 @@ class BatchedGMGSolver:
         return graphs
 
     # ------------------------------------------------------------------ solve
-    def solve(self, tol=1e-8, maxiter=200):
-        """Solves every instance from x = 0 to ||b - A x|| / ||b|| <= tol.  Returns
-        (x list per instance (views), iterations per instance, converged flags, rel residuals)."""
-        if self.graph and self._graphs is None:
-            self._graphs = self._capture()
-        n = self.n
+    def _upload_loads(self):
+        """Every load case into the resident right-hand side b (once), and |b|^2 per instance."""
         for k, (p, l) in enumerate(self.inst):
             fo, nf = self.off[0][0][k], self.hier[p][1].n
             self.b[fo:fo + nf] = voxelhip_device(self.problems[p].loads[l], torch.float64, self.dev)
         self.scal.zero_()
         L.call("vh_seg_reduce", self.segs.h, L.F64, L.RED_NORM2, self.b.data_ptr(), None, self.partial.data_ptr(),
                ops._stream())
-        self._finish((self.seg_offs, n, self.seg_total), 1, 9, L.OP_NONE)
-        nb = self.scal.view(n, 16)[:, 9].sqrt().cpu().numpy()
+        self._finish((self.seg_offs, self.n, self.seg_total), 1, 9, L.OP_NONE)
+        self.nb = self.scal.view(self.n, 16)[:, 9].sqrt().cpu().numpy()
+        self._loaded = True
+
+    def set_load(self, k, force):
+        """Replaces the load vector of instance k (a device or host vector of its problem's size)."""
+        p, _ = self.inst[k]
+        fo, nf = self.off[0][0][k], self.hier[p][1].n
+        self.b[fo:fo + nf] = voxelhip_device(force, torch.float64, self.dev)
+        self.nb[k] = float(self.b[fo:fo + nf].norm())
+
+    def solve(self, tol=1e-8, maxiter=200):
+        """Solves every instance from x = 0 to ||b - A x|| / ||b|| <= tol.  Returns
+        (x list per instance (views), iterations per instance, converged flags, rel residuals).
+
+        The load vectors stay resident in HBM between solves (uploaded once; set_load replaces
+        one).  Convergence is tested on the device: each iteration's residual pass ends in
+        OP_RESNORM per instance, which counts the instance's iterations and freezes its recurrence
+        (alpha = beta = 0) once |r|^2 <= tol^2 |b|^2.  The host reads the scalar block
+        asynchronously and keeps the next iteration queued while some instance's residual
+        history shows it cannot converge in the iteration being read, so the GPU does not wait
+        for the host between iterations; an iteration past an instance's convergence leaves its
+        iterate unchanged."""
+        if self.graph and self._graphs is None:
+            self._graphs = self._capture()
+        if not getattr(self, "_loaded", False):
+            self._upload_loads()
+        n = self.n
+        sv = self.scal.view(n, 16)
+        sv.zero_()
+        sv[:, L.S_TOL2] = torch.from_numpy((tol * self.nb) ** 2).to(self.dev)
         self.xs[1].zero_()
         self.ps[1].zero_()
         self.r.copy_(self.b)
@@ -441,34 +466,92 @@ class BatchedGMGSolver:
         L.call("vh_seg_reduce", self.segs.h, L.F64, L.RED_DOT, self.r.data_ptr(), self.zs[0].data_ptr(),
                self.partial.data_ptr(), ops._stream())
         self._finish((self.seg_offs, n, self.seg_total), 1, L.S_RZ, L.OP_NONE)
-        its = np.zeros(n, dtype=int)
-        done = np.zeros(n, dtype=bool)
-        rel = np.full(n, np.nan)
-        sv = self.scal.view(n, 16)
-        it = 0
-        for it in range(1, maxiter + 1):
-            i = (it - 1) % 2
+        if not hasattr(self, "_hbuf"):
+            self._hbuf = [torch.empty(16 * n, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+            self._hev = [torch.cuda.Event() for _ in range(2)]
+        pending, hist = [], []
+        nb = np.where(self.nb > 0, self.nb, 1.0)
+
+        def launch(k):
+            i = (k - 1) % 2
             if self._graphs is not None:
                 self._graphs[i].replay()
             else:
                 self._iteration(i)
-            r2 = sv[:, 8].cpu().numpy()
-            cur = np.sqrt(r2) / np.where(nb > 0, nb, 1.0)
-            newly = (~done) & (cur <= tol)
-            rel[~done] = cur[~done]
-            its[~done] = it
-            if newly.any():
-                done |= newly
-                idx = torch.from_numpy(np.nonzero(newly)[0] * 16 + L.S_DONE).to(self.dev)
-                self.scal[idx] = 1.0
-            if done.all():
-                break
-        x = self.xs[(it - 1) % 2]
-        out = []
-        for k, (p, _) in enumerate(self.inst):
-            fo, nf = self.off[0][0][k], self.hier[p][1].n
-            out.append(x[fo:fo + nf])
-        return out, its, done, rel
+            self._hbuf[k % 2].copy_(self.scal, non_blocking=True)
+            self._hev[k % 2].record()
+            pending.append(k)
+
+        def next_cannot_converge():
+            # some unconverged instance's residual, shrunk at its recent rate, stays above twice
+            # the tolerance: the batch needs the iteration after the next one as well
+            if not hist:
+                return False
+            h1 = hist[-1]
+            h2 = hist[-2] if len(hist) > 1 else np.ones_like(h1)
+            rate = h1 / np.maximum(h2, 1e-300)
+            if len(hist) > 2:
+                rate = np.maximum(rate, h2 / np.maximum(hist[-3], 1e-300))
+            live = h1 > tol
+            return bool((h1[live] * rate[live] > 2.0 * tol).any())
+
+        launch(1)
+        while True:
+            if len(pending) == 1 and pending[0] < maxiter and next_cannot_converge():
+                launch(pending[0] + 1)
+            k = pending.pop(0)
+            self._hev[k % 2].synchronize()
+            v = self._hbuf[k % 2].view(n, 16).numpy()
+            rel = np.sqrt(v[:, 8]) / nb
+            hist.append(rel)
+            done = v[:, L.S_DONE] != 0.0
+            if done.all() or k >= maxiter:
+                for q in pending:                     # iterations queued past convergence
+                    self._hev[q % 2].synchronize()
+                its = np.where(done, v[:, L.S_ITS], k).astype(int)
+                last = pending[-1] if pending else k
+                x = self.xs[(last - 1) % 2]
+                out = []
+                for j, (p, _) in enumerate(self.inst):
+                    fo, nf = self.off[0][0][j], self.hier[p][1].n
+                    out.append(x[fo:fo + nf])
+                return out, its, done, rel
+            if not pending:
+                launch(k + 1)
+
+
+def random_cantilevers(seed=4, n=48):
+    """A batch workload: n cantilever-type problems with grids drawn from the seed between
+    16 x 16 x 16 and 96 x 48 x 48 elements (every extent a multiple of 8), densities iid
+    U[0.01, 1] smoothed by the radius-1.5 cone filter (E = 1e-9 + (1 - 1e-9) rho^3), x = 0 face
+    clamped, and 1-4 load cases of one unit point load each at a random node of a random face
+    other than x = 0.  Returns a list of BatchProblem."""
+    from .simp import ConeFilter, modulus
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = []
+    for _ in range(n):
+        nx = 8 * int(rng.integers(2, 13))
+        ny = 8 * int(rng.integers(2, min(6, nx // 8) + 1))
+        nz = 8 * int(rng.integers(2, min(6, nx // 8) + 1))
+        shape = (nx, ny, nz)
+        rho = torch.from_numpy(rng.uniform(0.01, 1.0, nx * ny * nz)).to(dev)
+        E = modulus(ConeFilter(shape, 1.5, device=dev).forward(rho), 3.0, 1.0, 1e-9)
+        mask = np.ones((nx + 1, ny + 1, nz + 1, 3))
+        mask[0] = 0
+        loads = []
+        for _ in range(int(rng.integers(1, 5))):
+            f = np.zeros((nx + 1, ny + 1, nz + 1, 3))
+            face = int(rng.integers(0, 5))
+            ix, iy, iz = int(rng.integers(1, nx + 1)), int(rng.integers(0, ny + 1)), int(rng.integers(0, nz + 1))
+            ix = nx if face == 0 else ix
+            iy = 0 if face == 1 else (ny if face == 2 else iy)
+            iz = 0 if face == 3 else (nz if face == 4 else iz)
+            d = rng.normal(size=3)
+            f[ix, iy, iz] = d / np.linalg.norm(d)
+            loads.append(f.reshape(-1))
+        out.append(BatchProblem(shape, E, mask.reshape(-1), loads))
+    return out
 
 
 def voxelhip_device(a, dtype, dev):

@@ -5,9 +5,10 @@ It restates the reference's harness (code/hex_modal_cuda/common.py:57-117 and ru
 experiments/session_i/make_inputs.py:31-49 and run_gpu.py:90-104,255-330) for the paths of this
 library:
 
-- inputs: `regenerate(shape)` (every DOF of the x = 0 face clamped; a consistent line load of total
-  1 in -z along y at x = nx, z = 0), `upsample(data, factor)` (nearest-neighbour refinement of a
-  design, the boundary data regenerated), `to_author` (natural (z, y, x[, 3]) -> author layout);
+- inputs: `cantilever_boundary(shape)` (every DOF of the x = 0 face clamped; a consistent line load
+  of total 1 in -z along y at x = nx, z = 0), `refine_design(rho, shape, factor)` (nearest-neighbour
+  refinement of a base design), `load_case` (both at a study case's resolution), all in the author
+  layout; `to_author` (natural (z, y, x[, 3]) -> author layout) for the reference's input files;
 - correctness: `product_check` (relative and per-sqrt(DOF) absolute L2 limits), `exactness` (zero in,
   zero out; identity rows at fixed DOFs), `physical_check` (true residual recomputed by the dense
   node kernel, an arithmetic independent of the parity product, and the energy / force-work check);
@@ -48,7 +49,8 @@ PROTOCOL = {
     "timing": {"rounds": 5, "calls_per_batch": 10, "bootstrap_seed": 20260923, "bootstrap_resamples": 10000,
                "median_ratio_gate": 1.15, "lower_bound_gate": 1.05},
 }
-CASES = {"q4x2": ("smoke-q4.npz", 2), "q96x1": ("optimized-q96.npz", 1), "q64x2": ("optimized-q64.npz", 2),
+CASES = {"q4x2": ("smoke-q4.npz", 2), "q64x1": ("optimized-q64.npz", 1), "q96x1": ("optimized-q96.npz", 1),
+         "q64x2": ("optimized-q64.npz", 2),
          "q96x2": ("optimized-q96.npz", 2), "q96x3": ("optimized-q96.npz", 3)}
 
 # fields of a solve record and their types (result.json "solves" rows)
@@ -62,28 +64,32 @@ RESULT_SCHEMA = {
 
 
 # ---------------------------------------------------------------------------------- inputs
-def regenerate(shape):
-    """Mask and force of the cantilever in the natural layout (z, y, x, 3) at any resolution."""
+def cantilever_boundary(shape):
+    """Author-layout (mask, force) of the study cantilever at any resolution: every DOF of the
+    node plane ix = 0 fixed; a load of total 1 in -z along the line ix = nx, iz = 0, shared
+    trapezoidally over y (weight 1/ny per node, half at iy = 0 and iy = ny)."""
     nx, ny, nz = (int(n) for n in shape)
-    mask = np.ones((nz + 1, ny + 1, nx + 1, 3))
-    mask[:, :, 0, :] = 0.0
-    force = np.zeros((nz + 1, ny + 1, nx + 1, 3))
-    line = np.full(ny + 1, -1.0 / ny)
-    line[[0, -1]] = -0.5 / ny
-    force[0, :, nx, 2] = line
-    return mask.ravel(), force.ravel()
+    plane = 3 * (ny + 1) * (nz + 1)                # DOFs of one x node plane
+    mask = np.ones(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    mask[:plane] = 0.0
+    force = np.zeros_like(mask)
+    iy = np.arange(ny + 1)
+    weight = np.where((iy == 0) | (iy == ny), 0.5, 1.0) / ny
+    force[nx * plane + 3 * (nz + 1) * iy + 2] = -weight
+    return mask, force
 
 
-def upsample(data, factor):
-    """Each element repeated `factor` times per axis; boundary data regenerated at the new size."""
-    nx, ny, nz = (int(n) for n in data["shape"])
-    rho = np.asarray(data["rho"]).reshape(nz, ny, nx)
-    for axis in range(3):
-        rho = np.repeat(rho, int(factor), axis=axis)
-    shape = np.array([nx * factor, ny * factor, nz * factor])
-    mask, force = regenerate(shape)
-    return {"shape": shape, "rho": np.ascontiguousarray(rho.ravel()), "mask": mask, "force": force,
-            "ke": np.asarray(data["ke"])}
+def refine_design(rho, shape, factor):
+    """Author-layout densities of the design `rho` (natural (z, y, x) order, `shape` elements)
+    refined `factor` times per axis: fine element (ex, ey, ez) takes the value of coarse element
+    (ex // f, ey // f, ez // f)."""
+    nx, ny, nz = (int(n) for n in shape)
+    f = int(factor)
+    base = np.asarray(rho).reshape(nz, ny, nx)
+    ex = np.arange(nx * f)[:, None, None] // f
+    ey = np.arange(ny * f)[None, :, None] // f
+    ez = np.arange(nz * f)[None, None, :] // f
+    return np.ascontiguousarray(base[ez, ey, ex]).ravel()
 
 
 def to_author(a, shape, nodal=False):
@@ -95,11 +101,15 @@ def to_author(a, shape, nodal=False):
 
 
 def load_case(inputs, case):
-    """The named study case from the base designs in directory `inputs`."""
+    """The named study case built from its base design in directory `inputs`: the design refined
+    by the case's factor and the boundary data of the refined grid, all in the author layout."""
     base, factor = CASES[case]
     with np.load(Path(inputs) / base) as f:
-        data = {k: np.asarray(f[k]) for k in f.files}
-    return upsample(data, factor)
+        shape = [int(v) for v in f["shape"]]
+        rho, ke = np.asarray(f["rho"]), np.asarray(f["ke"])
+    fine = np.array([n * factor for n in shape])
+    mask, force = cantilever_boundary(fine)
+    return {"shape": fine, "rho": refine_design(rho, shape, factor), "mask": mask, "force": force, "ke": ke}
 
 
 # ----------------------------------------------------------------------------- correctness
@@ -215,8 +225,12 @@ def timing_decision(timings, candidate, primary=True, seed=None, resamples=None)
 
 
 # ---------------------------------------------------------------------------------- solves
-def solve_case(data, case="", device=None):
-    """The W2b FP64 multigrid solve of one case through the drop-in voxel_mg; returns the record."""
+def solve_case(data, case="", device=None, vcycle_dtype=np.float64, seed=None):
+    """The W2b multigrid solve of one case (author-layout data of load_case) through the drop-in
+    voxel_mg: FP64 flexible CG around a V-cycle in `vcycle_dtype` (np.float32: the reference's
+    mg32 arms, FP32 products and smoothing with an FP64 coarsest factor cast to FP32, the omegas
+    from the FP64 hierarchy; `seed` of the power iteration, default the study's order seed);
+    returns the record and the solution."""
     import voxel_mg as mg
     import voxelhip
     p = PROTOCOL
@@ -225,12 +239,9 @@ def solve_case(data, case="", device=None):
     ke = np.asarray(data["ke"], dtype=np.float64)
     emin = p["material"]["Emin"]
     t0 = time.perf_counter()
-    E_h = to_author(emin + (1 - emin) * np.asarray(data["rho"]) ** p["material"]["penal"], shape)
-    m_h = to_author(data["mask"], shape, True)
-    f_h = to_author(data["force"], shape, True)
-    E = torch.from_numpy(E_h).to(dev)
-    mask = torch.from_numpy(m_h).to(dev)
-    force = torch.from_numpy(f_h).to(dev)
+    E = torch.from_numpy(emin + (1 - emin) * np.asarray(data["rho"], dtype=np.float64) ** p["material"]["penal"]).to(dev)
+    mask = torch.from_numpy(np.asarray(data["mask"], dtype=np.float64)).to(dev)
+    force = torch.from_numpy(np.asarray(data["force"], dtype=np.float64)).to(dev)
     shapes = mg.hierarchy_shapes(shape, p["multigrid"]["max_coarse_x"])
     prods = [mg.HipProduct(s, ke * 2 ** i, np.float64) for i, s in enumerate(shapes)]
     # coarsest operator: moduli averaged and mask injected level by level, assembled and factored
@@ -244,13 +255,20 @@ def solve_case(data, case="", device=None):
     xp = voxelhip.xp
     M = mg.Multigrid(xp, shapes, prods, E, mask, factor, np.float64, pre=p["multigrid"]["pre"],
                      post=p["multigrid"]["post"])
-    omegas, lambdas = mg.spectral_omegas(xp, M.levels, p["multigrid"]["power_iterations"], p["order_seed"])
+    omegas, lambdas = mg.spectral_omegas(xp, M.levels, p["multigrid"]["power_iterations"],
+                                         p["order_seed"] if seed is None else seed)
     M.omegas = omegas
+    if np.dtype(vcycle_dtype) != np.float64:
+        prods = [mg.HipProduct(s, ke * 2 ** i, vcycle_dtype) for i, s in enumerate(shapes)]
+        M = mg.Multigrid(xp, shapes, prods, E, mask, factor, vcycle_dtype, pre=p["multigrid"]["pre"],
+                         post=p["multigrid"]["post"], omegas=omegas)
+        fine64 = mg.Level(xp, shape, mg.HipProduct(shape, ke, np.float64), E, mask, np.float64)
+    else:
+        fine64 = M.levels[0]
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
-    fine = M.levels[0]
     t1 = time.perf_counter()
-    x, it, ok, rel, _ = mg.fcg(xp, fine.apply, force, M, p["solver"]["requested_rtol"],
+    x, it, ok, rel, _ = mg.fcg(xp, fine64.apply, force, M, p["solver"]["requested_rtol"],
                                p["solver"]["maximum_iterations"], check_every=p["solver"]["check_every"])
     torch.cuda.synchronize()
     seconds = time.perf_counter() - t1

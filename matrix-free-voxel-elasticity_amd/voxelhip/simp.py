@@ -132,15 +132,39 @@ class Compliance(torch.autograd.Function):
         return grad * dc, None
 
 
+class _JacobiPCG:
+    """K(E) u = f by the device Jacobi-PCG (voxelhip.solvers.pcg, true residual every
+    `check_every` iterations) on the fused FP64 product: the state solver for grids whose extents
+    do not halve far enough for a multigrid hierarchy with a small coarsest level."""
+
+    def __init__(self, shape, ke, mask, check_every=50):
+        import voxel_mg as mg
+        self._mg, self.shape, self.mask, self.check_every = mg, shape, mask, check_every
+        self.product = mg.HipProduct(shape, ke)
+        self.level = None
+
+    def update(self, E):
+        from . import xp
+        self.level = self._mg.Level(xp, self.shape, self.product, E, self.mask, torch.float64)
+
+    def solve(self, b, tol=1e-8, maxiter=20000, x0=None):
+        from .solvers import pcg
+        lvl = self.level
+        x, it, ok, rel = pcg(lvl.apply, b, lvl.inv_diag, x0=x0, tol=tol, maxiter=maxiter, mask=lvl.mask,
+                             check_every=self.check_every)
+        return x, it, ok, rel, None
+
+
 class SimpProblem:
     """State problem of the SIMP loop: K(E) u = f on the device-resident GMG-FCG engine
-    (voxelhip.engine.GMGSolver; precision "mixed" = FP32 V-cycle in the FP64 outer iteration).
+    (voxelhip.engine.GMGSolver; precision "mixed" = FP32 V-cycle in the FP64 outer iteration),
+    or, with method="pcg", the device Jacobi-PCG in FP64 (for grids that do not coarsen).
     solve(E) rebuilds the hierarchy for E and returns a copy of u; the iteration count and the
-    residual of the last solve are kept in last_iterations / last_rel."""
+    residual of the last solve are kept in last_iterations / last_rel, whether the last solve
+    fell back from a warm start to a solve from zero in last_retried."""
 
     def __init__(self, shape, ke, mask, force, *, precision="mixed", tol=1e-8, maxiter=200, penal=3.0, E0=1.0,
-                 Emin=1e-9, graph=False, warm_start=False, **solver_kw):
-        from .engine import GMGSolver
+                 Emin=1e-9, graph=False, warm_start=False, method="gmg", **solver_kw):
         import voxel_kernels as vk
         self.shape = tuple(int(s) for s in shape)
         self.penal, self.E0, self.Emin = float(penal), float(E0), float(Emin)
@@ -149,15 +173,33 @@ class SimpProblem:
         self.force = force if isinstance(force, torch.Tensor) else torch.from_numpy(force)
         self.force = self.force.to("cuda", torch.float64).contiguous()
         self.mask = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape)
-        solver_kw.setdefault("smoother", "chebyshev")
-        if warm_start:       # the spectral bounds also continue from the previous state's
-            solver_kw.setdefault("bound_warm_iterations", 5)
-        self.solver = GMGSolver(self.shape, ke, self.mask, None, precision=precision, graph=graph, **solver_kw)
+        if method == "pcg":
+            if precision != "fp64":
+                raise ValueError("method='pcg' solves in FP64")
+            self.solver = _JacobiPCG(self.shape, ke, self.mask, **solver_kw)
+        elif method == "gmg":
+            from .engine import GMGSolver
+            solver_kw.setdefault("smoother", "chebyshev")
+            if warm_start:       # the spectral bounds also continue from the previous state's
+                solver_kw.setdefault("bound_warm_iterations", 5)
+            self.solver = GMGSolver(self.shape, ke, self.mask, None, precision=precision, graph=graph, **solver_kw)
+        else:
+            raise ValueError(f"method: {method!r}")
+        self.method = method
         # warm start: each solve starts from the previous state (the tolerance stays relative to
         # |f|), with the fail-closed retry from zero of simp_r2.py:680-708
         self.warm_start = bool(warm_start)
         self._u = None
         self.last_iterations, self.last_rel, self.last_retried = None, None, False
+
+    def _cold_retry(self, E):
+        """The retry from zero after a warm start missed the tolerance.  A GMG rebuild whose
+        Chebyshev bounds continued the previous power iteration re-estimates them from the seeded
+        vector first: a short continued iteration can lag a spectrum the new design has moved."""
+        s = self.solver
+        if getattr(s, "bound_warm_iterations", None) is not None and getattr(s, "_bound_vectors", None) is not None:
+            s._bound_vectors = None
+            s.update(E)
 
     def solve(self, E):
         self.solver.update(E)
@@ -166,6 +208,7 @@ class SimpProblem:
         self.last_retried = False
         if not ok and x0 is not None:
             self.last_retried = True
+            self._cold_retry(E)
             x, it2, ok, rel, _ = self.solver.solve(self.force, tol=self.tol, maxiter=self.maxiter)
             it += it2
         if not ok:
@@ -195,8 +238,9 @@ def simp_run(problem, steps, volfrac=0.3, rmin=1.5, move=0.2, stop_change=None):
         rho = filt.forward(x_new)
         change = float((x_new - x).abs().max())
         x = x_new
-        history.append({"step": k, "compliance": float(c), "change": change, "volume": float(rho.mean()),
+        history.append({"step": k, "compliance": float(c.detach()), "change": change, "volume": float(rho.mean()),
                         "iterations": getattr(problem, "last_iterations", None),
+                        "retried": bool(getattr(problem, "last_retried", False)),
                         "seconds": time.perf_counter() - t0})
         if stop_change is not None and change <= stop_change:
             break
@@ -258,7 +302,7 @@ def oc_update(rho, dc, dv, volfrac, move, filt, beta, eta, rho_lb, bisect_tol=1e
     probes = 0
     if lam_hint is not None and lam_hint > 0:
         l1, l2 = lam_hint / 4.0, lam_hint * 4.0
-        for _ in range(30):              # expand until the root is bracketed
+        for _ in range(30):              # widen both ends 8x until the volumes straddle volfrac
             v1 = design(l1)
             v2 = design(l2)
             probes += 2
@@ -302,6 +346,7 @@ def _state_solve(problem, E, x0, tol):
     retried = False
     if not ok and x0 is not None:
         retried = True
+        problem._cold_retry(E)
         x, it2, ok, rel, _ = s.solve(problem.force, tol=tol, maxiter=problem.maxiter)
         it += it2
     if not ok:

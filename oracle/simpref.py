@@ -9,6 +9,10 @@ Restates:
   ConeFilter        forward = H x / s, transpose = H (g / s), s = H 1 (filter_r2.py:93-183)
   oc_top88          bisection on the multiplier until (l2-l1)/(l1+l2) <= tol, volume on the
                     filtered field (experiments/session_h/run_w1b.py:46-62)
+  top88_run         the density-filtered top88 SIMP loop (run_w1b.py:163-222) with direct
+                    state solves (state_solve)
+  point_load_case   clamp and point load of the donor's problem presets
+                    (donor_yang_gmg/src/gpu_fem/bc_generator.py:381-447)
   sensitivity       dc/drho = -p (E0 - Emin) rho^(p-1) u_e^T ke u_e
                     (donor_yang_gmg/src/gpu_fem/solver_v4.py:246-251)
   regenerate/upsample  clamp and consistent line load, nearest-neighbour upsampling
@@ -193,14 +197,54 @@ def oc_update(rho, dc, dv, volfrac, move, filt, beta, eta, rho_lb, bisect_tol=1e
     return rho_new, achieved, lam, probes
 
 
-def run_simp(shape, ke, mask, force, steps, volfrac=0.30, rmin=1.5, penal=None, E0=1.0, Emin=1e-9,
-             rho_lb=1e-3, eta=0.5, schedule=None):
-    """`steps` outer iterations of the projected SIMP loop with exact state solves; returns the
-    per-step (compliance, physical volume) history and the final design variable."""
+def state_solve(shape, ke, mask, force, E):
+    """Direct sparse solve of K(E) u = f on the free DOFs (u = 0 at fixed DOFs); returns
+    (u, compliance f.u, element energies u_e^T ke u_e)."""
     from scipy.sparse.linalg import spsolve
     from .hexref import edof_author
     edof = edof_author(shape)
     free = np.flatnonzero(np.asarray(mask) != 0)
+    nd = 3 * (shape[0] + 1) * (shape[1] + 1) * (shape[2] + 1)
+    K = sp.coo_matrix(((np.asarray(E)[:, None, None] * ke[None]).ravel(),
+                       (np.repeat(edof, 24, axis=1).ravel(), np.tile(edof, (1, 24)).ravel())),
+                      shape=(nd, nd)).tocsr()
+    u = np.zeros(nd)
+    u[free] = spsolve(K[free][:, free].tocsc(), np.asarray(force)[free])
+    ue = u[edof]
+    return u, float(np.asarray(force) @ u), np.einsum("ei,ij,ej->e", ue, ke, ue)
+
+
+def top88_run(shape, ke, mask, force, steps, volfrac=0.3, rmin=1.5, move=0.2, penal=3.0, E0=1.0,
+              Emin=1e-9, stop_change=None):
+    """The density-filtered SIMP loop with the top88 OC of experiments/session_h/run_w1b.py:163-222
+    (update "top88"): x starts at volfrac, rho = filtered x; each step solves the state on rho,
+    takes dc/drho = -p (E0-Emin) rho^(p-1) c_e, maps dc and dv = 1 back through the filter
+    transpose, bisects the OC multiplier on the filtered volume, and records compliance, the
+    largest design change and the filtered volume.  State solves are direct.
+    Returns (history [(compliance, change, volume)], x, rho)."""
+    filt = ConeFilter(shape, rmin)
+    n = int(np.prod(shape))
+    x = np.full(n, float(volfrac))
+    rho = filt.forward(x)
+    dv_x = filt.transpose(np.ones(n))
+    hist = []
+    for _ in range(steps):
+        _, c, ce = state_solve(shape, ke, mask, force, modulus(rho, penal, E0, Emin))
+        dc_x = filt.transpose(sensitivity(rho, ce, penal, E0, Emin))
+        x_new = oc_top88(x, dc_x, dv_x, volfrac, move, density=filt.forward)
+        rho = filt.forward(x_new)
+        change = float(np.max(np.abs(x_new - x)))
+        x = x_new
+        hist.append((c, change, float(rho.mean())))
+        if stop_change is not None and change <= stop_change:
+            break
+    return hist, x, rho
+
+
+def run_simp(shape, ke, mask, force, steps, volfrac=0.30, rmin=1.5, penal=None, E0=1.0, Emin=1e-9,
+             rho_lb=1e-3, eta=0.5, schedule=None):
+    """`steps` outer iterations of the projected SIMP loop with exact state solves; returns the
+    per-step (compliance, physical volume) history and the final design variable."""
     n = int(np.prod(shape))
     rho = np.full(n, float(volfrac))
     hist = []
@@ -213,15 +257,7 @@ def run_simp(shape, ke, mask, force, steps, volfrac=0.30, rmin=1.5, penal=None, 
         rho_bar = filt.forward(rho)
         rp = project(rho_bar, beta, eta)
         E = modulus(rp, p_it, E0, Emin)
-        nd = 3 * (shape[0] + 1) * (shape[1] + 1) * (shape[2] + 1)
-        K = sp.coo_matrix(((E[:, None, None] * ke[None]).ravel(),
-                           (np.repeat(edof, 24, axis=1).ravel(), np.tile(edof, (1, 24)).ravel())),
-                          shape=(nd, nd)).tocsr()
-        u = np.zeros(nd)
-        u[free] = spsolve(K[free][:, free].tocsc(), np.asarray(force)[free])
-        c = float(np.asarray(force) @ u)
-        ue = u[edof]
-        ce = np.einsum("ei,ij,ej->e", ue, ke, ue)
+        _, c, ce = state_solve(shape, ke, mask, force, E)
         dc_drp = sensitivity(rp, ce, p_it, E0, Emin)
         dprj = dproject(rho_bar, beta, eta)
         dc = filt.transpose(dc_drp * dprj)
@@ -254,6 +290,27 @@ def upsample(data, factor):
     mask, force = regenerate(shape)
     return {"shape": shape, "rho": np.ascontiguousarray(rho.ravel()), "mask": mask,
             "force": force, "ke": data["ke"]}
+
+
+def point_load_case(dims, lengths, point, load, clamp="left"):
+    """Author-layout (mask, force) of a donor problem preset with one face clamped and one point
+    load (donor_yang_gmg/src/gpu_fem/bc_generator.py:381-447): node coordinates i*L/n, the load
+    goes to the node nearest `point` (first minimum of the Euclidean distance), every DOF of the
+    nodes on the x = 0 ("left") face is fixed.  The author node numbering
+    (ix*(ny+1) + iy)*(nz+1) + iz is the donor's."""
+    nx, ny, nz = (int(n) for n in dims)
+    if clamp != "left":
+        raise ValueError(clamp)
+    ix, iy, iz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    coords = np.stack([ix.ravel() * (lengths[0] / nx), iy.ravel() * (lengths[1] / ny),
+                       iz.ravel() * (lengths[2] / nz)], axis=1)
+    node = int(np.argmin(np.linalg.norm(coords - np.asarray(point, dtype=np.float64)[None], axis=1)))
+    mask = np.ones((nx + 1, ny + 1, nz + 1, 3))
+    mask[0] = 0.0
+    force = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    force[3 * node:3 * node + 3] += np.asarray(load, dtype=np.float64)
+    force *= mask.ravel()
+    return mask.ravel(), force
 
 
 def fixture_case(data, emin=None):

@@ -105,3 +105,15 @@ def test_assemble_dense_matches_the_oracle():
     E = rng.random(12)
     mask = (rng.random(3 * hx.n_nodes(shape)) > 0.2).astype(float)
     assert np.abs(mg.assemble_dense(shape, ke, E, mask) - hx.assemble_dense(shape, ke, E, mask)).max() < 1e-15
+
+
+def test_product_continuation_schedule_matches_the_donor_module(golden):
+    """voxelhip.simp.schedule_at against the donor's simp_r2.schedule_at (tests/golden/schedule_golden.json)."""
+    import json
+    from voxelhip import simp
+    g = json.loads((golden / "schedule_golden.json").read_text())
+    for it, row in enumerate(g["default"], start=1):
+        assert list(simp.schedule_at(it)) == row
+    table = [tuple(r) for r in g["custom_table"]]
+    for it, row in enumerate(g["custom"], start=1):
+        assert list(simp.schedule_at(it, table)) == row

@@ -47,8 +47,8 @@ def test_product_gates_and_paired_timing(golden):
     shape = tuple(int(s) for s in data["shape"])
     ke = data["ke"]
     emin = harness.PROTOCOL["material"]["Emin"]
-    E = torch.from_numpy(harness.to_author(emin + (1 - emin) * data["rho"] ** 3, shape)).cuda()
-    mask = torch.from_numpy(harness.to_author(data["mask"], shape, True)).cuda()
+    E = torch.from_numpy(emin + (1 - emin) * data["rho"] ** 3).cuda()
+    mask = torch.from_numpy(data["mask"]).cuda()
     m = ops.Mask.from_values(mask, shape)
     blocks = vk.Modal(ke).blocks
     u = torch.from_numpy(np.random.default_rng(3).standard_normal(mask.numel())).cuda()

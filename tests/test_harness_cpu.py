@@ -14,22 +14,40 @@ def _harness():
 
 
 @pytest.mark.parametrize("name", ["smoke-q4.npz", "optimized-q64.npz", "optimized-q96.npz"])
-def test_regeneration_is_exact_at_factor_one(golden, name):
+def test_boundary_data_is_the_fixtures_at_factor_one(golden, name):
     h = _harness()
     d = dict(np.load(golden / name))
-    mask, force = h.regenerate(d["shape"])
-    assert np.array_equal(mask, d["mask"]) and np.array_equal(force, d["force"])
-    up = h.upsample(d, 1)
-    assert np.array_equal(up["rho"], d["rho"]) and np.array_equal(up["force"], d["force"])
+    shape = [int(s) for s in d["shape"]]
+    mask, force = h.cantilever_boundary(shape)
+    assert np.array_equal(mask, h.to_author(d["mask"], shape, True))
+    assert np.array_equal(force, h.to_author(d["force"], shape, True))
+    assert np.array_equal(h.refine_design(d["rho"], shape, 1), h.to_author(d["rho"], shape))
 
 
-def test_upsampling_preserves_design(golden):
+@pytest.mark.parametrize("case", ["q4x2", "q64x2", "q96x3"])
+def test_study_inputs_equal_the_oracle_regeneration_bitwise(golden, case):
+    """load_case (author layout, built directly) against the oracle's natural-layout restatement of
+    experiments/session_i/make_inputs.py:31-50 transposed to the author layout."""
+    from oracle import simpref as sr
+    h = _harness()
+    base, factor = h.CASES[case]
+    d = dict(np.load(golden / base))
+    ref = sr.upsample(d, factor)
+    got = h.load_case(golden, case)
+    shape = [int(s) for s in ref["shape"]]
+    assert list(got["shape"]) == shape
+    assert np.array_equal(got["rho"], h.to_author(ref["rho"], shape))
+    assert np.array_equal(got["mask"], h.to_author(ref["mask"], shape, True))
+    assert np.array_equal(got["force"], h.to_author(ref["force"], shape, True))
+
+
+def test_refinement_preserves_the_design(golden):
     h = _harness()
     d = dict(np.load(golden / "optimized-q64.npz"))
-    up = h.upsample(d, 2)
-    nx, ny, nz = d["shape"]
-    r = up["rho"].reshape(2 * nz, 2 * ny, 2 * nx)
-    assert np.array_equal(r[1::2, ::2, 1::2], d["rho"].reshape(nz, ny, nx))
+    up = h.load_case(golden, "q64x2")
+    nx, ny, nz = (int(s) for s in d["shape"])
+    r = up["rho"].reshape(2 * nx, 2 * ny, 2 * nz)
+    assert np.array_equal(r[1::2, ::2, 1::2], h.to_author(d["rho"], (nx, ny, nz)).reshape(nx, ny, nz))
     assert abs(up["force"].sum() + 1) < 1e-12 and up["mask"].size == 3 * (2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1)
     assert list(up["shape"]) == [2 * nx, 2 * ny, 2 * nz]
 

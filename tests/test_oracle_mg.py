@@ -240,6 +240,59 @@ def test_donor_chebyshev_variant_is_a_different_polynomial():
     assert np.abs(a - b).max() > 1e-2
 
 
+def test_donor_chebyshev_variant_has_the_hand_derived_degree_two_error():
+    """Pins mgref.chebyshev(variant="donor") by the error polynomial of the donor's d-form
+    (multigrid_v4.py:874-886) worked out by hand for a diagonal operator (D = I, A = lam,
+    rhs = lam, x0 = 0, exact solution 1).  Step 1: x1 = z0/theta, e1 = 1 - lam/theta, and the
+    direction d0 = z0 = lam is kept unscaled.  Step 2: c = delta^2 a0/4 with a0 = 1/theta,
+    q = theta - c, a1 = 1/q, d1 = a1 (lam e1 + c lam), so
+        e2 = (1 - lam/theta)(1 - lam/q) - lam c/q.
+    Saad's a0 = 2/theta (the first coefficient the donor's comment names) gives a different e2."""
+    lmax, frac = 2.0, 1 / 30
+    lam = np.linspace(0.0, 2.2, 57)
+    theta, delta = 0.5 * lmax * (1 + frac), 0.5 * lmax * (1 - frac)
+    c = delta * delta / (4 * theta)
+    q = theta - c
+    e2 = (1 - lam / theta) * (1 - lam / q) - lam * c / q
+    x = mg.chebyshev(lambda v: lam * v, lam.copy(), np.zeros_like(lam), np.ones_like(lam), lmax, frac, 2,
+                     variant="donor")
+    assert np.abs((1 - x) - e2).max() < 1e-13
+    c_saad = delta * delta / (2 * theta)
+    e2_saad = (1 - lam / theta) * (1 - lam / (theta - c_saad)) - lam * c_saad / (theta * (theta - c_saad))
+    assert np.abs(e2 - e2_saad).max() > 1e-2
+    xs = mg.chebyshev(lambda v: lam * v, lam.copy(), np.zeros_like(lam), np.ones_like(lam), lmax, frac, 2)
+    assert np.abs((1 - xs) - e2_saad).max() < 1e-13
+
+
+def test_donor_chebyshev_variant_on_a_diagonal_operator_with_jacobi_scaling():
+    """Degree 3 on a diagonal operator with a non-trivial D: the donor's variant equals a
+    scalar-by-scalar evaluation of the same d-form in exact rational arithmetic."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(5)
+    a = rng.uniform(0.5, 3.0, 12)          # diagonal of A
+    dscale = rng.uniform(0.8, 1.2, 12)     # D^-1 = dscale / a, so D^-1 A = dscale
+    rhs = rng.standard_normal(12)
+    inv = dscale / a
+    lmax, frac, deg = 1.25, 0.1, 3
+    got = mg.chebyshev(lambda v: a * v, rhs, np.zeros(12), inv, lmax, frac, deg, variant="donor")
+    lm, lo = F(lmax), F(lmax) * F(frac)
+    sig, dl = (lm + lo) / 2, (lm - lo) / 2
+    for i in range(12):
+        ai, bi, wi = F(a[i]), F(rhs[i]), F(inv[i])
+        x = F(0)
+        z = wi * (bi - ai * x)
+        alpha = 2 / (lm + lo)
+        dvec = z
+        x = x + alpha * dvec
+        for _ in range(1, deg):
+            z = wi * (bi - ai * x)
+            alpha_new = 1 / (sig - dl * dl * alpha / 4)
+            dvec = alpha_new * (z + dl * dl * alpha / 4 * dvec)
+            x = x + dvec
+            alpha = alpha_new
+        assert abs(float(x) - got[i]) <= 1e-13 * max(1.0, abs(float(x)))
+
+
 def test_chebyshev_multigrid_converges(golden, KE):
     shape, E, mask, force = smoke(golden, q=8, seed=3)
     _, M = mg.build_hierarchy(shape, KE, E, mask, max_coarse_x=2, smoother="chebyshev", degree=3)

@@ -112,6 +112,96 @@ def test_oc_top88_invariants():
     assert np.allclose(xu, xu[0]) and abs(xu[0] - 0.45) < 1e-3
 
 
+def _interior_ratio_check(x, xn, B, lo, hi):
+    """For every element the bisection left strictly inside its box, x_new = x sqrt(B / lam) with
+    one common lam: x_new^2 / (x^2 B) is the same constant 1/lam (the sqrt step of the OC update,
+    eta = 1/2).  Returns the largest relative spread of that constant."""
+    inside = (xn > lo + 1e-9) & (xn < hi - 1e-9)
+    assert inside.sum() >= 10
+    k = xn[inside] ** 2 / (x[inside] ** 2 * B[inside])
+    return float((k.max() - k.min()) / k.mean())
+
+
+def _oc_top88_without_sqrt(x, dc, dv, volfrac, move, l2=1e9, tol=1e-3):
+    l1 = 0.0
+    while (l2 - l1) / (l1 + l2) > tol:
+        lmid = 0.5 * (l2 + l1)
+        x_new = np.clip(x * np.maximum(-dc / dv / lmid, 0.0), np.maximum(x - move, 0.0), np.minimum(x + move, 1.0))
+        if x_new.mean() > volfrac:
+            l1 = lmid
+        else:
+            l2 = lmid
+    return x_new
+
+
+def test_oc_top88_step_is_the_square_root_of_the_sensitivity_ratio():
+    rng = np.random.default_rng(11)
+    n = 400
+    x = rng.uniform(0.2, 0.6, n)
+    dc = -rng.uniform(0.5, 2.0, n)
+    dv = rng.uniform(0.8, 1.2, n)
+    B = -dc / dv
+    xn = sr.oc_top88(x, dc, dv, 0.4, 0.3)
+    assert _interior_ratio_check(x, xn, B, np.maximum(x - 0.3, 0), np.minimum(x + 0.3, 1)) < 1e-12
+    # a step that drops the square root fails the same check
+    bad = _oc_top88_without_sqrt(x, dc, dv, 0.4, 0.3)
+    inside = (bad > np.maximum(x - 0.3, 0) + 1e-9) & (bad < np.minimum(x + 0.3, 1) - 1e-9)
+    k = bad[inside] ** 2 / (x[inside] ** 2 * B[inside])
+    assert (k.max() - k.min()) / k.mean() > 0.1
+
+
+def test_projected_oc_step_is_the_square_root_of_the_sensitivity_ratio():
+    shape = (10, 6, 6)
+    rng = np.random.default_rng(12)
+    n = int(np.prod(shape))
+    filt = sr.ConeFilter(shape, 1.5)
+    rho = rng.uniform(0.2, 0.6, n)
+    dc, dv = -rng.uniform(0.5, 2.0, n), rng.uniform(0.8, 1.2, n) / n
+    move = 0.3
+    for beta in (1.0, 8.0):
+        new, _, lam, _ = sr.oc_update(rho, dc, dv, 0.4, move, filt, beta, 0.5, 1e-3)
+        lo, hi = np.maximum(rho - move, 1e-3), np.minimum(rho + move, 1.0)
+        assert _interior_ratio_check(rho, new, -dc / dv, lo, hi) < 1e-12
+        inside = (new > lo + 1e-9) & (new < hi - 1e-9)
+        # and the common constant is the returned multiplier
+        assert np.allclose(new[inside] ** 2 / (rho[inside] ** 2 * (-dc / dv)[inside]), 1.0 / lam, rtol=1e-12)
+
+
+def test_continuation_schedule_matches_the_donor_module(golden):
+    """Against schedule_at evaluated by the donor's own simp_r2.py (tests/golden/make_schedule_golden.py)."""
+    g = json.loads((golden / "schedule_golden.json").read_text())
+    for it, row in enumerate(g["default"], start=1):
+        assert list(sr.schedule_at(it)) == row
+    table = [tuple(r) for r in g["custom_table"]]
+    for it, row in enumerate(g["custom"], start=1):
+        assert list(sr.schedule_at(it, table)) == row
+
+
+def test_point_load_case_of_the_donor_cantilever_preset():
+    """cantilever_gpu_large (presets.py:146-152): 120x60x30 on 2 x 1 x 0.5, left face clamped, fy = -1 at
+    (2.0, 0.5, 0.25) -> node (120, 30, 15)."""
+    mask, force = sr.point_load_case((120, 60, 30), (2.0, 1.0, 0.5), (2.0, 0.5, 0.25), (0.0, -1.0, 0.0))
+    node = (120 * 61 + 30) * 31 + 15
+    assert force[3 * node + 1] == -1.0 and np.count_nonzero(force) == 1
+    m = mask.reshape(121, 61, 31, 3)
+    assert np.all(m[0] == 0) and np.all(m[1:] == 1)
+
+
+def test_top88_oracle_loop_keeps_the_volume_and_lowers_the_compliance():
+    shape = (12, 6, 6)
+    ke = 2 * hx.quadrature([1, 1, 1], 0.3)
+    mask, force = sr.point_load_case(shape, (2.0, 1.0, 1.0), (2.0, 0.5, 0.5), (0.0, -1.0, 0.0))
+    hist, x, rho = sr.top88_run(shape, ke, mask, force, steps=6, volfrac=0.3, move=0.15)
+    c = [h[0] for h in hist]
+    assert all(b < a for a, b in zip(c, c[1:]))
+    assert all(abs(h[2] - 0.3) < 2e-3 for h in hist)
+    assert all(h[1] <= 0.15 + 1e-15 for h in hist)
+    assert np.allclose(rho, sr.ConeFilter(shape, 1.5).forward(x))
+    # step 0 is the uniform design: compliance of E = 0.3^3 everywhere
+    _, c0, _ = sr.state_solve(shape, ke, mask, force, np.full(x.size, sr.modulus(0.3)))
+    assert c[0] == pytest.approx(c0, rel=1e-12)
+
+
 def test_sensitivity_matches_central_finite_differences():
     shape = (4, 2, 2)
     ke = 2 * hx.quadrature([1, 1, 1], 0.3)
@@ -158,14 +248,6 @@ def test_projection_closed_values_monotonicity_and_derivative(beta):
     assert np.allclose(sr.dproject(xm, beta), fd, rtol=1e-6, atol=1e-9)
     # the beta -> 0 limit is the identity
     assert np.abs(sr.project(x, 1e-6) - x).max() < 1e-12 and np.array_equal(sr.project(x, 0.0), x)
-
-
-def test_continuation_schedule_phases():
-    assert sr.schedule_at(1) == (1.5, 1.0, 0.20, 1.50, False)
-    assert sr.schedule_at(15) == (1.5, 1.0, 0.20, 1.50, False)
-    assert sr.schedule_at(16) == (3.5, 4.0, 0.15, 1.35, False)
-    assert sr.schedule_at(120) == (4.5, 32.0, 0.05, 1.20, True)
-    assert sr.schedule_at(400) == (4.5, 32.0, 0.05, 1.20, True)
 
 
 def test_projected_oc_invariants_and_uniform_case():
