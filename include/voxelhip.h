@@ -115,6 +115,16 @@ typedef struct vh_epilogue {
 int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y,
                     int nx, int ny, int nz, int mask_kind, const uint8_t* mask, const vh_epilogue* ep,
                     void* stream);
+/* The DOT pass of vh_matvec_fused (y = A u, or with y == NULL only the dot) with no workspace and
+ * no allocation: *pq (one FP64 device value) = u . A u.  The per-tile partial sums go to an
+ * array that the library holds in device memory for the life of the process (512 KiB per
+ * GPU, 65,536 tiles: grids whose default plan needs more tiles are rejected with -2).  The first
+ * call on a device looks up that device's copy of the array (no allocation then either).
+ * Calls that overlap on one GPU (two streams, two threads) race on the array: serialise them.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int vh_matvec_dot_noalloc(int dtype, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny,
+                          int nz, int mask_kind, const uint8_t* mask, double* pq, void* stream);
 /* 1 if the 72 block values have the voxel-cube pattern (8 distinct values; the form the
  * load-time combinations and the batched product require), 0 otherwise.  A query of the
  * kernel's coefficient form (host), with no counterpart in the reference. */

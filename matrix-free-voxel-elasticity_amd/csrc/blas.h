@@ -17,8 +17,11 @@ enum Slot : int { S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_BETA = 3, S_FLAG = 4, S_RZN
                   S_DONE = 10, S_TOL2 = 11, S_ITS = 12 };
 
 // Stores the `count` reduced values v into s[dst..] and evaluates the scalar recurrence `op`.
+// With OP_NONE only s[dst .. dst + count) is touched (s may then be a single output value);
+// the recurrences read and write the slots of a 16-value scalar block.
 __device__ inline void apply_scalar_op(double* s, const double* v, int count, int dst, int op) {
   for (int j = 0; j < count; ++j) s[dst + j] = v[j];
+  if (op == OP_NONE) return;
   const bool done = s[S_DONE] != 0.0;
   switch (op) {
     case OP_ALPHA: {         // alpha = rz / pq  (pq just written to s[dst])

@@ -259,6 +259,31 @@ static int device_ke(int dtype, const double* ke576, hipStream_t st, const void*
   return 0;
 }
 
+// ---- allocation-free DOT pass: the per-tile partial sums live in a library-owned device array.
+// Each GPU holds its own copy; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name is a host-side
+// stand-in), looked up once per device on the first call there and kept in g_scratch_addr.
+constexpr int kScratchTiles = 65536;     // 512 KiB of FP64 partial sums per device
+__device__ __align__(256) double scratch[kScratchTiles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_addr[kMaxDevices] = {};
+static std::mutex g_scratch_mutex;
+
+static int scratch_address(double** out) {
+  int dev = 0;
+  if (int r = hip_status(hipGetDevice(&dev), "hipGetDevice")) return r;
+  if (dev < 0 || dev >= kMaxDevices) return fail(-2, "vh_matvec_dot_noalloc: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* ptr = nullptr;
+    if (int r = hip_status(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return r;
+    if (ptr == nullptr) return fail(-1, "hipGetSymbolAddress(scratch): null address");
+    g_scratch_addr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
+
 extern "C" {
 
 int vh_abi_version(void) { return VH_ABI_VERSION; }
@@ -330,6 +355,26 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
 int vh_matvec(int dtype, int storage, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny,
               int nz, int mask_kind, const uint8_t* mask, void* stream) {
   return vh_matvec_fused(dtype, storage, blocks72, E, u, y, nx, ny, nz, mask_kind, mask, nullptr, stream);
+}
+
+int vh_matvec_dot_noalloc(int dtype, const double* blocks72, const void* E, const void* u, void* y, int nx, int ny,
+                          int nz, int mask_kind, const uint8_t* mask, double* pq, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype), "vh_matvec_dot_noalloc: dtype %d not supported", dtype);
+  VH_REQUIRE(blocks72 && pq, "vh_matvec_dot_noalloc: null pointer");
+  VH_REQUIRE(dims_ok(nx, ny, nz), "vh_matvec_dot_noalloc: bad grid %d x %d x %d", nx, ny, nz);
+  double k8[8];
+  const MatvecPlan pl = matvec_plan(dtype, dtype, cube_pattern(blocks72, k8), nx, ny, nz, 0, 0);
+  VH_REQUIRE(pl.n_tiles >= 1 && pl.n_tiles <= kScratchTiles,
+             "vh_matvec_dot_noalloc: %d tiles exceed the %d partial sums of the library's array", pl.n_tiles,
+             kScratchTiles);
+  double* partial = nullptr;
+  if (int r = scratch_address(&partial)) return r;
+  vh_epilogue ep{};
+  ep.mode = MODE_DOT;
+  ep.partial = partial;
+  // every one of the plan's n_tiles tiles writes its partial sum, so the array needs no clearing
+  if (int r = vh_matvec_fused(dtype, dtype, blocks72, E, u, y, nx, ny, nz, mask_kind, mask, &ep, stream)) return r;
+  return finish_launch(partial, pl.n_tiles, 1, pq, 0, OP_NONE, static_cast<hipStream_t>(stream));
 }
 
 int vh_dense_product(int dtype, const double* ke576, const void* E, const void* u, void* y, int nx, int ny, int nz,

@@ -58,6 +58,7 @@ SIGNATURES = {
     "vh_modal_blocks": [_vp, _d, _vp, _dp],
     "vh_matvec": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp],
     "vh_matvec_fused": [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Epilogue), _vp],
+    "vh_matvec_dot_noalloc": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp],
     "vh_matvec_tiles": [_i, _i, _vp, _i, _i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "vh_dense_product": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp],
     "vh_lanes24_product": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp],
