@@ -1,6 +1,7 @@
 """Benchmark of the voxel elasticity hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-baseline sample|full|none]
+                    [--no-batch] [--no-reference] [--simp-steps N]
 
 Workload (one "step" = one complete solve from x = 0 to the FP64 true relative residual
 ||b - A x|| / ||b|| <= 1e-8, setup excluded as in the reference's timing boundary):
@@ -10,13 +11,18 @@ Workload (one "step" = one complete solve from x = 0 to the FP64 true relative r
   flexible CG (5 levels, 2x coarsening, Chebyshev-Jacobi degree 3 on [lambda_max / 15, lambda_max],
   degree 2 on the levels from the third down as the donor's coarse_smooth_iters, dense coarsest), mixed
   precision: FP32 V-cycle inside the FP64 outer CG with FP64 true residual and FP64 dot
-  accumulation.  The plain FP64 solve of the same system is timed beside it.
+  accumulation.  The plain FP64 solve and the FP64 iterative refinement around the FP32 solve
+  of the same system are timed beside it; each solution's error is stated against the FP64
+  NumPy oracle (its constrained residual computed on the host).
 
-Printed JSON line: value = mixed-precision solves/s; extra fields carry the FP64 solve, the
-128 x 64 x 64 FP64 GMG solve, matvec GDOF/s at both sizes, the roofline of the dominant kernel
-(the FP64 parity product at 256 x 128 x 128, timed with HIP events on its launch stream), and
-the CPU baseline (the NumPy oracle on the host cores, a bounded sample: one flexible-CG
-iteration of the same solve, scaled by the GPU's iteration count).
+Printed JSON line: value = mixed-precision solves/s; extra fields carry the FP64 and refinement
+solves, the 128 x 64 x 64 FP64 GMG solve, a high-contrast solve with the full Galerkin hierarchy,
+the 48-problem batch, the 384 x 192 x 192 SIMP run, the reference's own study solves (q64x1, q64x2,
+q96x2) beside its published times, the product's GDOF/s under the automatic and the tuned launch
+plan at three sizes, the roofline of the dominant kernel (the FP64 parity product at 256 x 128 x 128,
+tuned plan, HIP events on its launch stream), and the CPU baseline (the NumPy oracle on the host
+cores: a bounded sample of the same solve, or with --cpu-baseline full the products, the 32x16x16
+Jacobi-PCG and the 128x64x64 GMG-FCG solves in two thread settings).
 """
 from __future__ import annotations
 
@@ -132,12 +138,63 @@ def simp_leg(ke, shape=(384, 192, 192), steps=100, seed=5):
     hist, rho = simp_run(pb, steps=steps, volfrac=0.3, rmin=1.5, move=0.2)
     total = time.perf_counter() - t0
     its = [h["iterations"] for h in hist]
+    retried = [h["step"] for h in hist if h.get("retried")]
     return {"shape": list(shape), "ndof": int(force.numel()), "steps": len(hist), "warm_start": True,
+            "continued_bound_steps": pb.solver.bound_warm_iterations,
+            "warm_start_retries": len(retried), "retried_steps": retried,
             "seconds_total": total,
             "seconds_per_step": total / len(hist), "compliance_first": hist[0]["compliance"],
             "compliance_last": hist[-1]["compliance"], "volume_last": hist[-1]["volume"],
             "change_last": hist[-1]["change"], "solve_iterations_min_max_mean": [min(its), max(its),
                                                                                    float(np.mean(its))]}
+
+
+# the reference's own study solves (inputs regenerated from its base designs in tests/golden),
+# with its published times on other GPUs
+REFERENCE_WORKLOADS = {
+    "q64x1": {"design": "optimized-q64 (session D)", "omega_seed": 2026092320, "iterations": 13,
+              "published": {"fp64_s": 0.1037, "mixed_s": 0.0626, "gpu": "RTX 4090",
+                            "source": "paper/session_d_tables.md:26,29"}},
+    "q64x2": {"design": "optimized-q64 upsampled x2 (session I)", "omega_seed": 2026092621, "iterations": 14,
+              "published": {"fp64_s": 0.340, "mixed_s": 0.236, "gpu": "H100 80GB HBM3",
+                            "source": "paper/supplement/outcomes-scale-tables.md:217,219"}},
+    "q96x2": {"design": "optimized-q96 upsampled x2 (session I)", "omega_seed": 2026092621, "iterations": 24,
+              "published": {"fp64_s": 2.200, "mixed_s": 1.392, "gpu": "H100 80GB HBM3",
+                            "source": "paper/supplement/outcomes-scale-tables.md:221,226"}},
+}
+
+
+def reference_workloads(steps, warmup, graph, tuner):
+    """The reference's GMG-FCG study solves (voxel hierarchy halved to x <= 16, damped Jacobi
+    V(4,4) with omega = 4/(3 lambda) from 30 seeded power steps, dense coarsest, FP64 flexible CG
+    to a true relative residual of 1e-8 tested every iteration; Emin = 1e-6) on the engine, FP64
+    and with the FP32 V-cycle, timed like the reference (setup excluded)."""
+    import torch
+    from voxelhip import harness
+    from voxelhip.engine import GMGSolver
+    out = {}
+    for case, ref in REFERENCE_WORKLOADS.items():
+        data = harness.load_case(ROOT / "tests" / "golden", case)
+        shape = tuple(int(v) for v in data["shape"])
+        emin = harness.PROTOCOL["material"]["Emin"]
+        E = torch.from_numpy(emin + (1 - emin) * np.asarray(data["rho"]) ** 3).cuda()
+        mask = torch.from_numpy(np.asarray(data["mask"], dtype=np.float64)).cuda()
+        force = torch.from_numpy(np.asarray(data["force"])).cuda()
+        row = {"design": ref["design"], "shape": list(shape), "ndof": int(force.numel()),
+               "reference_iterations": ref["iterations"], "published": ref["published"]}
+        for prec in ("fp64", "mixed"):
+            s = GMGSolver(shape, data["ke"], mask, E, smoother="jacobi", pre=4, post=4, max_coarse_x=16,
+                          precision=prec, graph=graph, tune=tuner, omega_seed=ref["omega_seed"])
+            dt, st = timed_solves(s, force, steps, warmup)
+            pub = ref["published"][f"{prec}_s"]
+            row[prec] = {"seconds_per_solve": dt, "iterations": sorted({x[0] for x in st}),
+                         "converged": all(x[1] for x in st), "max_rel_residual": max(x[2] for x in st),
+                         "speedup_vs_published": pub / dt}
+            del s
+            torch.cuda.empty_cache()
+        row["mixed_speedup_vs_fp64"] = row["fp64"]["seconds_per_solve"] / row["mixed"]["seconds_per_solve"]
+        out[case] = row
+    return out
 
 
 def timed_solves(solver, b, steps, warmup):
@@ -155,8 +212,9 @@ def timed_solves(solver, b, steps, warmup):
     return dt, stats
 
 
-def matvec_timing(shape, dtype, reps=20):
-    """Average duration of the parity product, HIP events on the launch stream."""
+def matvec_timing(shape, dtype, reps=20, plan=None):
+    """Average duration of the parity product (raw K u), HIP events on the launch stream; plan =
+    (tile, lx) of the launch, None for the library's automatic plan."""
     import torch
     import voxel_kernels as vk
     from voxelhip import ops
@@ -167,22 +225,55 @@ def matvec_timing(shape, dtype, reps=20):
     E = torch.rand(ne, generator=g, device="cuda", dtype=dtype)
     us = [torch.randn(nd, generator=g, device="cuda", dtype=dtype) for _ in range(2)]
     ys = [torch.empty(nd, device="cuda", dtype=dtype) for _ in range(2)]
+    tile, lx = plan if plan is not None else (0, 0)
     for i in range(4):
-        ops.matvec(blocks, E, us[i % 2], ys[i % 2], shape)
+        ops.matvec(blocks, E, us[i % 2], ys[i % 2], shape, tile=tile, lx=lx)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     stream.synchronize()
     e0.record(stream)
     for i in range(reps):
-        ops.matvec(blocks, E, us[i % 2], ys[i % 2], shape)
+        ops.matvec(blocks, E, us[i % 2], ys[i % 2], shape, tile=tile, lx=lx)
     e1.record(stream)
     e1.synchronize()
     sec = e0.elapsed_time(e1) / 1e3 / reps
     word = 8 if dtype == torch.float64 else 4
     algo = word * (2 * nd + ne)
+    nt, nw = ops.matvec_tiles(shape, lx=lx, tile=tile, dtype=dtype, blocks=blocks)
     return {"shape": list(shape), "dtype": str(dtype).replace("torch.", ""), "us": sec * 1e6,
             "gdof_s": nd / sec / 1e9, "algorithmic_bytes": algo, "achieved_GBs": algo / sec / 1e9,
-            "frac_hbm": algo / sec / HBM_PEAK}
+            "frac_hbm": algo / sec / HBM_PEAK, "plan": {"tile": tile, "lx": lx, "workgroups": nw}}
+
+
+TILE_NAMES = {0: "auto", 1: "16x32", 2: "8x32", 3: "4x8", 4: "wide"}
+
+
+def tuned_matvec(shape, dtype, reps=20):
+    """The product under the launch plan the library's tuner would pick for it: every (tile, lx)
+    candidate of voxelhip.tuner timed briefly, the fastest timed again in full."""
+    from voxelhip import tuner
+    best = None
+    for tile, lx in tuner._candidates(shape[0]):
+        r = matvec_timing(shape, dtype, reps=5, plan=(tile, lx))
+        if best is None or r["us"] < best[0]:
+            best = (r["us"], (tile, lx))
+    r = matvec_timing(shape, dtype, reps=reps, plan=best[1])
+    r["plan"]["tile_name"] = TILE_NAMES.get(best[1][0], str(best[1][0]))
+    return r
+
+
+def oracle_residual(shape, ke, E, mask, force, x):
+    """||f - A x|| / ||f|| of a device solution, A the constrained operator of the NumPy oracle
+    (oracle/hexref.py NumpyProduct, FP64, host): the error statement of a GPU solution against
+    the FP64 oracle."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import hexref as hx
+    Eh = E.detach().cpu().numpy().astype(np.float64)
+    m = mask.detach().cpu().numpy().astype(np.float64)
+    f = force.detach().cpu().numpy().astype(np.float64)
+    u = x.detach().cpu().numpy().astype(np.float64)
+    Au = hx.NumpyProduct(shape, ke).raw(u * m, Eh) * m + u * (1 - m)
+    return float(np.linalg.norm(f - Au) / np.linalg.norm(f))
 
 
 def cpu_baseline(shape, E, mask, force, ke, lambdas, iterations):
@@ -214,11 +305,30 @@ def cpu_baseline(shape, E, mask, force, ke, lambdas, iterations):
     fine.raw(z * mask, E) * mask + z * (1 - mask)
     t_a = time.perf_counter() - t0
     solve_sec = (iterations + 1) * t_v + 2 * iterations * t_a
-    return {"value": 1.0 / solve_sec, "unit": "solves/s", "cores": os.cpu_count(), "kind": "port",
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+    return {"value": 1.0 / solve_sec, "unit": "solves/s", "cores": cores, "kind": "port",
             "sample": f"NumPy oracle, FP64 GMG-FCG on {shape[0]}x{shape[1]}x{shape[2]}: one V-cycle "
                       f"({t_v:.2f} s) and one constrained fine product ({t_a:.2f} s) timed, solve = "
                       f"(it+1) V-cycles + 2 it products at the GPU iteration count it = {iterations}; "
-                      f"BLAS threads = all host cores, np.add.at single-threaded"}
+                      f"BLAS threads = {cores}, np.add.at single-threaded"}
+
+
+def cpu_baseline_full():
+    """The CPU-baseline protocol (oracle/cpu_baseline.py: products at 128x64x64 and 256x128x128,
+    the 32x16x16 Jacobi-PCG solve, the 128x64x64 GMG-FCG solve; medians of 5 after a warm-up) in
+    two thread settings: the process's own (OMP_NUM_THREADS as set, else every host core) and 1.
+    Children run one after another, so each has the host to itself."""
+    import subprocess
+    runs = {}
+    own = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
+    for label, n in (("all", own), ("one", "1")):
+        env = dict(os.environ, OMP_NUM_THREADS=n, OPENBLAS_NUM_THREADS=n, MKL_NUM_THREADS=n)
+        r = subprocess.run([sys.executable, str(ROOT / "oracle" / "cpu_baseline.py")], env=env,
+                           capture_output=True, text=True, timeout=1800)
+        if r.returncode != 0:
+            raise RuntimeError(f"oracle/cpu_baseline.py failed: {r.stderr[-2000:]}")
+        runs[label] = json.loads(r.stdout.strip().splitlines()[-1])
+    return runs
 
 
 def main():
@@ -238,7 +348,14 @@ def main():
                          "solves of --precision (for a per-kernel trace of one solve)")
     ap.add_argument("--precision", choices=["mixed", "fp64", "both"], default="both",
                     help="precisions of the 256x128x128 solve legs")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="skip the reference's own study solves (q64x1, q64x2, q96x2)")
+    ap.add_argument("--cpu-baseline", choices=["sample", "full", "none"], default="sample",
+                    help="sample: the bounded C3 sample (about 10 s); full: the CPU-baseline protocol "
+                         "(oracle/cpu_baseline.py in two thread settings, a few minutes); none: skip")
     args = ap.parse_args()
+    if args.no_cpu_baseline:
+        args.cpu_baseline = "none"
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
@@ -249,7 +366,7 @@ def main():
     from voxelhip.tuner import LaunchTuner
     torch.cuda.set_device(0)
     if args.only == "matvec":
-        dom = matvec_timing((256, 128, 128), torch.float64)
+        dom = tuned_matvec((256, 128, 128), torch.float64)
         nd = 3 * 257 * 129 * 129
         src = torch.ones(nd, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -278,13 +395,25 @@ def main():
     if args.only == "solve" or args.precision != "both":
         print(json.dumps({"only": "solve", "solves": out}), flush=True)
         return
-    # solution error of the mixed solve against the FP64 solve
+    # the precision scheme C3 names: FP64 iterative refinement around the FP32 GMG-FCG
+    s_ir = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision="ir", graph=bool(args.graph),
+                     tune=tuner, lower_frac=1.0 / 15.0, coarse_degree=2, coarse_level=2)
+    dt, st = timed_solves(s_ir, force, max(1, args.steps // 2), 1)
+    out["ir"] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "inner_iterations": sorted({x[0] for x in st}),
+                 "outer_corrections": s_ir.last_outer, "converged": all(x[1] for x in st),
+                 "max_rel_residual": max(x[2] for x in st)}
+    solvers["ir"] = s_ir
+    # error of every solution against the FP64 oracle (the constrained residual of the NumPy
+    # oracle's product on the host) and against the FP64 GPU solve
     x64 = solvers["fp64"].solve(force)[0].clone()
-    x32 = solvers["mixed"].solve(force)[0]
-    out["mixed"]["rel_diff_vs_fp64_solution"] = float((x32 - x64).norm() / x64.norm())
+    for prec in ("mixed", "fp64", "ir"):
+        x = solvers[prec].solve(force)[0].clone()
+        out[prec]["oracle_rel_residual"] = oracle_residual(shape, ke, E, mask, force, x)
+        if prec != "fp64":
+            out[prec]["rel_diff_vs_fp64_solution"] = float((x - x64).norm() / x64.norm())
     lambdas = solvers["fp64"].M.lambdas
     iters64 = out["fp64"]["iterations"][0]
-    del solvers
+    del solvers, s_ir, x64, x
     torch.cuda.empty_cache()
     # the FP64 GMG-FCG of the 128 x 64 x 64 configuration
     shape2 = (128, 64, 64)
@@ -293,7 +422,8 @@ def main():
                    graph=bool(args.graph), tune=tuner, lower_frac=1.0 / 15.0)
     dt2, st2 = timed_solves(s2, f2, args.steps, args.warmup)
     out["c2_fp64"] = {"shape": list(shape2), "seconds_per_solve": dt2, "solves_per_s": 1 / dt2,
-                      "iterations": sorted({x[0] for x in st2}), "converged": all(x[1] for x in st2)}
+                      "iterations": sorted({x[0] for x in st2}), "converged": all(x[1] for x in st2),
+                      "oracle_rel_residual": oracle_residual(shape2, ke, E2, m2, f2, s2.solve(f2)[0])}
     del s2
     # high-contrast variant of the same grid (void, E = 1e-9, where rho <= 0.5): the rediscretised
     # hierarchy against the full Galerkin hierarchy, mixed precision, 200 iterations at most
@@ -317,12 +447,24 @@ def main():
     if args.simp_steps > 0:
         out["c5_simp"] = simp_leg(ke, steps=args.simp_steps)
         torch.cuda.empty_cache()
-    mv = {f"{d}@{'x'.join(map(str, s))}": matvec_timing(s, getattr(torch, d))
-          for s in ((256, 128, 128), (128, 64, 64), (384, 192, 192)) for d in ("float64", "float32")}
+    if not args.no_reference:
+        out["reference_workloads"] = reference_workloads(max(2, args.steps // 4), 1, bool(args.graph), tuner)
+        torch.cuda.empty_cache()
+    # the product under the library's automatic plan and under the tuned plan (the one the
+    # solvers launch); the roofline is that of the tuned FP64 product at 256 x 128 x 128
+    mv = {}
+    for sh in ((256, 128, 128), (128, 64, 64), (384, 192, 192)):
+        for d in ("float64", "float32"):
+            key = f"{d}@{'x'.join(map(str, sh))}"
+            mv[key] = tuned_matvec(sh, getattr(torch, d))
+            mv[key]["auto_plan_us"] = matvec_timing(sh, getattr(torch, d))["us"]
     dom = mv["float64@256x128x128"]
     cpu = None
-    if not args.no_cpu_baseline:
+    if args.cpu_baseline == "sample":
         cpu = cpu_baseline(shape, E, mask, force, ke, lambdas, iters64)
+    elif args.cpu_baseline == "full":
+        cpu = cpu_baseline(shape, E, mask, force, ke, lambdas, iters64)
+        cpu["protocol"] = cpu_baseline_full()
     line = {
         "metric": "CG solves/s at FP64-equiv rel.res 1e-8 (mixed vs FP64); matvec GDOF/s, %HBM",
         "value": out["mixed"]["solves_per_s"],
@@ -342,7 +484,8 @@ def main():
         "matvec": mv,
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": dom["frac_hbm"], "traffic": measured_traffic(),
-                     "kernel": "parity_xmarch<double> (FP64 product, 256x128x128)",
+                     "kernel": "parity_xmarch<double> (FP64 product, 256x128x128, tuned plan "
+                               f"{dom['plan']['tile_name']} tile, lx {dom['plan']['lx']})",
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "avg_launch_us": dom["us"]},
         "cpu_baseline": cpu,
     }

@@ -53,7 +53,8 @@ class GMGSolver:
     def __init__(self, shape, ke, mask, E=None, *, max_coarse_x=16, n_levels=None, smoother="jacobi",
                  degree=3, pre=4, post=4, precision="fp64", coarse="rediscretize", graph=False,
                  lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False,
-                 coarse_degree=None, coarse_level=2, fp32_direction=True, bound_warm_iterations=None):
+                 coarse_degree=None, coarse_level=2, fp32_direction=True, bound_warm_iterations=None,
+                 omega_seed=20260923):
         self.shape = tuple(int(s) for s in shape)
         self.ke = np.asarray(ke, dtype=np.float64)
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -89,6 +90,7 @@ class GMGSolver:
         self.graph = graph
         self.lower_frac = lower_frac
         self.omega_iterations = omega_iterations
+        self.omega_seed = int(omega_seed)
         self.mask = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape, self.dev)
         self.fine = None
         self.M = None
@@ -108,7 +110,7 @@ class GMGSolver:
         self.fine, self.M = build_hierarchy(self.shape, self.ke, self.mask, E64, self.shapes, self.pdtype,
                                             smoother=self.smoother, degree=self.degree, pre=self.pre,
                                             post=self.post, lower_frac=self.lower_frac,
-                                            omega_iterations=self.omega_iterations,
+                                            omega_iterations=self.omega_iterations, omega_seed=self.omega_seed,
                                             coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin,
                                             stencils=stencils, coarse_degree=self.coarse_degree,
                                             coarse_level=self.coarse_level, **self._bound_args())
@@ -402,7 +404,7 @@ class GMGSolver:
 def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev", degree=3, pre=4, post=4,
                     lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None,
                     stencils=None, coarse_degree=None, coarse_level=2, bound_start=None, bound_keep=None,
-                    bound_iterations=30):
+                    bound_iterations=30, omega_seed=20260923):
     """The FP64 fine Level and the Multigrid (levels in pdtype, rediscretised ke * 2^l, 2x2x2
     modulus averaging, injected masks, explicit inverse of the coarsest operator, spectral
     omegas / lambdas) of one problem.  coarse_matrix: a precomputed coarsest operator (Galerkin);
@@ -437,7 +439,7 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
         M.lambdas = mg.chebyshev_bounds(None, M.levels, iterations=bound_iterations, start=bound_start,
                                         keep=bound_keep)
     else:
-        M.omegas, M.lambdas = mg.spectral_omegas(None, M.levels, iterations=omega_iterations)
+        M.omegas, M.lambdas = mg.spectral_omegas(None, M.levels, iterations=omega_iterations, seed=omega_seed)
     return fine, M
 
 
