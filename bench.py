@@ -403,17 +403,26 @@ def main():
                  "outer_corrections": s_ir.last_outer, "converged": all(x[1] for x in st),
                  "max_rel_residual": max(x[2] for x in st)}
     solvers["ir"] = s_ir
+    # BF16-storage ablation: the mixed solve with the finest level's smoothing and residual passes
+    # reading the V-cycle iterate through a bfloat16 copy
+    s_bf = GMGSolver(shape, ke, mask, E, smoother="chebyshev", degree=3, precision="mixed", graph=bool(args.graph),
+                     tune=tuner, lower_frac=1.0 / 15.0, coarse_degree=2, coarse_level=2, vcycle_storage="bf16")
+    dt, st = timed_solves(s_bf, force, max(1, args.steps // 2), 1)
+    out["mixed_bf16_storage"] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt,
+                                 "iterations": sorted({x[0] for x in st}), "converged": all(x[1] for x in st),
+                                 "max_rel_residual": max(x[2] for x in st)}
+    solvers["mixed_bf16_storage"] = s_bf
     # error of every solution against the FP64 oracle (the constrained residual of the NumPy
     # oracle's product on the host) and against the FP64 GPU solve
     x64 = solvers["fp64"].solve(force)[0].clone()
-    for prec in ("mixed", "fp64", "ir"):
+    for prec in ("mixed", "fp64", "ir", "mixed_bf16_storage"):
         x = solvers[prec].solve(force)[0].clone()
         out[prec]["oracle_rel_residual"] = oracle_residual(shape, ke, E, mask, force, x)
         if prec != "fp64":
             out[prec]["rel_diff_vs_fp64_solution"] = float((x - x64).norm() / x64.norm())
     lambdas = solvers["fp64"].M.lambdas
     iters64 = out["fp64"]["iterations"][0]
-    del solvers, s_ir, x64, x
+    del solvers, s_ir, s_bf, x64, x
     torch.cuda.empty_cache()
     # the FP64 GMG-FCG of the 128 x 64 x 64 configuration
     shape2 = (128, 64, 64)

@@ -250,8 +250,10 @@ int vh_xpby(int dtype, size_t n, const void* z, void* p, const double* scal, voi
 /* y = a x (w) + b y with host scalars (w optional, elementwise). */
 int vh_axpby(int dtype, size_t n, double a, const void* x, const void* w, double b, void* y, void* stream);
 /* y = x * s (accumulate: y += x * s), s = scale or *dscale (device), inverted if inv_scale;
- * converts between float64 and float32.  Replaces: the FP32 cast of the scaled residual and
- * the FP64 update of donor pcg_ir simp_r2.py:372-379. */
+ * converts between float64 and float32, or float32 to bfloat16 (dtype_out VH_BF16: rounded to
+ * nearest even, no accumulate; the BF16 storage of a V-cycle iterate).  Replaces: the FP32 cast
+ * of the scaled residual and the FP64 update of donor pcg_ir simp_r2.py:372-379, and the BF16
+ * cast of the fine smoother's input in donor_yang_gmg multigrid_v4.py:851-856. */
 int vh_convert(int dtype_in, int dtype_out, size_t n, const void* x, void* y, double scale, const double* dscale,
                int inv_scale, int accumulate, void* stream);
 

@@ -69,6 +69,8 @@ template <typename T> int xpby_launch(size_t n, const T* z, T* p, const double* 
 template <typename T> int axpby_launch(size_t n, T a, const T* x, const T* w, T b, T* y, hipStream_t st);
 template <typename Ti, typename To> int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dscale,
                                                        int inv_scale, int accumulate, hipStream_t st);
+int convert_bf16_launch(size_t n, const float* x, uint16_t* y, double scale, const double* dscale, int inv_scale,
+                        hipStream_t st);
 template <typename Ti, typename Tv>
 int cvt_dotdiff_launch(size_t n, const Ti* zin, const Tv* r, const Tv* zold, Tv* znew, double* partial, hipStream_t st);
 template <typename TA, typename TV>

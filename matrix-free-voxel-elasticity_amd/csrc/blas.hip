@@ -153,6 +153,17 @@ __global__ void __launch_bounds__(BLAS_BLOCK) convert_kernel(size_t n, const Ti*
   }
 }
 
+// y = bfloat16(x * scale), rounded to nearest even: BF16 storage of an FP32 vector.
+__global__ void __launch_bounds__(BLAS_BLOCK) convert_bf16_kernel(size_t n, const float* __restrict__ x,
+                                                                 uint16_t* __restrict__ y, double scale,
+                                                                 const double* __restrict__ dscale, int inv_scale) {
+  double sc = dscale ? *dscale : scale;
+  if (inv_scale) sc = 1.0 / sc;
+  const float s = float(sc);
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK)
+    y[i] = f32_to_bf16_rne(s == 1.0f ? x[i] : x[i] * s);
+}
+
 // z_new = (double) z_in; partials of r.z_new and r.(z_new - z_old): the preconditioned vector
 // of a flexible CG iteration taken from the V-cycle output (any precision) with both
 // Polak-Ribiere dots in the same pass.
@@ -287,6 +298,12 @@ int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dsc
   hipLaunchKernelGGL((convert_kernel<Ti, To>), dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, x, y, scale, dscale,
                      inv_scale, accumulate);
   return hip_status(hipGetLastError(), "convert_kernel");
+}
+
+int convert_bf16_launch(size_t n, const float* x, uint16_t* y, double scale, const double* dscale, int inv_scale,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(convert_bf16_kernel, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, x, y, scale, dscale, inv_scale);
+  return hip_status(hipGetLastError(), "convert_bf16_kernel");
 }
 
 template <typename Ti, typename Tv>

@@ -622,8 +622,13 @@ int vh_axpby(int dtype, size_t n, double a, const void* x, const void* w, double
 
 int vh_convert(int dtype_in, int dtype_out, size_t n, const void* x, void* y, double scale, const double* dscale,
                int inv_scale, int accumulate, void* stream) {
-  VH_REQUIRE(dtype_ok(dtype_in) && dtype_ok(dtype_out) && x && y && n >= 1, "vh_convert: bad arguments");
+  VH_REQUIRE(dtype_ok(dtype_in) && (dtype_ok(dtype_out) || dtype_out == BF16) && x && y && n >= 1,
+             "vh_convert: bad arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype_out == BF16) {
+    VH_REQUIRE(dtype_in == F32 && !accumulate, "vh_convert: bfloat16 output from float32, without accumulate");
+    return convert_bf16_launch(n, static_cast<const float*>(x), static_cast<uint16_t*>(y), scale, dscale, inv_scale, st);
+  }
   if (dtype_in == F64 && dtype_out == F32)
     return convert_launch<double, float>(n, static_cast<const double*>(x), static_cast<float*>(y), scale, dscale, inv_scale, accumulate, st);
   if (dtype_in == F32 && dtype_out == F64)

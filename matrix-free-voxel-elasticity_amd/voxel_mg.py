@@ -229,8 +229,8 @@ class Level:
             plan = self._plan(kw.get("mode", L.MODE_RAW), kw.get("w") is not None)
             if plan is not None:
                 kw["tile"], kw["lx"] = plan
-        return ops.matvec(self.product.blocks, self.E, u, y, self.shape, mask=self.maskobj,
-                          storage=self.product.storage, **kw)
+        storage = kw.pop("storage", self.product.storage)
+        return ops.matvec(self.product.blocks, self.E, u, y, self.shape, mask=self.maskobj, storage=storage, **kw)
 
     def tune(self, tuner, passes):
         """Empirical launch plans for the given (mode, combine) pass kinds (voxelhip.tuner)."""
@@ -255,8 +255,17 @@ class Multigrid:
 
     def __init__(self, xp, shapes, products, E_fine, mask_fine, coarse_factor, dtype,
                  pre=4, post=4, omegas=None, smoother="jacobi", lambdas=None,
-                 lower_frac=1.0 / 30.0, cheb_variant="saad", coarse_dtype=None, coarse_sweeps=None, coarse_from=2):
+                 lower_frac=1.0 / 30.0, cheb_variant="saad", coarse_dtype=None, coarse_sweeps=None, coarse_from=2,
+                 storage=None):
         self.xp, self.pre, self.post = xp, pre, post
+        # storage="bf16" (FP32 V-cycle): the finest level's smoothing and residual passes read their
+        # iterate through a bfloat16 copy, the BF16 storage of displacements (the donor's bf16
+        # fine smoother, donor_yang_gmg multigrid_v4.py:851-856); arithmetic stays FP32
+        if storage not in (None, "bf16"):
+            raise ValueError(f"storage: {storage!r}")
+        if storage == "bf16" and _tdt(dtype) != torch.float32:
+            raise ValueError("bf16 storage of the iterate needs an FP32 V-cycle")
+        self.storage = storage
         # levels >= coarse_from smooth with coarse_sweeps steps instead of pre / post (the donor's
         # separate coarse smoothing count, coarse_smooth_iters of multigrid_v4.py:466,934-967)
         self.coarse_sweeps, self.coarse_from = coarse_sweeps, int(coarse_from)
@@ -296,6 +305,10 @@ class Multigrid:
         for lv in self.levels:
             z = lambda: torch.zeros(lv.n, dtype=self.dtype, device=dev)  # noqa: E731
             self._work.append({"x": z(), "x2": z(), "res": z(), "d": z(), "rhs": z()})
+        if storage == "bf16":
+            if not (self.levels[0].fused and not self.levels[0].stencil):
+                raise ValueError("bf16 storage needs the parity product on the finest level")
+            self._xb = torch.zeros(self.levels[0].n, dtype=torch.bfloat16, device=dev)
 
     def tune(self, tuner):
         """Launch plans of every smoothing level's passes (smoother and residual)."""
@@ -307,6 +320,15 @@ class Multigrid:
             lv.tune(tuner, passes)
 
     # ---------------------------------------------------------------- smoothing
+    def _pass(self, level, u, y, **kw):
+        """One fused pass of `level` on the iterate u (through its bfloat16 copy with storage="bf16"
+        on the finest level)."""
+        Lv = self.levels[level]
+        if self.storage == "bf16" and level == 0 and kw.get("w") is None:
+            ops.convert(u, self._xb)
+            return Lv.matvec(self._xb, y, storage=L.BF16, **kw)
+        return Lv.matvec(u, y, **kw)
+
     def _dinv_kw(self, Lv):
         """D^-1 formed from the moduli inside the pass when ke allows it, else the stored array."""
         kd = getattr(Lv.product, "kdiag", None)
@@ -327,9 +349,9 @@ class Multigrid:
 
         def last_pass(xin, partner, **kw):
             if dots is not None:
-                Lv.matvec(xin, out, mode=L.MODE_CHEB3D, b=r, prev=partner, partial=dots[0], zold=dots[1], **kw)
+                self._pass(level, xin, out, mode=L.MODE_CHEB3D, b=r, prev=partner, partial=dots[0], zold=dots[1], **kw)
             else:
-                Lv.matvec(xin, out, mode=L.MODE_CHEB3, b=r, prev=partner, **kw)
+                self._pass(level, xin, out, mode=L.MODE_CHEB3, b=r, prev=partner, **kw)
             return out
 
         if self.smoother == "chebyshev":
@@ -354,7 +376,7 @@ class Multigrid:
             elif out is not None and sweeps == 1:
                 return last_pass(x, other, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
             else:
-                Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
+                self._pass(level, x, other, mode=L.MODE_CHEB3, b=r, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
                 x, other = other, x
             for k in range(start, sweeps):
                 c = 0.25 * delta * delta * alpha
@@ -362,7 +384,8 @@ class Multigrid:
                 cd = c * theta if (self.cheb_variant == "donor" and k == 1) else c
                 if out is not None and k == sweeps - 1:
                     return last_pass(x, other, cheb_a=alpha, cheb_c=cd, prev_zero=prev_zero, **dk)
-                Lv.matvec(x, other, mode=L.MODE_CHEB3, b=r, cheb_a=alpha, cheb_c=cd, prev_zero=prev_zero, **dk)
+                self._pass(level, x, other, mode=L.MODE_CHEB3, b=r, cheb_a=alpha, cheb_c=cd, prev_zero=prev_zero,
+                           **dk)
                 prev_zero = False
                 x, other = other, x
             return x
@@ -371,7 +394,7 @@ class Multigrid:
             ops.axpby(om, r, 0.0, x, w=Lv.inv_diag)
             sweeps -= 1
         for _ in range(sweeps):
-            Lv.matvec(x, other, mode=L.MODE_JACOBI, b=r, omega=om, **dk)
+            self._pass(level, x, other, mode=L.MODE_JACOBI, b=r, omega=om, **dk)
             x, other = other, x
         return x
 
@@ -394,7 +417,7 @@ class Multigrid:
             return self.coarse_solve(r, w["x"] if out is None else out)
         pre, post = self._sweeps(level)
         x = self._smooth(level, w["x"], r, pre, True)
-        Lv.matvec(x, w["res"], mode=L.MODE_RESID, b=r, fixed_zero=True)      # (r - A x) M
+        self._pass(level, x, w["res"], mode=L.MODE_RESID, b=r, fixed_zero=True)      # (r - A x) M
         nxt = self.levels[level + 1]
         rc = self._work[level + 1]["rhs"]
         ops.restrict(w["res"], rc, Lv.shape, cmask=nxt.maskobj)

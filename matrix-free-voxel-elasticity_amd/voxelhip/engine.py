@@ -27,6 +27,11 @@ precision:
 coarse_degree / coarse_level: smoothing steps on the levels from coarse_level down (the donor's
   coarse_smooth_iters, multigrid_v4.py:466,934-967); None keeps degree / pre / post there.
 
+vcycle_storage="bf16" (mixed / ir): the finest level's smoothing and residual passes read the
+  V-cycle iterate through a bfloat16 copy (FP32 arithmetic): the BF16-storage ablation.
+
+omega_seed: the seed of the power iteration behind the Jacobi omegas (src/voxel_mg.py:228-244).
+
 bound_warm_iterations: for a sequence of rebuilds with slowly changing moduli (a SIMP run), the
   Chebyshev bounds of each rebuild after the first continue the previous rebuild's power iteration
   for this many steps instead of 30 steps from the seeded vector (None: always from the seed).
@@ -54,7 +59,7 @@ class GMGSolver:
                  degree=3, pre=4, post=4, precision="fp64", coarse="rediscretize", graph=False,
                  lower_frac=1.0 / 30.0, omega_iterations=30, inner_tol=1e-3, coarse_fp64=True, tune=False,
                  coarse_degree=None, coarse_level=2, fp32_direction=True, bound_warm_iterations=None,
-                 omega_seed=20260923):
+                 omega_seed=20260923, vcycle_storage=None):
         self.shape = tuple(int(s) for s in shape)
         self.ke = np.asarray(ke, dtype=np.float64)
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -91,6 +96,9 @@ class GMGSolver:
         self.lower_frac = lower_frac
         self.omega_iterations = omega_iterations
         self.omega_seed = int(omega_seed)
+        if vcycle_storage not in (None, "bf16") or (vcycle_storage and precision == "fp64"):
+            raise ValueError("vcycle_storage='bf16' is for the FP32 V-cycle of the mixed and ir precisions")
+        self.vcycle_storage = vcycle_storage
         self.mask = mask if isinstance(mask, ops.Mask) else ops.Mask.from_values(mask, self.shape, self.dev)
         self.fine = None
         self.M = None
@@ -111,6 +119,7 @@ class GMGSolver:
                                             smoother=self.smoother, degree=self.degree, pre=self.pre,
                                             post=self.post, lower_frac=self.lower_frac,
                                             omega_iterations=self.omega_iterations, omega_seed=self.omega_seed,
+                                            storage=self.vcycle_storage,
                                             coarse_dtype=self.coarse_dtype, coarse_matrix=galerkin,
                                             stencils=stencils, coarse_degree=self.coarse_degree,
                                             coarse_level=self.coarse_level, **self._bound_args())
@@ -404,7 +413,7 @@ class GMGSolver:
 def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev", degree=3, pre=4, post=4,
                     lower_frac=1.0 / 30.0, omega_iterations=30, coarse_dtype=None, coarse_matrix=None,
                     stencils=None, coarse_degree=None, coarse_level=2, bound_start=None, bound_keep=None,
-                    bound_iterations=30, omega_seed=20260923):
+                    bound_iterations=30, omega_seed=20260923, storage=None):
     """The FP64 fine Level and the Multigrid (levels in pdtype, rediscretised ke * 2^l, 2x2x2
     modulus averaging, injected masks, explicit inverse of the coarsest operator, spectral
     omegas / lambdas) of one problem.  coarse_matrix: a precomputed coarsest operator (Galerkin);
@@ -432,7 +441,7 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
         kw = dict(smoother="chebyshev", pre=degree, post=degree, lower_frac=lower_frac)
     else:
         kw = dict(pre=pre, post=post)
-    kw.update(coarse_sweeps=coarse_degree, coarse_from=coarse_level)
+    kw.update(coarse_sweeps=coarse_degree, coarse_from=coarse_level, storage=storage)
     M = mg.Multigrid(None, shapes, prods, E64, mask, Lc, pdtype, coarse_dtype=coarse_dtype, **kw)
     # Jacobi: the reference's spectral omegas; Chebyshev: safe upper bounds of the spectrum
     if smoother == "chebyshev":

@@ -12,7 +12,7 @@ import torch
 
 from . import _lib as L
 
-_DT = {torch.float64: L.F64, torch.float32: L.F32}
+_DT = {torch.float64: L.F64, torch.float32: L.F32, torch.bfloat16: L.BF16}
 
 
 def _p(t):

@@ -175,3 +175,40 @@ def test_continued_power_iteration_bounds(problem):
     x, it, ok, r, _ = s.solve(torch.from_numpy(force).cuda())
     assert ok and r <= 1e-8 and it <= ito + 1
     assert rel(x, xo) < 1e-7
+
+
+def test_bf16_conversion_rounds_to_nearest_even():
+    """vh_convert float32 -> bfloat16 equals torch's round-to-nearest-even cast bit for bit."""
+    from voxelhip import ops
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randn(100003, generator=g, device="cuda", dtype=torch.float32) * 1e3
+    x[:4] = torch.tensor([1.0, -0.0, 3.0e38, 1.00390625], device="cuda")   # (the last is a tie)
+    y = torch.empty(x.numel(), dtype=torch.bfloat16, device="cuda")
+    ops.convert(x, y)
+    assert torch.equal(y.view(torch.int16), x.to(torch.bfloat16).view(torch.int16))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "ir"])
+def test_bf16_storage_vcycle_reaches_the_fp64_tolerance(precision):
+    """The BF16-storage ablation (the finest level's smoothing and residual passes read the
+    V-cycle iterate in bfloat16): the outer FP64 loop still reaches 1e-8 on the oracle's residual,
+    within a few iterations of the FP32-storage solve."""
+    from oracle import hexref as hx
+    from oracle import simpref as sr
+    from voxelhip.engine import GMGSolver
+    shape = (32, 16, 16)
+    ke = 2 * hx.quadrature([1, 1, 1], 0.3)
+    _, E, mask, force = sr.cantilever(shape, seed=3, rmin=1.5)
+    f = torch.from_numpy(force).cuda()
+    out = {}
+    for storage in (None, "bf16"):
+        print("case", precision, storage, flush=True)
+        s = GMGSolver(shape, ke, mask, torch.from_numpy(E).cuda(), max_coarse_x=8, smoother="chebyshev", degree=3,
+                      precision=precision, vcycle_storage=storage)
+        x, it, ok, rel, _ = s.solve(f, tol=1e-8, maxiter=100)
+        u = x.cpu().numpy()
+        Au = hx.NumpyProduct(shape, ke).raw(u * mask, E) * mask + u * (1 - mask)
+        oracle_rel = np.linalg.norm(force - Au) / np.linalg.norm(force)
+        assert ok and rel <= 1e-8 and oracle_rel <= 1.01e-8, (storage, rel, oracle_rel)
+        out[storage] = it
+    assert out["bf16"] <= out[None] + 4
