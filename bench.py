@@ -250,16 +250,21 @@ TILE_NAMES = {0: "auto", 1: "16x32", 2: "8x32", 3: "4x8", 4: "wide"}
 
 def tuned_matvec(shape, dtype, reps=20):
     """The product under the launch plan the library's tuner would pick for it: every (tile, lx)
-    candidate of voxelhip.tuner timed briefly, the fastest timed again in full."""
+    candidate of voxelhip.tuner and the automatic plan timed briefly, the four fastest timed
+    again with `reps` launches each, and the fastest of those kept (the short first round alone
+    is at the mercy of run-to-run noise of a few per cent)."""
     from voxelhip import tuner
+    first = []
+    for plan in [(0, 0)] + list(tuner._candidates(shape[0])):
+        first.append((matvec_timing(shape, dtype, reps=5, plan=plan)["us"], plan))
+    first.sort()
     best = None
-    for tile, lx in tuner._candidates(shape[0]):
-        r = matvec_timing(shape, dtype, reps=5, plan=(tile, lx))
-        if best is None or r["us"] < best[0]:
-            best = (r["us"], (tile, lx))
-    r = matvec_timing(shape, dtype, reps=reps, plan=best[1])
-    r["plan"]["tile_name"] = TILE_NAMES.get(best[1][0], str(best[1][0]))
-    return r
+    for _, plan in first[:4]:
+        r = matvec_timing(shape, dtype, reps=reps, plan=plan)
+        if best is None or r["us"] < best["us"]:
+            best = r
+    best["plan"]["tile_name"] = TILE_NAMES.get(best["plan"]["tile"], str(best["plan"]["tile"]))
+    return best
 
 
 def oracle_residual(shape, ke, E, mask, force, x):
@@ -342,10 +347,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batched leg")
     ap.add_argument("--simp-steps", type=int, default=100,
                     help="SIMP steps of the 384x192x192 optimisation leg (0 skips it)")
-    ap.add_argument("--only", choices=["matvec", "solve"], default=None,
-                    help="matvec: only the dominant kernel (FP64 product at 256x128x128) and a plain "
-                         "copy of known size, for counter collection; solve: only the 256x128x128 "
-                         "solves of --precision (for a per-kernel trace of one solve)")
+    ap.add_argument("--only", choices=["matvec", "solve", "batch"], default=None,
+                    help="matvec: only the FP64 and FP32 products at 256x128x128 (plans from --mv-plan) "
+                         "and a plain copy of known size, for counter collection; solve: only the "
+                         "256x128x128 solves of --precision (for a per-kernel trace of one solve); batch: "
+                         "only the 48-problem batched solve (for a trace of its launches)")
+    ap.add_argument("--mv-plan", default="tuned",
+                    help="launch plan of the --only matvec products: 'tuned', or 'tile,lx' for both "
+                         "precisions, or 'tile,lx;tile,lx' for FP64;FP32 (tile 1 = 16x32, 2 = 8x32, 4 = wide)")
     ap.add_argument("--precision", choices=["mixed", "fp64", "both"], default="both",
                     help="precisions of the 256x128x128 solve legs")
     ap.add_argument("--no-reference", action="store_true",
@@ -366,14 +375,27 @@ def main():
     from voxelhip.tuner import LaunchTuner
     torch.cuda.set_device(0)
     if args.only == "matvec":
-        dom = tuned_matvec((256, 128, 128), torch.float64)
+        res = {}
+        plans = [None, None]
+        if args.mv_plan != "tuned":
+            parts = [tuple(int(v) for v in q.split(",")) for q in args.mv_plan.split(";")]
+            plans = [parts[0], parts[-1]]
+        for dt_, plan in zip((torch.float64, torch.float32), plans):
+            key = str(dt_).replace("torch.", "")
+            res[key] = (tuned_matvec((256, 128, 128), dt_) if plan is None
+                        else matvec_timing((256, 128, 128), dt_, plan=plan))
         nd = 3 * 257 * 129 * 129
         src = torch.ones(nd, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(5):          # plain copy of 2 x 102.6 MB: the counter calibration reference
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"only": "matvec", "matvec": dom, "copy_bytes_each_way": nd * 8}), flush=True)
+        print(json.dumps({"only": "matvec", "matvec": res, "copy_bytes_each_way": nd * 8}), flush=True)
+        return
+    if args.only == "batch":
+        ke = 2 * vk.quadrature([1, 1, 1], 0.3)
+        r = timed_batch(c4_problems(), ke, args.steps, args.warmup, graph=bool(args.graph), sequential=False)
+        print(json.dumps({"only": "batch", "c4_batched": r}), flush=True)
         return
     ke = 2 * vk.quadrature([1, 1, 1], 0.3)          # the donor's KE_UNIT_3D (ke_unit_3d.npy)
     # launch plans of every product pass, timed at set-up (outside the timed solves)
