@@ -214,6 +214,14 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
                           int* tiles_out) {
   std::vector<MatvecParams<T>> descs(n);
   std::vector<int> prefix(n + 1, 0);
+  // x chunk lengths planned for the batch as a whole (items that give their own lx keep it)
+  std::vector<int> bnx(n), bny(n), bnz(n), blx(n);
+  for (int i = 0; i < n; ++i) {
+    bnx[i] = items[i].nx;
+    bny[i] = items[i].ny;
+    bnz[i] = items[i].nz;
+  }
+  matvec_batch_chunks<T>(n, bnx.data(), bny.data(), bnz.data(), tile, blx.data());
   for (int i = 0; i < n; ++i) {
     const vh_product_item& it = items[i];
     MatvecParams<T>& p = descs[i];
@@ -244,7 +252,7 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
     p.fixed_zero = ep.fixed_zero;
     p.load_scale = T(ep.load_scale);
     p.prev = static_cast<const T*>(ep.prev);
-    const MatvecPlan pl = matvec_batch_plan<T>(it.nx, it.ny, it.nz, ep.lx, tile);
+    const MatvecPlan pl = matvec_batch_plan<T>(it.nx, it.ny, it.nz, ep.lx > 0 ? ep.lx : blx[i], tile);
     p.tile = pl.tile;
     p.lx = pl.lx;
     p.tiles_y = pl.tiles_y;

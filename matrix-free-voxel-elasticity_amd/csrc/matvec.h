@@ -104,5 +104,7 @@ struct MatvecBatch {
 };
 template <typename T> int matvec_batch_launch(const MatvecBatch& b, hipStream_t st);
 template <typename T> MatvecPlan matvec_batch_plan(int nx, int ny, int nz, int lx_hint, int tile);
+// x chunk lengths lx[i] of the n problems of one batched launch, chosen for the batch as a whole
+template <typename T> void matvec_batch_chunks(int n, const int* nx, const int* ny, const int* nz, int tile, int* lx);
 
 }  // namespace vh

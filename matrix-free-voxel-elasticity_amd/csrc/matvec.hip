@@ -34,7 +34,9 @@
 // This file is compiled twice, through matvec_f64.hip (VH_MATVEC_F64: the FP64 instantiations)
 // and matvec_f32.hip (VH_MATVEC_F32: FP32 with FP32 or BF16 storage), so the two halves build
 // in parallel.
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "common.h"
 #include "matvec.h"
@@ -770,13 +772,56 @@ MatvecPlan matvec_batch_plan(int nx, int ny, int nz, int lx_hint, int tile) {
   }
   return p;
 }
+
+// The problems of one batched launch share the chip: planned one by one, each small problem would
+// be cut into one-element x chunks to fill every CU by itself, recomputing a halo element plane
+// per owned one.  Here one chunk length L is chosen for the whole batch, minimising the list-
+// scheduling bound  (sum of workgroup plane steps) / (resident workgroup slots) + (longest
+// workgroup): a workgroup of a chunk of l planes takes l + 2 steps (the halo element plane and
+// the first node plane loaded before the march).  Problem i gets ceil(nx_i / L) chunks, spread
+// evenly over its nx_i element planes.
+template <typename T>
+void matvec_batch_chunks(int n, const int* nx, const int* ny, const int* nz, int tile, int* lx) {
+  if (n <= 0) return;
+  std::vector<long> per(n);
+  int max_nx = 1;
+  for (int i = 0; i < n; ++i) {
+    const MatvecPlan p = matvec_batch_plan<T>(nx[i], ny[i], nz[i], nx[i], tile);
+    per[i] = long(p.tiles_y) * p.tiles_z;
+    max_nx = std::max(max_nx, nx[i]);
+  }
+  const int occ = tile == MV_TILE_8x32 ? resident_batch<T, Tile8>() : resident_batch<T, Tile16>();
+  const double slots = double(device_cus()) * occ;
+  int best_l = max_nx;
+  double best = 1e300;
+  for (int L = 1; L <= max_nx; ++L) {
+    double work = 0.0;
+    int longest = 0;
+    for (int i = 0; i < n; ++i) {
+      const int c = (nx[i] + L - 1) / L, l = (nx[i] + c - 1) / c;
+      work += double(per[i]) * c * (l + 2);
+      longest = std::max(longest, l + 2);
+    }
+    const double cost = work / slots + longest;
+    if (cost < best - 1e-9) {
+      best = cost;
+      best_l = L;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const int c = (nx[i] + best_l - 1) / best_l;
+    lx[i] = (nx[i] + c - 1) / c;
+  }
+}
 #endif
 #if defined(VH_MATVEC_BATCH_F64)
 template int matvec_batch_launch<double>(const MatvecBatch&, hipStream_t);
 template MatvecPlan matvec_batch_plan<double>(int, int, int, int, int);
+template void matvec_batch_chunks<double>(int, const int*, const int*, const int*, int, int*);
 #elif defined(VH_MATVEC_BATCH_F32)
 template int matvec_batch_launch<float>(const MatvecBatch&, hipStream_t);
 template MatvecPlan matvec_batch_plan<float>(int, int, int, int, int);
+template void matvec_batch_chunks<float>(int, const int*, const int*, const int*, int, int*);
 #endif
 
 }  // namespace vh

@@ -122,3 +122,58 @@ def test_full_batch_workload_meets_the_tolerance_on_an_independent_residual():
             assert r <= 1.01e-8, (k, pb.shape, r)
             k += 1
     print("worst independent residual", worst, "iterations", its.min(), its.max(), flush=True)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_product_equals_single_products(dtype):
+    """One batched launch over problems of different sizes (x extents 8 ... 96, so that the
+    batch-wide x chunking differs from each problem's own plan), for the raw product and the
+    three-term Chebyshev step: the automatic chunk length and fixed ones (1, 5) give bitwise the
+    same result (the element arithmetic and the gather order do not depend on the plan), and it
+    matches the single-problem kernel to rounding (the two kernels are compiled separately, and
+    the compiler may fuse a different set of multiply-adds: 1e-13 of max |y| in FP64, 1e-5 in
+    FP32)."""
+    from voxelhip import _lib as L
+    from voxelhip import ops
+    from voxelhip.batch import _epilogue, product_batch
+    print("case batched product", dtype, flush=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    shapes = [(96, 16, 24), (8, 8, 8), (40, 24, 40), (16, 48, 16), (72, 8, 16)]
+    blocks = np.ascontiguousarray(ops.modal_blocks(KE)[0], dtype=np.float64)
+    keep = blocks.reshape(72)
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    for mode in (L.MODE_RAW, L.MODE_CHEB3):
+        data = []
+        for s in shapes:
+            nd = 3 * (s[0] + 1) * (s[1] + 1) * (s[2] + 1)
+            E = (0.01 + torch.rand(s[0] * s[1] * s[2], generator=g, device="cuda", dtype=torch.float64)).to(dtype)
+            u = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64).to(dtype)
+            b = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64).to(dtype)
+            prev = torch.randn(nd, generator=g, device="cuda", dtype=torch.float64).to(dtype)
+            ep = dict(b=b, kdiag=np.diag(KE)[:3], cheb_a=0.7, cheb_c=0.3) if mode == L.MODE_CHEB3 else {}
+            y_ref = prev.clone()
+            ops.matvec(blocks, E, u, y_ref, s, mask=ops.Mask(L.MASK_X0), mode=mode, **ep)
+            data.append((s, E, u, prev, ep, y_ref))
+        results = []
+        for lx in (0, 1, 5):
+            items, outs = [], []
+            for s, E, u, prev, ep, _ in data:
+                y = prev.clone()                 # CHEB3 reads the previous iterate from y
+                it = L.ProductItem()
+                it.blocks72 = keep.ctypes.data
+                it.E, it.u, it.y = E.data_ptr(), u.data_ptr(), y.data_ptr()
+                it.nx, it.ny, it.nz = s
+                it.mask = None
+                it.ep = _epilogue(mode, lx=lx, **{k: (v.data_ptr() if isinstance(v, torch.Tensor) else v)
+                                                  for k, v in ep.items()})
+                items.append(it)
+                outs.append(y)
+            h, _ = product_batch(dtype, L.MASK_X0, L.TILE_16x32, items)
+            h.launch()
+            torch.cuda.synchronize()
+            results.append(outs)
+        for k, (s, *_, y_ref) in enumerate(data):
+            for outs in results[1:]:
+                assert torch.equal(outs[k], results[0][k]), (mode, s)
+            err = float((results[0][k] - y_ref).abs().max() / y_ref.abs().max())
+            assert err <= tol, (mode, s, err)
