@@ -2,10 +2,10 @@
 
 Same names and argument meaning (src/voxel_kernels.py:28-110):
 
-  PROTOTYPE_SOURCE, make_source(real)   the HIP source of the parity product; FP64 and FP32 are
-                                        two instantiations of the same template, so both
-                                        'double' and 'float' return that text (other values:
-                                        ValueError, as src/voxel_kernels.py:39)
+  PROTOTYPE_SOURCE, make_source(real)   self-contained HIP source of the parity product: 'double'
+                                        returns PROTOTYPE_SOURCE (the FP64 instantiation),
+                                        'float' the FP32 one (other values: ValueError, as
+                                        src/voxel_kernels.py:34-43)
   Modal, quadrature, XYZ                element algebra (host setup; Modal uses the C-ABI's
                                         vh_modal_blocks and raises ValueError outside the
                                         parity-block model, code/hex_modal/modal.py:64-66)
@@ -21,6 +21,7 @@ Inputs may be numpy arrays or torch tensors; numpy in -> numpy out.
 """
 from __future__ import annotations
 
+import re
 from pathlib import Path
 
 import numpy as np
@@ -30,16 +31,47 @@ import voxelhip
 from voxelhip import ops
 
 _CSRC = Path(__file__).resolve().parent / "csrc"
-PROTOTYPE_SOURCE = (_CSRC / "matvec.hip").read_text()
+
+
+def _translation_unit(name: str) -> str:
+    """The text of csrc/<name> with its project includes inlined once each (system headers
+    stay as #include lines): one self-contained source that hipcc compiles on its own."""
+    seen = set()
+
+    def expand(fname):
+        out = []
+        for line in (_CSRC / fname).read_text().splitlines(keepends=True):
+            m = re.match(r'\s*#include "([^"]+)"', line)
+            if m and (_CSRC / m.group(1)).exists():
+                if m.group(1) not in seen:
+                    seen.add(m.group(1))
+                    out.append(expand(m.group(1)))
+            elif not line.startswith("#pragma once"):
+                out.append(line)
+        return "".join(out)
+
+    return expand(name)
+
+
+# the FP64 instantiation of the parity product (kernel template, epilogues and launcher)
+PROTOTYPE_SOURCE = _translation_unit("matvec_f64.hip")
+_FLOAT_SOURCE = None
 
 XYZ = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                 [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
 
 
 def make_source(real: str) -> str:
-    if real in ("double", "float"):
+    """HIP source for `real` in {'double', 'float'}; 'double' returns PROTOTYPE_SOURCE unchanged,
+    'float' the FP32 instantiation of the same template (FP32 arithmetic, FP32 or BF16 storage)."""
+    global _FLOAT_SOURCE
+    if real == "double":
         return PROTOTYPE_SOURCE
-    raise ValueError(real)
+    if real != "float":
+        raise ValueError(real)
+    if _FLOAT_SOURCE is None:
+        _FLOAT_SOURCE = _translation_unit("matvec_f32.hip")
+    return _FLOAT_SOURCE
 
 
 def quadrature(lengths, nu):

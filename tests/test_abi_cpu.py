@@ -68,13 +68,29 @@ def test_product_quadrature_matches_the_oracle(golden):
     assert np.abs(2 * vk.quadrature([1, 1, 1], 0.3) - np.load(golden / "ke_unit_3d.npy")).max() < 1e-15
 
 
-def test_make_source_returns_the_hip_kernel_source():
+def test_make_source_returns_the_hip_kernel_source(tmp_path):
+    """'double' returns PROTOTYPE_SOURCE, 'float' a different text that instantiates the FP32
+    kernels only; both are self-contained (they pass hipcc's front end for gfx950 alone)."""
+    import shutil
+    import subprocess
     import voxel_kernels as vk
     src = vk.make_source("double")
     assert src is vk.PROTOTYPE_SOURCE and "parity_xmarch" in src
-    assert vk.make_source("float") is src
+    f32 = vk.make_source("float")
+    assert f32 != src and "parity_xmarch" in f32
+    assert "#define VH_MATVEC_F32 1" in f32 and "#define VH_MATVEC_F64 1" not in f32
+    assert "#define VH_MATVEC_F64 1" in src and "#define VH_MATVEC_F32 1" not in src
+    assert '#include "' not in src and '#include "' not in f32
     with pytest.raises(ValueError):
         vk.make_source("half")
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        pytest.skip("no hipcc")
+    for name, text in (("f64.hip", src), ("f32.hip", f32)):
+        (tmp_path / name).write_text(text)
+        r = subprocess.run([hipcc, "-std=c++17", "--offload-arch=gfx950", "--cuda-device-only", "-fsyntax-only",
+                            str(tmp_path / name)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
 
 
 def test_device_entry_points_fail_loudly_without_a_gpu():
