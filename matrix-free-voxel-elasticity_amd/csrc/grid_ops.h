@@ -18,16 +18,17 @@ struct GridArgs {
 // 1/2 if |i - 2I| == 1 (src/voxel_mg.py:74-105: P copies even nodes, averages odd ones; R = P^T).
 // (nx, ny, nz) is always the FINE element grid.  Both operators work on blocks of whole rows
 // of the author layout (a row = the 3 (nz + 1) values of one (ix, iy) z-line, contiguous in HBM):
-//  restriction: coarse plane Ix x RB coarse rows x a z segment of coarse nodes.  The x weights
-//    combine the three fine planes 2Ix - 1 .. 2Ix + 1 into 2 RB + 1 weighted fine rows in LDS
+//  restriction: coarse plane Ix x R coarse rows x a z segment of coarse nodes.  The x weights
+//    combine the three fine planes 2Ix - 1 .. 2Ix + 1 into 2 R + 1 weighted fine rows in LDS
 //    (one coalesced pass, every load of the block in flight together); each coarse DOF then takes
 //    its y and z neighbours from LDS.
-//  prolongation: fine plane ix x PB fine rows x a z segment of fine nodes.  The fine values
-//    (for xf +=) are loaded first, then the PB / 2 + 1 coarse rows are combined with the x
+//  prolongation: fine plane ix x R fine rows x a z segment of fine nodes.  The fine values
+//    (for xf +=) are loaded first, then the R / 2 + 1 coarse rows are combined with the x
 //    weight into LDS; each fine DOF interpolates along y and z from LDS.
-// Blocks of one grid are numbered z segment fastest, then y block, then x plane.  The block
-// shape and workgroup size are compile-time configurations (RestrictCfg / ProlongCfg); the
-// library's kernels use TR_RESTRICT and TR_PROLONG.
+// Blocks of one grid are numbered z segment fastest, then y block, then x plane.  The workgroup
+// size and the largest z segment are compile-time configurations (RestrictCfg / ProlongCfg; the
+// library's kernels use TR_RESTRICT and TR_PROLONG); the rows per block R are RB / PB, or more on
+// grids whose z segments are short (restrict_geom, prolong_geom).
 template <int RB_, int CZ_, int NT_, int U_>
 struct RestrictCfg {
   static constexpr int RB = RB_;   // coarse rows per block
@@ -67,6 +68,7 @@ struct RowCursor {
 struct TransferGeom {
   int ny1, nz1;   // nodes per column / row of the grid the blocks tile (coarse: restriction)
   int seg;        // z nodes per segment (prolongation: even, so a segment starts on a coarse node)
+  int rows;       // rows per block (restriction: coarse rows; prolongation: fine rows, even)
   int nyb, nzb;   // blocks along y, segments along z
   int nblocks;
   size_t lds;     // LDS values per block
@@ -79,9 +81,13 @@ __host__ __device__ inline TransferGeom restrict_geom(int nx, int ny, int nz) {
   g.nz1 = nz / 2 + 1;
   g.nzb = (g.nz1 + C::CZ - 1) / C::CZ;
   g.seg = (g.nz1 + g.nzb - 1) / g.nzb;
-  g.nyb = (g.ny1 + C::RB - 1) / C::RB;
+  // RB coarse rows per block, or more where the z segment is short (narrow grids, coarse
+  // levels), so that a block still stages about 2 U NT fine values and keeps its threads busy
+  const int flen = 3 * (2 * g.seg + 1);
+  g.rows = max(C::RB, ((2 * C::U * C::NT) / flen - 1) / 2);
+  g.nyb = (g.ny1 + g.rows - 1) / g.rows;
   g.nblocks = (nx / 2 + 1) * g.nyb * g.nzb;
-  g.lds = size_t(2 * C::RB + 1) * 3 * (2 * g.seg + 1);
+  g.lds = size_t(2 * g.rows + 1) * flen;
   return g;
 }
 
@@ -92,9 +98,15 @@ __host__ __device__ inline TransferGeom prolong_geom(int nx, int ny, int nz) {
   g.nz1 = nz + 1;
   g.nzb = (g.nz1 + C::FZ - 1) / C::FZ;
   g.seg = (((g.nz1 + g.nzb - 1) / g.nzb) + 1) & ~1;
-  g.nyb = (g.ny1 + C::PB - 1) / C::PB;
+  // PB fine rows per block, or more (even) where the z segment is short, as far as the PF fine
+  // and CS coarse values per thread hold them: rows 3 seg <= PF NT, (rows / 2 + 1) 3 (seg / 2 + 1)
+  // <= CS NT (PB meets both, since seg <= FZ)
+  const int by_fine = (C::PF * C::NT) / (3 * g.seg);
+  const int by_coarse = 2 * ((C::CS * C::NT) / (3 * (g.seg / 2 + 1))) - 2;
+  g.rows = max(C::PB, min(by_fine, by_coarse) & ~1);
+  g.nyb = (g.ny1 + g.rows - 1) / g.rows;
   g.nblocks = (nx + 1) * g.nyb * g.nzb;
-  g.lds = size_t(C::PB / 2 + 1) * 3 * (g.seg / 2 + 1);
+  g.lds = size_t(g.rows / 2 + 1) * 3 * (g.seg / 2 + 1);
   return g;
 }
 
@@ -108,8 +120,8 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
   const int zb = blk % g.nzb;
   const int yb = (blk / g.nzb) % g.nyb;
   const int Ix = blk / (g.nzb * g.nyb);
-  const int Iy0 = yb * C::RB, Iz0 = zb * g.seg;
-  const int nrow = min(C::RB, g.ny1 - Iy0);        // coarse rows of the block
+  const int Iy0 = yb * g.rows, Iz0 = zb * g.seg;
+  const int nrow = min(g.rows, g.ny1 - Iy0);       // coarse rows of the block
   const int ncz = min(g.seg, g.nz1 - Iz0);         // coarse nodes of the segment
   const int flen = 3 * (2 * ncz + 1);              // fine nodes 2 Iz0 - 1 .. 2 (Iz0 + ncz) - 1
   const int nslot = (2 * nrow + 1) * flen;         // fine rows 2 Iy0 - 1 .. 2 (Iy0 + nrow) - 1
@@ -174,12 +186,12 @@ __device__ __forceinline__ void prolong_block(const T* __restrict__ ec, T* __res
   const int zb = blk % g.nzb;
   const int yb = (blk / g.nzb) % g.nyb;
   const int ix = blk / (g.nzb * g.nyb);
-  const int iy0 = yb * C::PB, iz0 = zb * g.seg;    // both even
-  const int nrow = min(C::PB, g.ny1 - iy0);
+  const int iy0 = yb * g.rows, iz0 = zb * g.seg;  // both even
+  const int nrow = min(g.rows, g.ny1 - iy0);
   const int flen = 3 * min(g.seg, g.nz1 - iz0);
   const int cy1 = ny / 2 + 1, cz1 = nz / 2 + 1;
   const int Y0 = iy0 >> 1, Z0 = iz0 >> 1;
-  const int crow = min(C::PB / 2 + 1, cy1 - Y0);
+  const int crow = min(g.rows / 2 + 1, cy1 - Y0);
   const int clen = 3 * min(g.seg / 2 + 1, cz1 - Z0);
   const int fz3 = 3 * g.nz1;
   const size_t cz3 = size_t(3) * cz1;
