@@ -316,19 +316,26 @@ int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_
   VH_REQUIRE(dt_ok(dtype) && n >= 1 && items && handle, "vh_product_batch_create: bad arguments");
   VH_REQUIRE(mask_kind == MASK_X0 || mask_kind == MASK_ARRAY, "vh_product_batch_create: mask kind %d", mask_kind);
   const int mode = items[0].ep.mode;
-  const int combine = items[0].ep.w != nullptr;
+  // load-time combination: 0 none, 1 w of the compute type, 2 an FP32 w in an FP64 RESID pass
+  // (the mixed-precision Krylov loop's search direction, flag MV_W_F32)
+  const int combine = items[0].ep.w == nullptr ? 0 : ((items[0].ep.flags & MV_W_F32) ? 2 : 1);
+  VH_REQUIRE(combine != 2 || (dtype == F64 && mode == MODE_RESID),
+             "vh_product_batch_create: an FP32 w needs the FP64 RESID pass");
   VH_REQUIRE(mode == MODE_RAW || mode == MODE_RESID || mode == MODE_DOT || mode == MODE_JACOBI || mode == MODE_CHEB3,
              "vh_product_batch_create: mode %d", mode);
   VH_REQUIRE(!combine || mode == MODE_RESID || mode == MODE_DOT || mode == MODE_CHEB3,
              "vh_product_batch_create: combination needs DOT/RESID/CHEB3");
   for (int i = 0; i < n; ++i) {
     const vh_product_item& it = items[i];
-    VH_REQUIRE(it.blocks72 && it.E && it.u && (it.y || mode == MODE_DOT) && it.u != it.y,
+    VH_REQUIRE(it.blocks72 && it.E && it.u && (it.y || mode == MODE_DOT || (mode == MODE_RESID && it.ep.y32)) &&
+                   it.u != it.y,
                "vh_product_batch_create: item %d pointers", i);
     VH_REQUIRE(it.nx >= 1 && it.ny >= 1 && it.nz >= 1 && it.nz <= 2048 &&
                    3.0 * (it.nx + 1.0) * (it.ny + 1.0) * (it.nz + 1.0) < 2147483648.0,
                "vh_product_batch_create: item %d grid", i);
-    VH_REQUIRE(it.ep.mode == mode && (it.ep.w != nullptr) == combine, "vh_product_batch_create: item %d differs", i);
+    VH_REQUIRE(it.ep.mode == mode && (it.ep.w == nullptr ? 0 : ((it.ep.flags & MV_W_F32) ? 2 : 1)) == combine,
+               "vh_product_batch_create: item %d differs", i);
+    VH_REQUIRE((it.ep.flags & ~(MV_PREV_ZERO | MV_W_F32)) == 0, "vh_product_batch_create: item %d flags", i);
     VH_REQUIRE(mask_kind != MASK_ARRAY || it.mask, "vh_product_batch_create: item %d mask", i);
     VH_REQUIRE(!it.ep.prev || (mode == MODE_CHEB3 && !combine && it.ep.prev != it.y),
                "vh_product_batch_create: item %d prev", i);

@@ -97,7 +97,7 @@ int matvec_launch(MatvecParams<T> p, int storage, int mode, int mask_kind, hipSt
 // Every problem shares the mode, the mask kind, the tile variant, the load-time combination and
 // the voxel-cube coefficient form.
 struct MatvecBatch {
-  int mode, mask_kind, tile, combine;
+  int mode, mask_kind, tile, combine;   // combine: 0 none, 1 AXPY / MUL, 2 AXPY with an FP32 w (FP64)
   int n, total;
   const void* descs;
   const int* offs;

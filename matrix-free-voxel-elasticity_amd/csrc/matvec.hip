@@ -722,6 +722,12 @@ static int batch_mode(const MatvecBatch& bt, hipStream_t st) {
   switch (bt.mode) {
     case MODE_RAW: batch_tile<T, MODE_RAW, MASK, LOAD_NONE>(bt, st); break;
     case MODE_RESID:
+      if constexpr (sizeof(T) == 8) {
+        if (bt.combine == 2) {          // FP32 w (checked at creation)
+          batch_tile<T, MODE_RESID, MASK, LOAD_AXPY32>(bt, st);
+          break;
+        }
+      }
       if (bt.combine) batch_tile<T, MODE_RESID, MASK, LOAD_AXPY>(bt, st);
       else batch_tile<T, MODE_RESID, MASK, LOAD_NONE>(bt, st);
       break;

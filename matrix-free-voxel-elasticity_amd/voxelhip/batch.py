@@ -10,13 +10,15 @@ is one launch of a descriptor-table kernel (include/voxelhip.h, "batched problem
 of different sizes have different hierarchy depths: an instance whose coarsest level is depth d
 takes the dense solve there while deeper instances keep smoothing, and all coarsest solves of
 the batch are one launch between the down and the up sweep.  With graph=True the iteration is
-captured in two HIP graphs (buffer parity), replayed once per iteration; the host reads the
-instances' residual norms after each replay and freezes converged instances (their scalar
-recurrence yields alpha = beta = 0, so their iterate stops moving).
+captured in two HIP graphs (buffer parity), replayed once per iteration.  Convergence is
+decided on the device: each iteration's residual pass ends in a per-instance scalar step that
+freezes a converged instance (its recurrence yields alpha = beta = 0, so its iterate stops
+moving); the host reads the residual norms asynchronously while the next iteration runs.
 
 The algorithm per instance is that of voxelhip.engine.GMGSolver (flexible CG with the true
 residual every iteration, src/voxel_mg.py:181-217; Chebyshev-Jacobi V-cycle), precision "fp64"
-or "mixed" (FP32 V-cycle in the FP64 outer iteration).
+or "mixed" (FP32 V-cycle and FP32 search direction in the FP64 outer iteration: x, b and the
+true residual's arithmetic stay FP64).
 """
 from __future__ import annotations
 
@@ -188,12 +190,20 @@ class BatchedGMGSolver:
                     tot += self.hier[p][2].levels[d].n
             self.off.append((offs, tot))
         f64 = dict(dtype=torch.float64, device=dev)
+        pd = dict(dtype=self.pdtype, device=dev)
         nf = self.off[0][1]
         z = lambda: torch.zeros(nf, **f64)  # noqa: E731
-        self.b, self.r = z(), z()
-        self.xs, self.ps, self.zs = [z(), z()], [z(), z()], [z(), z()]
+        # mixed precision forms the search direction in FP32, as GMGSolver does: z and p are FP32,
+        # the DOT pass runs on the FP32 fine level, the FP64 RESID pass reads the FP32 p and
+        # writes only the FP32 copy of r (the V-cycle right-hand side; the Polak-Ribiere dots
+        # take r from it), so no FP64 r exists
+        self.p32 = self.pdtype != torch.float64
+        self.b = z()
+        self.r = None if self.p32 else z()
+        self.xs = [z(), z()]
+        self.ps = [torch.zeros(nf, **pd), torch.zeros(nf, **pd)]
+        self.zs = [torch.zeros(nf, **pd), torch.zeros(nf, **pd)]
         self.scal = torch.zeros(16 * n, **f64)
-        pd = dict(dtype=self.pdtype, device=dev)
         self.X, self.X2, self.RES, self.RHS, self.DT = [], [], [], [], []
         for d in range(self.D + 1):
             tot = self.off[d][1]
@@ -234,14 +244,16 @@ class BatchedGMGSolver:
                     fine = self.hier[p][1]
                     fo = self.off[0][0][k]
                     sdev = _ptr(self.scal, 16 * k)
-                    items_d.append(self._item(fine, _ptr(self.zs[i], fo), None, L.MODE_DOT,
-                                              partial=base, w=_ptr(self.ps[o], fo), sdev=sdev, sidx=L.S_BETA,
-                                              u_out=_ptr(self.ps[i], fo)))
-                    y32 = _ptr(self.RHS[0], fo) if self.pdtype != f64 else None
-                    items_r.append(self._item(fine, _ptr(self.xs[o], fo), _ptr(self.r, fo), L.MODE_RESID,
-                                              b=_ptr(self.b, fo), partial=base, w=_ptr(self.ps[i], fo), sdev=sdev,
-                                              sidx=L.S_ALPHA, u_out=_ptr(self.xs[i], fo), y32=y32))
-                self.dot_h.append(product_batch(f64, mk, self.tile, items_d))
+                    items_d.append(self._item(self.hier[p][2].levels[0] if self.p32 else fine, _ptr(self.zs[i], fo),
+                                              None, L.MODE_DOT, partial=base, w=_ptr(self.ps[o], fo), sdev=sdev,
+                                              sidx=L.S_BETA, u_out=_ptr(self.ps[i], fo)))
+                    items_r.append(self._item(fine, _ptr(self.xs[o], fo), None if self.p32 else _ptr(self.r, fo),
+                                              L.MODE_RESID, b=_ptr(self.b, fo), partial=base,
+                                              w=_ptr(self.ps[i], fo), sdev=sdev, sidx=L.S_ALPHA,
+                                              u_out=_ptr(self.xs[i], fo),
+                                              y32=_ptr(self.RHS[0], fo) if self.p32 else None,
+                                              flags=L.W_F32 if self.p32 else 0))
+                self.dot_h.append(product_batch(self.pdtype, mk, self.tile, items_d))
                 self.res_h.append(product_batch(f64, mk, self.tile, items_r))
             if attempt == 0:
                 need = max(max(sum(t) for _, t in self.dot_h + self.res_h), 2 * self.seg_total)
@@ -397,7 +409,8 @@ class BatchedGMGSolver:
         self.res_h[i][0].launch()
         self._finish(self.res_offs[i], 1, 8, L.OP_RESNORM)
         zl = self.vcycle()
-        L.call("vh_seg_dotdiff", self.segs.h, _DT[zl.dtype], L.F64, zl.data_ptr(), self.r.data_ptr(),
+        vd = _DT[self.zs[i].dtype]
+        L.call("vh_seg_dotdiff", self.segs.h, _DT[zl.dtype], vd, zl.data_ptr(), self.RHS[0].data_ptr(),
                self.zs[i].data_ptr(), self.zs[o].data_ptr(), self.partial.data_ptr(), ops._stream())
         self._finish((self.seg_offs, self.n, self.seg_total), 2, 5, L.OP_BETA_PR)
 
@@ -458,13 +471,14 @@ class BatchedGMGSolver:
         sv[:, L.S_TOL2] = torch.from_numpy((tol * self.nb) ** 2).to(self.dev)
         self.xs[1].zero_()
         self.ps[1].zero_()
-        self.r.copy_(self.b)
-        if self.pdtype != torch.float64:
-            ops.convert(self.r, self.RHS[0])
+        if self.p32:
+            ops.convert(self.b, self.RHS[0])
+        else:
+            self.r.copy_(self.b)
         zl = self.vcycle()
-        ops.convert(zl, self.zs[0])
-        L.call("vh_seg_reduce", self.segs.h, L.F64, L.RED_DOT, self.r.data_ptr(), self.zs[0].data_ptr(),
-               self.partial.data_ptr(), ops._stream())
+        self.zs[0].copy_(zl)
+        L.call("vh_seg_reduce", self.segs.h, _DT[self.zs[0].dtype], L.RED_DOT, self.RHS[0].data_ptr(),
+               self.zs[0].data_ptr(), self.partial.data_ptr(), ops._stream())
         self._finish((self.seg_offs, n, self.seg_total), 1, L.S_RZ, L.OP_NONE)
         if not hasattr(self, "_hbuf"):
             self._hbuf = [torch.empty(16 * n, dtype=torch.float64, pin_memory=True) for _ in range(2)]
