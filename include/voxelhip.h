@@ -300,7 +300,8 @@ typedef struct vh_product_item {
 /* Products of all items in one launch (same dtype, mode, mask kind, tile and load-time
  * combination).  DOT/RESID partials: every item passes the same ep.partial base; item i writes
  * its tiles at [offs[i], offs[i+1]) of it (offs from vh_batch_offsets, for vh_seg_finish).
- * tiles_out (n ints, optional) receives each item's tile count.  Modes RAW, RESID, DOT,
+ * tiles_out (n ints, optional) receives each item's tile count.  tile: 2 = 8 x 32 nodes,
+ * 5 = 16 x 16 nodes (batched products only), anything else 16 x 32.  Modes RAW, RESID, DOT,
  * JACOBI, CHEB3.  As in vh_matvec_fused, an FP64 RESID pass may take an FP32 w
  * (ep.flags VH_W_F32, every item alike) and may omit y when it writes ep.y32. */
 int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_product_item* items, void** handle,

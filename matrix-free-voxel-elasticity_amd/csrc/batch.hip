@@ -265,7 +265,7 @@ static int product_create(int mask_kind, int tile, int n, const vh_product_item*
     p.partial = ep.partial ? ep.partial + prefix[i] : nullptr;
     if (tiles_out) tiles_out[i] = pl.n_tiles;
   }
-  h->tile = tile == MV_TILE_8x32 ? MV_TILE_8x32 : MV_TILE_16x32;
+  h->tile = (tile == MV_TILE_8x32 || tile == MV_TILE_16x16) ? tile : MV_TILE_16x32;
   return upload(&h->descs, descs.data(), descs.size() * sizeof(MatvecParams<T>));
 }
 

@@ -25,8 +25,11 @@ constexpr int LOAD_NONE = 0, LOAD_AXPY = 1, LOAD_MUL = 2, LOAD_AXPY32 = 3;
 // over the y tiles): 0 = automatic, 1 = 16 x 32 (576 threads), 2 = 8 x 32 (320 threads), 3 = 4 x 8
 // (64 threads: one wave per workgroup, for the latency-bound coarse levels), 4 = the wide tile that
 // fills one CU with a single workgroup spread evenly over its four SIMDs: 22 x 32 in FP64 (768
-// threads, three waves per SIMD), 30 x 32 in FP32 (1024 threads, four waves per SIMD)
-constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2, MV_TILE_4x8 = 3, MV_TILE_WIDE = 4;
+// threads, three waves per SIMD), 30 x 32 in FP32 (1024 threads, four waves per SIMD); 5 = 16 x 16
+// (320 threads; batched products only: it wastes fewer lanes on the narrow z extents of many
+// small problems)
+constexpr int MV_TILE_AUTO = 0, MV_TILE_16x32 = 1, MV_TILE_8x32 = 2, MV_TILE_4x8 = 3, MV_TILE_WIDE = 4,
+              MV_TILE_16x16 = 5;
 
 // Epilogue modes (A = constrained operator M K M + I - M when a mask is given, else K).
 enum Mode : int {

@@ -487,6 +487,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecPar
 using Tile16 = Tile<16, 32>;
 using Tile8 = Tile<8, 32>;
 using Tile4 = Tile<4, 8>;
+using Tile16s = Tile<16, 16>;     // batched products only
 // one workgroup per CU, its waves spread evenly over the four SIMDs: FP64 (<= 168 VGPRs) takes
 // three waves per SIMD (23 x 33 = 759 element columns), FP32 (<= 128 VGPRs) four (31 x 33 = 1023)
 template <typename T>
@@ -712,6 +713,9 @@ static void batch_tile(const MatvecBatch& bt, hipStream_t st) {
   if (bt.tile == MV_TILE_8x32)
     hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile8, true, LOADOP>), dim3(grid), dim3(Tile8::BLOCK), 0, st,
                        d, bt.offs, bt.n);
+  else if (bt.tile == MV_TILE_16x16)
+    hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile16s, true, LOADOP>), dim3(grid), dim3(Tile16s::BLOCK),
+                       0, st, d, bt.offs, bt.n);
   else
     hipLaunchKernelGGL((parity_xmarch_batch<T, T, MODE, MASK, Tile16, true, LOADOP>), dim3(grid), dim3(Tile16::BLOCK), 0,
                        st, d, bt.offs, bt.n);
@@ -753,12 +757,13 @@ int matvec_batch_launch(const MatvecBatch& bt, hipStream_t st) {
 }
 
 // occupancy of the batched kernel itself (the single-problem kernels live in other translation
-// units)
+// units), taken for the Chebyshev smoothing pass that makes up most of a batched solve (it holds
+// more registers than the raw product)
 template <typename T, typename TL>
 static int resident_batch() {
   static const int occ = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch_batch<T, T, MODE_RAW, MASK_X0, TL, true, LOAD_NONE>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, parity_xmarch_batch<T, T, MODE_CHEB3, MASK_X0, TL, true, LOAD_NONE>,
                                                     TL::BLOCK, 0) != hipSuccess || n <= 0)
       n = 1;
     return n;
@@ -772,6 +777,9 @@ MatvecPlan matvec_batch_plan(int nx, int ny, int nz, int lx_hint, int tile) {
   if (tile == MV_TILE_8x32) {
     plan_tile<Tile8>(nx, ny, nz, lx_hint, resident_batch<T, Tile8>(), &p);
     p.tile = MV_TILE_8x32;
+  } else if (tile == MV_TILE_16x16) {
+    plan_tile<Tile16s>(nx, ny, nz, lx_hint, resident_batch<T, Tile16s>(), &p);
+    p.tile = MV_TILE_16x16;
   } else {
     plan_tile<Tile16>(nx, ny, nz, lx_hint, resident_batch<T, Tile16>(), &p);
     p.tile = MV_TILE_16x32;
@@ -796,7 +804,9 @@ void matvec_batch_chunks(int n, const int* nx, const int* ny, const int* nz, int
     per[i] = long(p.tiles_y) * p.tiles_z;
     max_nx = std::max(max_nx, nx[i]);
   }
-  const int occ = tile == MV_TILE_8x32 ? resident_batch<T, Tile8>() : resident_batch<T, Tile16>();
+  const int occ = tile == MV_TILE_8x32    ? resident_batch<T, Tile8>()
+                  : tile == MV_TILE_16x16 ? resident_batch<T, Tile16s>()
+                                          : resident_batch<T, Tile16>();
   const double slots = double(device_cus()) * occ;
   int best_l = max_nx;
   double best = 1e300;

@@ -20,7 +20,7 @@ OP_NONE, OP_ALPHA, OP_BETA_FR, OP_BETA_PR, OP_SET_RZ, OP_INVSQRT, OP_RESNORM = 0
 S_RZ, S_PQ, S_ALPHA, S_BETA, S_FLAG = 0, 1, 2, 3, 4
 S_DONE, S_TOL2, S_ITS = 10, 11, 12
 PREV_ZERO, W_F32 = 1, 2           # vh_epilogue.flags (VH_PREV_ZERO, VH_W_F32)
-TILE_AUTO, TILE_16x32, TILE_8x32, TILE_4x8, TILE_WIDE = 0, 1, 2, 3, 4
+TILE_AUTO, TILE_16x32, TILE_8x32, TILE_4x8, TILE_WIDE, TILE_16x16 = 0, 1, 2, 3, 4, 5
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

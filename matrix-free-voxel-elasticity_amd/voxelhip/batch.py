@@ -138,7 +138,7 @@ def coarse_dof_shapes(shape, max_coarse_dofs=3000):
 
 class BatchedGMGSolver:
     def __init__(self, problems, ke, *, max_coarse_x=None, max_coarse_dofs=3000, degree=3, precision="mixed",
-                 graph=True, lower_frac=1.0 / 30.0, omega_iterations=30, tile=L.TILE_16x32, lx=0):
+                 graph=True, lower_frac=1.0 / 30.0, omega_iterations=30, tile=L.TILE_16x16, lx=0):
         if precision not in ("fp64", "mixed"):
             raise ValueError(precision)
         if int(degree) < 2:

@@ -128,8 +128,9 @@ def test_full_batch_workload_meets_the_tolerance_on_an_independent_residual():
 def test_batched_product_equals_single_products(dtype):
     """One batched launch over problems of different sizes (x extents 8 ... 96, so that the
     batch-wide x chunking differs from each problem's own plan), for the raw product and the
-    three-term Chebyshev step: the automatic chunk length and fixed ones (1, 5) give bitwise the
-    same result (the element arithmetic and the gather order do not depend on the plan), and it
+    three-term Chebyshev step: the three batched tiles (16 x 16, 16 x 32, 8 x 32 nodes) with the
+    automatic chunk length or fixed ones (1, 5) give bitwise the same result (the element
+    arithmetic and the gather order do not depend on the plan), and it
     matches the single-problem kernel to rounding (the two kernels are compiled separately, and
     the compiler may fuse a different set of multiply-adds: 1e-13 of max |y| in FP64, 1e-5 in
     FP32)."""
@@ -155,7 +156,7 @@ def test_batched_product_equals_single_products(dtype):
             ops.matvec(blocks, E, u, y_ref, s, mask=ops.Mask(L.MASK_X0), mode=mode, **ep)
             data.append((s, E, u, prev, ep, y_ref))
         results = []
-        for lx in (0, 1, 5):
+        for tile, lx in ((L.TILE_16x16, 0), (L.TILE_16x16, 1), (L.TILE_16x32, 5), (L.TILE_16x32, 0), (L.TILE_8x32, 0)):
             items, outs = [], []
             for s, E, u, prev, ep, _ in data:
                 y = prev.clone()                 # CHEB3 reads the previous iterate from y
@@ -168,7 +169,7 @@ def test_batched_product_equals_single_products(dtype):
                                                   for k, v in ep.items()})
                 items.append(it)
                 outs.append(y)
-            h, _ = product_batch(dtype, L.MASK_X0, L.TILE_16x32, items)
+            h, _ = product_batch(dtype, L.MASK_X0, tile, items)
             h.launch()
             torch.cuda.synchronize()
             results.append(outs)
