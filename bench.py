@@ -333,6 +333,7 @@ def cpu_baseline_full():
         if r.returncode != 0:
             raise RuntimeError(f"oracle/cpu_baseline.py failed: {r.stderr[-2000:]}")
         runs[label] = json.loads(r.stdout.strip().splitlines()[-1])
+        print(f"bench.py: CPU baseline protocol, {n} thread(s), done", file=sys.stderr, flush=True)
     return runs
 
 
@@ -398,6 +399,12 @@ def main():
         print(json.dumps({"only": "batch", "c4_batched": r}), flush=True)
         return
     ke = 2 * vk.quadrature([1, 1, 1], 0.3)          # the donor's KE_UNIT_3D (ke_unit_3d.npy)
+    t_start = time.perf_counter()
+
+    def progress(leg):
+        # one line per finished leg on stderr (the JSON result stays the only line on stdout)
+        print(f"bench.py: {leg} done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
     # launch plans of every product pass, timed at set-up (outside the timed solves)
     tuner = LaunchTuner() if args.tune else False
     shape = (256, 128, 128)
@@ -414,6 +421,7 @@ def main():
         its = sorted({x[0] for x in st})
         out[prec] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "iterations": its,
                      "converged": all(x[1] for x in st), "max_rel_residual": max(x[2] for x in st)}
+        progress(f"{prec} solve")
     if args.only == "solve" or args.precision != "both":
         print(json.dumps({"only": "solve", "solves": out}), flush=True)
         return
@@ -434,6 +442,7 @@ def main():
                                  "iterations": sorted({x[0] for x in st}), "converged": all(x[1] for x in st),
                                  "max_rel_residual": max(x[2] for x in st)}
     solvers["mixed_bf16_storage"] = s_bf
+    progress("ir and bf16-storage solves")
     # error of every solution against the FP64 oracle (the constrained residual of the NumPy
     # oracle's product on the host) and against the FP64 GPU solve
     x64 = solvers["fp64"].solve(force)[0].clone()
@@ -443,6 +452,7 @@ def main():
         if prec != "fp64":
             out[prec]["rel_diff_vs_fp64_solution"] = float((x - x64).norm() / x64.norm())
     lambdas = solvers["fp64"].M.lambdas
+    progress("oracle residuals")
     iters64 = out["fp64"]["iterations"][0]
     del solvers, s_ir, s_bf, x64, x
     torch.cuda.empty_cache()
@@ -472,15 +482,19 @@ def main():
                        "converged": all(x[1] for x in stv), "max_rel_residual": max(x[2] for x in stv)}
         del sv
     out["c2_void_mixed"] = {"shape": list(shape2), "void": "E = 1e-9 where rho <= 0.5", **gal}
+    progress("128x64x64 solves")
     if not args.no_batch:
         out["c4_batched"] = timed_batch(c4_problems(), ke, max(1, args.steps // 2), 1, graph=bool(args.graph))
         torch.cuda.empty_cache()
+        progress("batched solves")
     if args.simp_steps > 0:
         out["c5_simp"] = simp_leg(ke, steps=args.simp_steps)
         torch.cuda.empty_cache()
+        progress("SIMP run")
     if not args.no_reference:
         out["reference_workloads"] = reference_workloads(max(2, args.steps // 4), 1, bool(args.graph), tuner)
         torch.cuda.empty_cache()
+        progress("reference workloads")
     # the product under the library's automatic plan and under the tuned plan (the one the
     # solvers launch); the roofline is that of the tuned FP64 product at 256 x 128 x 128
     mv = {}
@@ -490,6 +504,7 @@ def main():
             mv[key] = tuned_matvec(sh, getattr(torch, d))
             mv[key]["auto_plan_us"] = matvec_timing(sh, getattr(torch, d))["us"]
     dom = mv["float64@256x128x128"]
+    progress("product table")
     cpu = None
     if args.cpu_baseline == "sample":
         cpu = cpu_baseline(shape, E, mask, force, ke, lambdas, iters64)
