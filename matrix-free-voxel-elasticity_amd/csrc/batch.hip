@@ -130,6 +130,7 @@ struct TransferDesc {
   int fmask_kind, cmask_kind;
   const uint8_t* fmask;
   const uint8_t* cmask;
+  int rows;                         // rows per block (restrict_rows / prolong_rows)
 };
 
 static_assert(TR_RESTRICT::NT == TR_PROLONG::NT, "one workgroup size for both transfer directions");
@@ -144,10 +145,10 @@ __global__ void __launch_bounds__(TR_RESTRICT::NT) transfer_batch_kernel(const T
   const TransferDesc<T>& t = d[i];
   const int blk = b - offs[i];
   if (RESTRICT)
-    restrict_block<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, t.cmask, t.cmask_kind, blk,
+    restrict_block<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, t.cmask, t.cmask_kind, t.rows, blk,
                       reinterpret_cast<T*>(smem_raw));
   else
-    prolong_block<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, assign, blk,
+    prolong_block<T>(t.src, t.dst, t.nx, t.ny, t.nz, t.fmask_kind, t.fmask, assign, t.rows, blk,
                      reinterpret_cast<T*>(smem_raw));
 }
 
@@ -275,10 +276,13 @@ static int transfer_create(int n, const vh_transfer_item* items, BatchHandle* h,
   size_t lds = 0;
   for (int i = 0; i < n; ++i) {
     const vh_transfer_item& it = items[i];
+    // the problems of a batch fill the chip together: rows grow with no block-count limit
+    const int rows = h->restrict_op ? restrict_rows<TR_RESTRICT>(it.nx, it.ny, it.nz, 0)
+                                    : prolong_rows<TR_PROLONG>(it.nx, it.ny, it.nz, 0);
     descs[i] = TransferDesc<T>{static_cast<const T*>(it.src), static_cast<T*>(it.dst), it.nx, it.ny, it.nz,
-                               it.fmask_kind, it.cmask_kind, it.fmask, it.cmask};
-    const TransferGeom g = h->restrict_op ? restrict_geom<TR_RESTRICT>(it.nx, it.ny, it.nz)
-                                          : prolong_geom<TR_PROLONG>(it.nx, it.ny, it.nz);
+                               it.fmask_kind, it.cmask_kind, it.fmask, it.cmask, rows};
+    const TransferGeom g = h->restrict_op ? restrict_geom<TR_RESTRICT>(it.nx, it.ny, it.nz, rows)
+                                          : prolong_geom<TR_PROLONG>(it.nx, it.ny, it.nz, rows);
     prefix[i + 1] = prefix[i] + g.nblocks;
     lds = std::max(lds, g.lds);
   }

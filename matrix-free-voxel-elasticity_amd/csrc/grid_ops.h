@@ -28,7 +28,7 @@ struct GridArgs {
 // Blocks of one grid are numbered z segment fastest, then y block, then x plane.  The workgroup
 // size and the largest z segment are compile-time configurations (RestrictCfg / ProlongCfg; the
 // library's kernels use TR_RESTRICT and TR_PROLONG); the rows per block R are RB / PB, or more on
-// grids whose z segments are short (restrict_geom, prolong_geom).
+// grids whose z segments are short (restrict_rows, prolong_rows).
 template <int RB_, int CZ_, int NT_, int U_>
 struct RestrictCfg {
   static constexpr int RB = RB_;   // coarse rows per block
@@ -75,48 +75,67 @@ struct TransferGeom {
 };
 
 template <typename C>
-__host__ __device__ inline TransferGeom restrict_geom(int nx, int ny, int nz) {
+__host__ __device__ inline TransferGeom restrict_geom(int nx, int ny, int nz, int rows) {
   TransferGeom g;
   g.ny1 = ny / 2 + 1;
   g.nz1 = nz / 2 + 1;
   g.nzb = (g.nz1 + C::CZ - 1) / C::CZ;
   g.seg = (g.nz1 + g.nzb - 1) / g.nzb;
-  // RB coarse rows per block, or more where the z segment is short (narrow grids, coarse
-  // levels), so that a block still stages about 2 U NT fine values and keeps its threads busy
-  const int flen = 3 * (2 * g.seg + 1);
-  g.rows = max(C::RB, ((2 * C::U * C::NT) / flen - 1) / 2);
+  g.rows = rows;
   g.nyb = (g.ny1 + g.rows - 1) / g.rows;
   g.nblocks = (nx / 2 + 1) * g.nyb * g.nzb;
-  g.lds = size_t(2 * g.rows + 1) * flen;
+  g.lds = size_t(2 * g.rows + 1) * 3 * (2 * g.seg + 1);
   return g;
 }
 
 template <typename C>
-__host__ __device__ inline TransferGeom prolong_geom(int nx, int ny, int nz) {
+__host__ __device__ inline TransferGeom prolong_geom(int nx, int ny, int nz, int rows) {
   TransferGeom g;
   g.ny1 = ny + 1;
   g.nz1 = nz + 1;
   g.nzb = (g.nz1 + C::FZ - 1) / C::FZ;
   g.seg = (((g.nz1 + g.nzb - 1) / g.nzb) + 1) & ~1;
-  // PB fine rows per block, or more (even) where the z segment is short, as far as the PF fine
-  // and CS coarse values per thread hold them: rows 3 seg <= PF NT, (rows / 2 + 1) 3 (seg / 2 + 1)
-  // <= CS NT (PB meets both, since seg <= FZ)
-  const int by_fine = (C::PF * C::NT) / (3 * g.seg);
-  const int by_coarse = 2 * ((C::CS * C::NT) / (3 * (g.seg / 2 + 1))) - 2;
-  g.rows = max(C::PB, min(by_fine, by_coarse) & ~1);
+  g.rows = rows;
   g.nyb = (g.ny1 + g.rows - 1) / g.rows;
   g.nblocks = (nx + 1) * g.nyb * g.nzb;
   g.lds = size_t(g.rows / 2 + 1) * 3 * (g.seg / 2 + 1);
   return g;
 }
 
-// rc = mask_c * R (mask_f * rf) on restriction block `blk`; xl: restrict_geom(..).lds values.
+// Rows per block (host side; the kernels take the result as an argument).  RB / PB, or more
+// where the z segment is short (narrow grids, coarse levels, the small problems of a batch), so
+// that a block still stages about 2 U NT fine values (restriction) or fills its PF fine and CS
+// coarse values per thread (prolongation: rows 3 seg <= PF NT and (rows / 2 + 1) 3 (seg / 2 + 1)
+// <= CS NT; PB meets both since seg <= FZ).  The rows grow only while the launch keeps at least
+// min_blocks blocks (0: no limit, for a batch whose problems fill the chip together).
+template <typename C>
+inline int restrict_rows(int nx, int ny, int nz, int min_blocks) {
+  const TransferGeom g = restrict_geom<C>(nx, ny, nz, C::RB);
+  const int flen = 3 * (2 * g.seg + 1);
+  int rows = ((2 * C::U * C::NT) / flen - 1) / 2;
+  while (rows > C::RB && restrict_geom<C>(nx, ny, nz, rows).nblocks < min_blocks) --rows;
+  return rows > C::RB ? rows : C::RB;
+}
+
+template <typename C>
+inline int prolong_rows(int nx, int ny, int nz, int min_blocks) {
+  const TransferGeom g = prolong_geom<C>(nx, ny, nz, C::PB);
+  const int by_fine = (C::PF * C::NT) / (3 * g.seg);
+  const int by_coarse = 2 * ((C::CS * C::NT) / (3 * (g.seg / 2 + 1))) - 2;
+  int rows = (by_fine < by_coarse ? by_fine : by_coarse) & ~1;
+  while (rows > C::PB && prolong_geom<C>(nx, ny, nz, rows).nblocks < min_blocks) rows -= 2;
+  return rows > C::PB ? rows : C::PB;
+}
+
+// rc = mask_c * R (mask_f * rf) on restriction block `blk` of the geometry with `rows` coarse rows
+// per block; xl: restrict_geom(..).lds values.
 template <typename T, typename C = TR_RESTRICT>
 __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __restrict__ rc, int nx, int ny, int nz,
                                                int fmask_kind, const uint8_t* __restrict__ fmask,
-                                               const uint8_t* __restrict__ cmask, int cmask_kind, int blk, T* xl) {
+                                               const uint8_t* __restrict__ cmask, int cmask_kind, int rows, int blk,
+                                               T* xl) {
   constexpr int NT = C::NT, U = C::U;
-  const TransferGeom g = restrict_geom<C>(nx, ny, nz);
+  const TransferGeom g = restrict_geom<C>(nx, ny, nz, rows);
   const int zb = blk % g.nzb;
   const int yb = (blk / g.nzb) % g.nyb;
   const int Ix = blk / (g.nzb * g.nyb);
@@ -176,13 +195,14 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
   }
 }
 
-// xf (+)= mask_f * P ec on prolongation block `blk`; cl: prolong_geom(..).lds values.
+// xf (+)= mask_f * P ec on prolongation block `blk` of the geometry with `rows` fine rows per block
+// (even); cl: prolong_geom(..).lds values.
 template <typename T, typename C = TR_PROLONG>
 __device__ __forceinline__ void prolong_block(const T* __restrict__ ec, T* __restrict__ xf, int nx, int ny, int nz,
-                                              int mask_kind, const uint8_t* __restrict__ mask, int assign, int blk,
-                                              T* cl) {
+                                              int mask_kind, const uint8_t* __restrict__ mask, int assign, int rows,
+                                              int blk, T* cl) {
   constexpr int NT = C::NT, PF = C::PF, CS = C::CS;
-  const TransferGeom g = prolong_geom<C>(nx, ny, nz);
+  const TransferGeom g = prolong_geom<C>(nx, ny, nz, rows);
   const int zb = blk % g.nzb;
   const int yb = (blk / g.nzb) % g.nyb;
   const int ix = blk / (g.nzb * g.nyb);

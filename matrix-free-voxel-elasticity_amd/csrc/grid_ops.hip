@@ -154,22 +154,23 @@ __global__ void __launch_bounds__(256) dense_node_kernel(const T* __restrict__ u
 template <typename T>
 __global__ void __launch_bounds__(TR_RESTRICT::NT) restrict_kernel(const T* __restrict__ rf, T* __restrict__ rc,
                                                              const GridArgs<T> g, const uint8_t* __restrict__ cmask,
-                                                             int cmask_kind, int nblocks) {
+                                                             int cmask_kind, int rows, int nblocks) {
   extern __shared__ unsigned char smem_raw[];
   const int blk = xcd_logical(blockIdx.x, gridDim.x);
   if (blk >= nblocks) return;
-  restrict_block<T>(rf, rc, g.nx, g.ny, g.nz, g.mask_kind, g.mask, cmask, cmask_kind, blk,
+  restrict_block<T>(rf, rc, g.nx, g.ny, g.nz, g.mask_kind, g.mask, cmask, cmask_kind, rows, blk,
                     reinterpret_cast<T*>(smem_raw));
 }
 
 // prolong: xf (+)= mask_f * P ec.
 template <typename T>
 __global__ void __launch_bounds__(TR_PROLONG::NT) prolong_kernel(const T* __restrict__ ec, T* __restrict__ xf,
-                                                            const GridArgs<T> g, int assign, int nblocks) {
+                                                            const GridArgs<T> g, int assign, int rows, int nblocks) {
   extern __shared__ unsigned char smem_raw[];
   const int blk = xcd_logical(blockIdx.x, gridDim.x);
   if (blk >= nblocks) return;
-  prolong_block<T>(ec, xf, g.nx, g.ny, g.nz, g.mask_kind, g.mask, assign, blk, reinterpret_cast<T*>(smem_raw));
+  prolong_block<T>(ec, xf, g.nx, g.ny, g.nz, g.mask_kind, g.mask, assign, rows, blk,
+                   reinterpret_cast<T*>(smem_raw));
 }
 
 // 2x2x2 mean of element moduli (src/voxel_mg.py:108-111); g = fine grid.
@@ -282,21 +283,36 @@ int dense_launch(const T* u, const T* E, const T* ke, T* y, int nx, int ny, int 
   return hip_status(hipGetLastError(), "dense_node_kernel");
 }
 
+// A single-problem transfer keeps at least two blocks per CU (fewer, fuller blocks would leave a
+// small grid's launch latency-bound on fewer CUs); the CU count is read once.
+static int transfer_min_blocks() {
+  static const int n = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    return 2 * cus;
+  }();
+  return n;
+}
+
 template <typename T>
 int restrict_launch(const T* rf, T* rc, const GridArgs<T>& g, const uint8_t* cmask, int cmask_kind, hipStream_t st) {
-  const TransferGeom tg = restrict_geom<TR_RESTRICT>(g.nx, g.ny, g.nz);
+  const int rows = restrict_rows<TR_RESTRICT>(g.nx, g.ny, g.nz, transfer_min_blocks());
+  const TransferGeom tg = restrict_geom<TR_RESTRICT>(g.nx, g.ny, g.nz, rows);
   const unsigned grid = unsigned((tg.nblocks + 7) / 8) * 8;
   hipLaunchKernelGGL(restrict_kernel<T>, dim3(grid), dim3(TR_RESTRICT::NT), tg.lds * sizeof(T), st, rf, rc, g, cmask,
-                     cmask_kind, tg.nblocks);
+                     cmask_kind, rows, tg.nblocks);
   return hip_status(hipGetLastError(), "restrict_kernel");
 }
 
 template <typename T>
 int prolong_launch(const T* ec, T* xf, const GridArgs<T>& g, int assign, hipStream_t st) {
-  const TransferGeom tg = prolong_geom<TR_PROLONG>(g.nx, g.ny, g.nz);
+  const int rows = prolong_rows<TR_PROLONG>(g.nx, g.ny, g.nz, transfer_min_blocks());
+  const TransferGeom tg = prolong_geom<TR_PROLONG>(g.nx, g.ny, g.nz, rows);
   const unsigned grid = unsigned((tg.nblocks + 7) / 8) * 8;
   hipLaunchKernelGGL(prolong_kernel<T>, dim3(grid), dim3(TR_PROLONG::NT), tg.lds * sizeof(T), st, ec, xf, g, assign,
-                     tg.nblocks);
+                     rows, tg.nblocks);
   return hip_status(hipGetLastError(), "prolong_kernel");
 }
 
