@@ -340,7 +340,9 @@ typedef struct vh_dense_item {
   void* y;
   const uint8_t* mask;
 } vh_dense_item;
-/* The coarsest solves y = mask * (A r) of all items in one launch, as vh_dense_mv. */
+/* The coarsest solves y = mask * (A r) of all items in one launch, as vh_dense_mv.  Consecutive
+ * items with the same A, n, ld and mask (the load cases of one problem) are applied in one pass
+ * over A, up to 4 at a time; each result is bitwise that of a separate pass. */
 int vh_dense_batch_create(int a_dtype, int v_dtype, int n, const vh_dense_item* items, void** handle);
 
 /* Launch a product, transfer or dense batch on `stream`. */

@@ -153,16 +153,20 @@ __global__ void __launch_bounds__(TR_RESTRICT::NT) transfer_batch_kernel(const T
 }
 
 // ------------------------------------------------------------------------------ dense solves
+// One descriptor per group of up to DENSE_RHS right-hand sides that share the matrix (the load cases
+// of one problem): the rows of A are read once for all of them.
+constexpr int DENSE_RHS = 4;
 template <typename TA, typename TV>
 struct DenseDesc {
   const TA* A;
-  const TV* r;
-  TV* y;
   const uint8_t* mask;
-  int n, ld;
+  int n, ld, k;                     // k right-hand sides (1 .. DENSE_RHS)
+  const TV* r[DENSE_RHS];
+  TV* y[DENSE_RHS];
 };
 
-// 4 rows per workgroup, one wave per row, as dense_mv_kernel (16-byte loads of A).
+// 4 rows per workgroup, one wave per row, as dense_mv_kernel (16-byte loads of A); each right-hand
+// side accumulates in the same order as a single-vector pass, so grouping does not change results.
 template <typename TA, typename TV>
 __global__ void __launch_bounds__(256) dense_batch_kernel(const DenseDesc<TA, TV>* __restrict__ d,
                                                           const int* __restrict__ offs, int nb) {
@@ -175,16 +179,35 @@ __global__ void __launch_bounds__(256) dense_batch_kernel(const DenseDesc<TA, TV
   if (row >= q.n) return;
   const TA* a = q.A + size_t(row) * q.ld;
   const int nfull = (q.n / V) * V;
-  TA acc = TA(0);
+  const int k = q.k;
+  TA acc[DENSE_RHS];
+#pragma unroll
+  for (int c = 0; c < DENSE_RHS; ++c) acc[c] = TA(0);
   for (int j = lane * V; j < nfull; j += 64 * V) {
     const Vec x = *reinterpret_cast<const Vec*>(a + j);
     const TA* e = reinterpret_cast<const TA*>(&x);
 #pragma unroll
-    for (int k = 0; k < V; ++k) acc += e[k] * TA(q.r[j + k]);
+    for (int c = 0; c < DENSE_RHS; ++c) {
+      if (c < k) {
+        const TV* r = q.r[c];
+#pragma unroll
+        for (int t = 0; t < V; ++t) acc[c] += e[t] * TA(r[j + t]);
+      }
+    }
   }
-  for (int t = nfull + lane; t < q.n; t += 64) acc += a[t] * TA(q.r[t]);
-  acc = wave_sum(acc);
-  if (lane == 0) q.y[row] = (q.mask && !q.mask[row]) ? TV(0) : TV(acc);
+  for (int t = nfull + lane; t < q.n; t += 64) {
+#pragma unroll
+    for (int c = 0; c < DENSE_RHS; ++c)
+      if (c < k) acc[c] += a[t] * TA(q.r[c][t]);
+  }
+  const bool fixed = q.mask && !q.mask[row];
+#pragma unroll
+  for (int c = 0; c < DENSE_RHS; ++c) {
+    if (c < k) {
+      const TA s = wave_sum(acc[c]);
+      if (lane == 0) q.y[c][row] = fixed ? TV(0) : TV(s);
+    }
+  }
 }
 
 static int upload(void** dst, const void* src, size_t bytes) {
@@ -292,12 +315,26 @@ static int transfer_create(int n, const vh_transfer_item* items, BatchHandle* h,
 
 template <typename TA, typename TV>
 static int dense_create(int n, const vh_dense_item* items, BatchHandle* h, std::vector<int>& prefix) {
-  std::vector<DenseDesc<TA, TV>> descs(n);
+  // consecutive items with the same matrix (pointer, size, stride and mask) share a descriptor
+  std::vector<DenseDesc<TA, TV>> descs;
+  prefix.assign(1, 0);
   for (int i = 0; i < n; ++i) {
     const vh_dense_item& it = items[i];
-    descs[i] = DenseDesc<TA, TV>{static_cast<const TA*>(it.A), static_cast<const TV*>(it.r), static_cast<TV*>(it.y),
-                                 it.mask, it.n, it.ld};
-    prefix[i + 1] = prefix[i] + (it.n + 3) / 4;
+    const TA* A = static_cast<const TA*>(it.A);
+    if (descs.empty() || descs.back().A != A || descs.back().n != it.n || descs.back().ld != it.ld ||
+        descs.back().mask != it.mask || descs.back().k == DENSE_RHS) {
+      DenseDesc<TA, TV> q{};
+      q.A = A;
+      q.mask = it.mask;
+      q.n = it.n;
+      q.ld = it.ld;
+      descs.push_back(q);
+      prefix.push_back(prefix.back() + (it.n + 3) / 4);
+    }
+    DenseDesc<TA, TV>& q = descs.back();
+    q.r[q.k] = static_cast<const TV*>(it.r);
+    q.y[q.k] = static_cast<TV*>(it.y);
+    ++q.k;
   }
   return upload(&h->descs, descs.data(), descs.size() * sizeof(DenseDesc<TA, TV>));
 }

@@ -178,3 +178,45 @@ def test_batched_product_equals_single_products(dtype):
                 assert torch.equal(outs[k], results[0][k]), (mode, s)
             err = float((results[0][k] - y_ref).abs().max() / y_ref.abs().max())
             assert err <= tol, (mode, s, err)
+
+
+def test_batched_coarsest_solves_group_the_load_cases_of_a_matrix():
+    """Consecutive items that share a matrix are applied in one pass over it, up to four at a time
+    (six items of one matrix split 4 + 2): bitwise the result of each item in a handle of its own,
+    and the single-vector vh_dense_mv to FP32 rounding."""
+    from voxelhip import _lib as L
+    from voxelhip import ops
+    from voxelhip.batch import dense_batch
+    print("case batched coarsest solves", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    mats = []
+    for n in (203, 96):
+        ld = (n + 3) // 4 * 4
+        A = torch.zeros(n, ld, device="cuda", dtype=torch.float32)
+        A[:, :n] = torch.randn(n, n, generator=g, device="cuda")
+        mask = (torch.rand(n, generator=g, device="cuda") > 0.1).to(torch.uint8)
+        mats.append((A, n, ld, mask))
+    counts = (6, 3)
+    rhs = [[torch.randn(m[1], generator=g, device="cuda") for _ in range(c)] for m, c in zip(mats, counts)]
+
+    def run(order):
+        items, outs = [], []
+        for mi, j in order:
+            A, n, ld, mask = mats[mi]
+            y = torch.full((n,), float("nan"), device="cuda")
+            it = L.DenseItem()
+            it.A, it.ld, it.n = A.data_ptr(), ld, n
+            it.r, it.y, it.mask = rhs[mi][j].data_ptr(), y.data_ptr(), mask.data_ptr()
+            items.append(it)
+            outs.append(((mi, j), y))
+        dense_batch(torch.float32, torch.float32, items).launch()
+        torch.cuda.synchronize()
+        return dict(outs)
+
+    grouped = run([(0, j) for j in range(6)] + [(1, j) for j in range(3)])
+    for key, y in grouped.items():
+        alone = run([key])[key]                    # a handle of one item: one pass over A for it alone
+        assert torch.equal(y, alone), key
+        A, n, ld, mask = mats[key[0]]
+        ref = ops.dense_mv(A, rhs[key[0]][key[1]], torch.empty(n, device="cuda"), ops.Mask(L.MASK_ARRAY, mask), n)
+        assert float((y - ref).abs().max() / ref.abs().max()) < 1e-6, key
