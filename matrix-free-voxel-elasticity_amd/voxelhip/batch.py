@@ -176,6 +176,7 @@ class BatchedGMGSolver:
         self._alloc()
         self._build()
         self._graphs = None
+        self._start_graph = None
 
     # ------------------------------------------------------------------ buffers
     def _alloc(self):
@@ -204,6 +205,7 @@ class BatchedGMGSolver:
         self.ps = [torch.zeros(nf, **pd), torch.zeros(nf, **pd)]
         self.zs = [torch.zeros(nf, **pd), torch.zeros(nf, **pd)]
         self.scal = torch.zeros(16 * n, **f64)
+        self.tol2 = torch.zeros(1, **f64)         # tol^2 of the running solve
         self.X, self.X2, self.RES, self.RHS, self.DT = [], [], [], [], []
         for d in range(self.D + 1):
             tot = self.off[d][1]
@@ -414,6 +416,35 @@ class BatchedGMGSolver:
                self.zs[i].data_ptr(), self.zs[o].data_ptr(), self.partial.data_ptr(), ops._stream())
         self._finish((self.seg_offs, self.n, self.seg_total), 2, 5, L.OP_BETA_PR)
 
+    def _start(self):
+        """The start of a solve on the device: the scalar blocks (tol^2 |b|^2 per instance), x = 0,
+        p = 0, z = M r with r = b, and r.z per instance (captured in a graph of its own)."""
+        sv = self.scal.view(self.n, 16)
+        sv.zero_()
+        torch.mul(self.nb2, self.tol2, out=sv[:, L.S_TOL2])
+        self.xs[1].zero_()
+        self.ps[1].zero_()
+        if self.p32:
+            ops.convert(self.b, self.RHS[0])
+        else:
+            self.r.copy_(self.b)
+        zl = self.vcycle()
+        self.zs[0].copy_(zl)
+        L.call("vh_seg_reduce", self.segs.h, _DT[self.zs[0].dtype], L.RED_DOT, self.RHS[0].data_ptr(),
+               self.zs[0].data_ptr(), self.partial.data_ptr(), ops._stream())
+        self._finish((self.seg_offs, self.n, self.seg_total), 1, L.S_RZ, L.OP_NONE)
+
+    def _capture_start(self):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):          # warm-up on a side stream (the replay redoes its work)
+            self._start()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._start()
+        return g
+
     def _capture(self):
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
@@ -439,7 +470,8 @@ class BatchedGMGSolver:
         L.call("vh_seg_reduce", self.segs.h, L.F64, L.RED_NORM2, self.b.data_ptr(), None, self.partial.data_ptr(),
                ops._stream())
         self._finish((self.seg_offs, self.n, self.seg_total), 1, 9, L.OP_NONE)
-        self.nb = self.scal.view(self.n, 16)[:, 9].sqrt().cpu().numpy()
+        self.nb2 = self.scal.view(self.n, 16)[:, 9].clone()      # |b|^2 per instance, on the device
+        self.nb = self.nb2.sqrt().cpu().numpy()
         self._loaded = True
 
     def set_load(self, k, force):
@@ -447,7 +479,8 @@ class BatchedGMGSolver:
         p, _ = self.inst[k]
         fo, nf = self.off[0][0][k], self.hier[p][1].n
         self.b[fo:fo + nf] = voxelhip_device(force, torch.float64, self.dev)
-        self.nb[k] = float(self.b[fo:fo + nf].norm())
+        self.nb2[k] = self.b[fo:fo + nf].square().sum()
+        self.nb[k] = float(self.nb2[k].sqrt())
 
     def solve(self, tol=1e-8, maxiter=200):
         """Solves every instance from x = 0 to ||b - A x|| / ||b|| <= tol.  Returns
@@ -466,20 +499,13 @@ class BatchedGMGSolver:
         if not getattr(self, "_loaded", False):
             self._upload_loads()
         n = self.n
-        sv = self.scal.view(n, 16)
-        sv.zero_()
-        sv[:, L.S_TOL2] = torch.from_numpy((tol * self.nb) ** 2).to(self.dev)
-        self.xs[1].zero_()
-        self.ps[1].zero_()
-        if self.p32:
-            ops.convert(self.b, self.RHS[0])
+        self.tol2.fill_(float(tol) ** 2)
+        if self.graph:
+            if self._start_graph is None:
+                self._start_graph = self._capture_start()
+            self._start_graph.replay()
         else:
-            self.r.copy_(self.b)
-        zl = self.vcycle()
-        self.zs[0].copy_(zl)
-        L.call("vh_seg_reduce", self.segs.h, _DT[self.zs[0].dtype], L.RED_DOT, self.RHS[0].data_ptr(),
-               self.zs[0].data_ptr(), self.partial.data_ptr(), ops._stream())
-        self._finish((self.seg_offs, n, self.seg_total), 1, L.S_RZ, L.OP_NONE)
+            self._start()
         if not hasattr(self, "_hbuf"):
             self._hbuf = [torch.empty(16 * n, dtype=torch.float64, pin_memory=True) for _ in range(2)]
             self._hev = [torch.cuda.Event() for _ in range(2)]

@@ -103,6 +103,7 @@ class GMGSolver:
         self.fine = None
         self.M = None
         self._graphs = None
+        self._start_graph = None
         if E is not None:
             self.update(E)
 
@@ -141,6 +142,7 @@ class GMGSolver:
                 self.M.levels[0].tune(self.tuner, [(L.MODE_DOT, True), (L.MODE_RESID, True)])
         self._alloc()
         self._graphs = None
+        self._start_graph = None
 
     def _bound_args(self):
         if self.bound_warm_iterations is None or self.smoother != "chebyshev":
@@ -180,6 +182,7 @@ class GMGSolver:
         self.partial = torch.zeros(2 * max(self.parts, self.t_dot, self.t_res, self.t_res0, self.t_dot32,
                                            self.t_res32, self.t_pr), dtype=torch.float64, device=self.dev)
         self.scal = torch.zeros(16, dtype=torch.float64, device=self.dev)
+        self.tol2 = torch.zeros(1, dtype=torch.float64, device=self.dev)     # tol^2 of the running solve
         # right-hand side of the finest V-cycle level: the FP32 copy of r written by the
         # residual pass (mixed), the inner residual (ir), or r itself (FP64)
         self.rhs0 = self.M._work[0]["rhs"] if self.pdtype != torch.float64 else self.r
@@ -254,6 +257,45 @@ class GMGSolver:
             graphs.append(g)
         return graphs
 
+    def _start(self, r0):
+        """The start of a solve on the device: z = M r, p = z (beta = 0 against a zero previous
+        direction), rz = r.z, and the scalar block (|b|^2 in slot 9, tol^2 |b|^2, iteration count
+        and freeze flag cleared).  r0 None: x = 0 and r = b (|b|^2 is formed here); otherwise the
+        caller has formed x0, r0 and |b|^2."""
+        if r0 is None:
+            ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
+            ops.finish(self.partial, self.parts, 1, self.scal, 9)
+            self.xs[1].zero_()
+            r0 = self.b
+        self.ps[1].zero_()
+        if self.pdtype != torch.float64:
+            ops.convert(r0, self.rhs0)
+        elif r0 is not self.r:
+            self.r.copy_(r0)
+        if self.fused_pr:
+            self.M.vcycle(self.rhs0, out=self.zs[0])
+            ops.reduce(L.RED_DOT, self.rhs0, self.zs[0], partial=self.partial)
+        else:
+            zl = self.M.vcycle(self.rhs0)
+            ops.convert(zl, self.zs[0])
+            ops.reduce(L.RED_DOT, r0, self.zs[0], partial=self.partial)
+        ops.finish(self.partial, self.parts, 1, self.scal, L.S_RZ)
+        # (slices filled on the device: assigning a Python number would copy from the host)
+        for slot in (L.S_BETA, L.S_DONE, L.S_ITS):
+            self.scal[slot:slot + 1].zero_()
+        torch.mul(self.scal[9:10], self.tol2, out=self.scal[L.S_TOL2:L.S_TOL2 + 1])
+
+    def _capture_start(self):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):          # warm-up on a side stream (the replay redoes its work)
+            self._start(None)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._start(None)
+        return g
+
     # ------------------------------------------------------------------ solve
     def solve(self, b, tol=1e-8, maxiter=200, x0=None):
         """Flexible CG (the true residual is formed and tested every iteration, as the reference
@@ -275,15 +317,22 @@ class GMGSolver:
         if self.graph and self._graphs is None:
             self._graphs = self._capture(self._iteration)
         self.b.copy_(b)
-        ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
-        ops.finish(self.partial, self.parts, 1, self.scal, 9)
-        nb = float(self.scal[9].sqrt())
+        self.tol2.fill_(float(tol) ** 2)
         start_mv = 0
         if x0 is None:
-            # x = 0: r = b (the first iteration forms r itself)
-            self.xs[1].zero_()
-            r0 = self.b
+            # x = 0 (r = b): the whole start runs without a host synchronisation, captured in a
+            # graph of its own (launched from Python, its ~25 small coarse-level passes would each
+            # wait for the host)
+            if self.graph:
+                if self._start_graph is None:
+                    self._start_graph = self._capture_start()
+                self._start_graph.replay()
+            else:
+                self._start(None)
         else:
+            ops.reduce(L.RED_NORM2, self.b, partial=self.partial)
+            ops.finish(self.partial, self.parts, 1, self.scal, 9)
+            nb = float(self.scal[9].sqrt())
             self.xs[1].copy_(x0)
             self.fine.matvec(self.xs[1], self.r, mode=L.MODE_RESID, b=self.b, partial=self.partial)
             ops.finish(self.partial, self.t_res0, 1, self.scal, 8)
@@ -291,24 +340,7 @@ class GMGSolver:
             rel0 = float(self.scal[8].sqrt()) / nb if nb > 0 else 0.0
             if rel0 <= tol:
                 return self.xs[1], 0, True, rel0, start_mv
-            r0 = self.r
-        # z = M r, p = z (beta = 0 against a zero previous direction), rz = r.z
-        self.ps[1].zero_()
-        if self.pdtype != torch.float64:
-            ops.convert(r0, self.rhs0)
-        elif r0 is not self.r:
-            self.r.copy_(r0)
-        if self.fused_pr:
-            self.M.vcycle(self.rhs0, out=self.zs[0])
-            ops.reduce(L.RED_DOT, self.rhs0, self.zs[0], partial=self.partial)
-        else:
-            zl = self.M.vcycle(self.rhs0)
-            ops.convert(zl, self.zs[0])
-            ops.reduce(L.RED_DOT, r0, self.zs[0], partial=self.partial)
-        ops.finish(self.partial, self.parts, 1, self.scal, L.S_RZ)
-        self.scal[L.S_BETA] = 0.0
-        # S_DONE, S_TOL2, S_ITS
-        self.scal[L.S_DONE:L.S_ITS + 1] = torch.tensor([0.0, (tol * nb) ** 2, 0.0], dtype=torch.float64)
+            self._start(self.r)
         if not hasattr(self, "_hbuf"):
             self._hbuf = [torch.empty(16, dtype=torch.float64, pin_memory=True) for _ in range(2)]
             self._hev = [torch.cuda.Event() for _ in range(2)]
@@ -338,6 +370,7 @@ class GMGSolver:
             k = pending.pop(0)
             self._hev[k % 2].synchronize()
             v = self._hbuf[k % 2].tolist()
+            nb = float(np.sqrt(v[9]))                 # |b|, from the scalar block of the start
             rel = float(np.sqrt(v[8])) / nb if nb > 0 else 0.0
             hist.append(rel)
             if v[L.S_DONE] != 0.0 or k >= maxiter:

@@ -215,7 +215,7 @@ def test_mixed_precision_state_solve_gives_the_fp64_sensitivities():
         r = torch.from_numpy(rho).cuda().requires_grad_(True)
         c = Compliance.apply(r, pb)
         c.backward()
-        out.append((float(c), r.grad.clone()))
+        out.append((float(c.detach()), r.grad.clone()))
     assert out[1][0] == pytest.approx(out[0][0], rel=1e-7)
     assert float((out[1][1] - out[0][1]).norm() / out[0][1].norm()) < 1e-6
 
