@@ -476,6 +476,8 @@ class BatchedGMGSolver:
 
     def set_load(self, k, force):
         """Replaces the load vector of instance k (a device or host vector of its problem's size)."""
+        if not getattr(self, "_loaded", False):
+            self._upload_loads()            # the other instances' loads first, so this one is kept
         p, _ = self.inst[k]
         fo, nf = self.off[0][0][k], self.hier[p][1].n
         self.b[fo:fo + nf] = voxelhip_device(force, torch.float64, self.dev)

@@ -220,3 +220,29 @@ def test_batched_coarsest_solves_group_the_load_cases_of_a_matrix():
         A, n, ld, mask = mats[key[0]]
         ref = ops.dense_mv(A, rhs[key[0]][key[1]], torch.empty(n, device="cuda"), ops.Mask(L.MASK_ARRAY, mask), n)
         assert float((y - ref).abs().max() / ref.abs().max()) < 1e-6, key
+
+
+def test_set_load_replaces_one_instance_before_and_between_solves(batch_problems):
+    """set_load before the first solve and between solves: the replaced instance solves the new
+    load (same iterate as a batch built with that load), the others are unaffected."""
+    from voxelhip.batch import BatchProblem, BatchedGMGSolver
+    print("case set_load", flush=True)
+    rng = np.random.default_rng(21)
+    k, p = 1, 0                                    # instance 1 = problem 0, load case 1
+    shape = batch_problems[p].shape
+    f_new = point_loads(shape, 1, rng)[0]
+    bs = BatchedGMGSolver(batch_problems, KE, max_coarse_dofs=600, precision="mixed", graph=True)
+    bs.set_load(k, f_new)
+    xs, its, ok, _ = bs.solve(tol=1e-8, maxiter=100)
+    swapped = [BatchProblem(pb.shape, pb.E, pb.mask, list(pb.loads)) for pb in batch_problems]
+    swapped[p].loads[1] = f_new
+    ref = BatchedGMGSolver(swapped, KE, max_coarse_dofs=600, precision="mixed", graph=True)
+    xr, itr, okr, _ = ref.solve(tol=1e-8, maxiter=100)
+    assert ok.all() and okr.all() and (its == itr).all()
+    for a, b in zip(xs, xr):
+        assert torch.equal(a, b)
+    assert _oracle_residual(batch_problems[p], xs[k], f_new) <= 1.01e-8
+    # back to the original load between solves
+    bs.set_load(k, batch_problems[p].loads[1])
+    xs2, _, ok2, _ = bs.solve(tol=1e-8, maxiter=100)
+    assert ok2.all() and _oracle_residual(batch_problems[p], xs2[k], batch_problems[p].loads[1]) <= 1.01e-8
