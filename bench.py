@@ -248,6 +248,35 @@ def matvec_timing(shape, dtype, reps=20, plan=None):
 TILE_NAMES = {0: "auto", 1: "16x32", 2: "8x32", 3: "4x8", 4: "wide"}
 
 
+def slab_share(dtype, shape=(48, 192, 192), reps=20):
+    """One GPU's share of the 384x192x192 grid split into x-slabs (48x192x192: eight GPUs): the
+    product on the slab (tuned plan) and the two steps of the shared-plane exchange one GPU can
+    show, timed with HIP events: staging the boundary node plane (contiguous in the author layout,
+    3 (ny+1)(nz+1) values) into a send buffer, and adding a received plane into y."""
+    import torch
+    r = tuned_matvec(shape, dtype)
+    nplane = 3 * (shape[1] + 1) * (shape[2] + 1)
+    y = torch.randn((shape[0] + 1) * nplane, device="cuda", dtype=dtype)
+    send, recv = torch.empty(nplane, device="cuda", dtype=dtype), torch.randn(nplane, device="cuda", dtype=dtype)
+    out = {"product": r, "plane_values": nplane, "plane_bytes": nplane * y.element_size()}
+    for name, fn in (("stage_plane_us", lambda: send.copy_(y[-nplane:])),
+                     ("add_received_plane_us", lambda: y[:nplane].add_(recv))):
+        for _ in range(3):
+            fn()
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps * 1e3
+    # the transfer itself is not measurable on one GPU: at ~153 GB/s per xGMI link
+    out["xgmi_transfer_us_estimate"] = out["plane_bytes"] / 153e9 * 1e6
+    return out
+
+
 def tuned_matvec(shape, dtype, reps=20):
     """The product under the launch plan the library's tuner would pick for it: every (tile, lx)
     candidate of voxelhip.tuner and the automatic plan timed briefly, the four fastest timed
@@ -504,6 +533,9 @@ def main():
             mv[key] = tuned_matvec(sh, getattr(torch, d))
             mv[key]["auto_plan_us"] = matvec_timing(sh, getattr(torch, d))["us"]
     dom = mv["float64@256x128x128"]
+    for parts in (8, 4):
+        mv[f"x_slab_share_of_384x192x192_over_{parts}"] = {
+            d: slab_share(getattr(torch, d), shape=(384 // parts, 192, 192)) for d in ("float64", "float32")}
     progress("product table")
     cpu = None
     if args.cpu_baseline == "sample":
