@@ -288,22 +288,44 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // thread numbers; past the last column of a row (lz + 1 == EZ) there is none: the zero pad
   const bool zlast = lz + 1 >= EZ;
   const int w0 = zlast ? SCN - 1 : tid + EZ + 1, w1 = tid + EZ, w2 = zlast ? SCN - 1 : tid + 1, w3 = tid;
+  // (each operand's three components are read by adjacent loads of one basic block, so they merge
+  //  into one wide load per node; the previous Chebyshev iterate is read unconditionally and
+  //  discarded when the step does not use it)
+  const T* __restrict__ prev_src = (MODE == MODE_CHEB3 || MODE == MODE_CHEB3D) ? (a.prev ? a.prev : a.y) : nullptr;
+  const bool prev_used = a.cheb_c != T(0) && !(a.flags & MV_PREV_ZERO);
   auto fetch_epilogue = [&](int ix) {
     const bool use = node_own && ix >= ox0 && ix < ox1;
     const size_t node = use ? (size_t(ix) * ny1 + iy_own) * nz1 + iz_own : 0;
+    if (NEED_U && MASK == MASK_ARRAY) {         // the staged plane is masked: fetch the input
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (NEED_U && MASK == MASK_ARRAY) {       // the staged plane is masked: fetch the input
-        ep_u[c] = load_as<S, T>(u, 3 * node + c);
+      for (int c = 0; c < 3; ++c) ep_u[c] = load_as<S, T>(u, 3 * node + c);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
         if (AXPY) ep_u[c] += sw * w_at(3 * node + c);
         if (LOADOP == LOAD_MUL) ep_u[c] = sw * (ep_u[c] * wv[3 * node + c]);
       }
-      if (NEED_B) ep_b[c] = a.b[3 * node + c];
-      if (MODE == MODE_CHEB) ep_p[c] = a.d[3 * node + c];
-      if (MODE == MODE_CHEB3 || MODE == MODE_CHEB3D)
-        ep_p[c] = (a.cheb_c == T(0) || (a.flags & MV_PREV_ZERO)) ? T(0) : (a.prev ? a.prev : a.y)[3 * node + c];
-      if (MODE == MODE_CHEB3D) ep_z[c] = a.zold[3 * node + c];
-      if (NEEDS_DINV && !dinv_from_E) ep_di[c] = a.dinv[3 * node + c];
+    }
+    if (NEED_B) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep_b[c] = a.b[3 * node + c];
+    }
+    if (MODE == MODE_CHEB) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep_p[c] = a.d[3 * node + c];
+    }
+    if (MODE == MODE_CHEB3 || MODE == MODE_CHEB3D) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep_p[c] = prev_src[3 * node + c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep_p[c] = prev_used ? ep_p[c] : T(0);
+    }
+    if (MODE == MODE_CHEB3D) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep_z[c] = a.zold[3 * node + c];
+    }
+    if (NEEDS_DINV && !dinv_from_E) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep_di[c] = a.dinv[3 * node + c];
     }
   };
 
