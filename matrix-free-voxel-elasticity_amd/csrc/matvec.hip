@@ -172,20 +172,36 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   auto w_at = [&](size_t dof) -> T { return LOADOP == LOAD_AXPY32 ? T(wf[dof]) : wv[dof]; };
   const T sw = AXPY ? T(a.sdev[a.sidx]) : (LOADOP == LOAD_MUL ? a.load_scale : T(0));
 
-  // ---- loader geometry (plane independent): this thread's LOADS slots of the flat plane
-  // image (NY rows of 3*NZ contiguous values), their in-plane DOF offsets and validity.
-  int ld_off[LOADS];
+  // ---- loader geometry (plane independent): this thread's LSLOTS slots of the staged plane
+  // image (NY rows of NZ nodes, 3 values per node, contiguous along a row), their in-plane DOF
+  // offsets and validity.  FP32 passes load whole nodes (three adjacent values, one wide load per
+  // slot); FP64 passes load one value per slot, flat over the row image.
+  constexpr bool NODE_LD = sizeof(T) == 4;
+  constexpr int NNODE = TL::NY * NZ;
+  constexpr int LSLOTS = NODE_LD ? (NNODE + BLOCK - 1) / BLOCK : LOADS;
+  constexpr int LW = NODE_LD ? 3 : 1;               // values per slot
+  constexpr int NSLOT = NODE_LD ? NNODE : NLOAD;    // slots of the image
+  int ld_off[LSLOTS];
   unsigned ld_okbits = 0;
 #pragma unroll
-  for (int j = 0; j < LOADS; ++j) {
+  for (int j = 0; j < LSLOTS; ++j) {
     const int k = tid + j * BLOCK;
-    const int row = k / ROWLEN;
-    const int col = k - row * ROWLEN;
-    const int iy = ybase + row;
-    const int iz = zbase + col / 3;
-    const bool ok = k < (tye + 2) * ROWLEN && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
-    ld_okbits |= unsigned(ok) << j;
-    ld_off[j] = ok ? 3 * (iy * nz1 + zbase) + col : 0;
+    if constexpr (NODE_LD) {
+      const int row = k / NZ;
+      const int iy = ybase + row;
+      const int iz = zbase + (k - row * NZ);
+      const bool ok = k < (tye + 2) * NZ && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
+      ld_okbits |= unsigned(ok) << j;
+      ld_off[j] = ok ? 3 * (iy * nz1 + iz) : 0;
+    } else {
+      const int row = k / ROWLEN;
+      const int col = k - row * ROWLEN;
+      const int iy = ybase + row;
+      const int iz = zbase + col / 3;
+      const bool ok = k < (tye + 2) * ROWLEN && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
+      ld_okbits |= unsigned(ok) << j;
+      ld_off[j] = ok ? 3 * (iy * nz1 + zbase) + col : 0;
+    }
   }
   const int e_off = col_ok ? ey * nz + ez : 0;
   const size_t pstride = size_t(3) * ny1 * nz1;     // DOFs per node plane
@@ -195,9 +211,9 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // ix-1: out-of-range slots load a clamped in-bounds address, and the zeroing of those slots
   // is deferred to commit(), so no instruction consumes a loaded value before the next step
   // (the batch stays in flight through this step's compute).
-  T pre[LOADS];
-  T pre_w[COMBINE ? LOADS : 1];
-  uint8_t pre_m[LOADS];
+  T pre[LSLOTS][LW];
+  T pre_w[COMBINE ? LSLOTS : 1][LW];
+  uint8_t pre_m[LSLOTS][LW];
   unsigned pre_keep = 0;
   T pre_E;
   bool pre_E_ok = false;
@@ -208,11 +224,18 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     const size_t base = (ix >= 0 && ix <= nx) ? size_t(ix) * pstride : 0;
     unsigned keep = 0;
 #pragma unroll
-    for (int j = 0; j < LOADS; ++j) {
-      const size_t dof = base + ld_off[j];
-      pre[j] = load_as<S, T>(u, dof);
-      if (COMBINE) pre_w[j] = w_at(dof);
-      if (MASK == MASK_ARRAY) pre_m[j] = a.mask[dof];
+    for (int j = 0; j < LSLOTS; ++j) {
+      // (the last slot row is partly beyond the image: waves wholly past it skip the load)
+      if (j < LSLOTS - 1 || tid + j * BLOCK < NSLOT) {
+        const size_t dof = base + ld_off[j];
+#pragma unroll
+        for (int c = 0; c < LW; ++c) pre[j][c] = load_as<S, T>(u, dof + c);
+#pragma unroll
+        for (int c = 0; c < LW; ++c) {
+          if (COMBINE) pre_w[j][c] = w_at(dof + c);
+          if (MASK == MASK_ARRAY) pre_m[j][c] = a.mask[dof + c];
+        }
+      }
       keep |= unsigned(plane_ok && ((ld_okbits >> j) & 1u)) << j;
     }
     pre_keep = keep;
@@ -228,12 +251,18 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     sE[tid] = pre_E_ok ? pre_E : T(0);
     T* flat = &su[ix & 1][0];
 #pragma unroll
-    for (int j = 0; j < LOADS; ++j) {
-      bool keep = (pre_keep >> j) & 1u;
-      if (MASK == MASK_ARRAY) keep = keep && pre_m[j] != 0;
-      const T v = AXPY ? pre[j] + sw * pre_w[j]
-                  : (LOADOP == LOAD_MUL ? sw * (pre[j] * pre_w[j]) : pre[j]);
-      if (tid + j * BLOCK < NLOAD) flat[tid + j * BLOCK] = keep ? v : T(0);
+    for (int j = 0; j < LSLOTS; ++j) {
+      const int k = tid + j * BLOCK;
+      if (j < LSLOTS - 1 || k < NSLOT) {
+#pragma unroll
+        for (int c = 0; c < LW; ++c) {
+          bool keep = (pre_keep >> j) & 1u;
+          if (MASK == MASK_ARRAY) keep = keep && pre_m[j][c] != 0;
+          const T v = AXPY ? pre[j][c] + sw * pre_w[j][c]
+                      : (LOADOP == LOAD_MUL ? sw * (pre[j][c] * pre_w[j][c]) : pre[j][c]);
+          flat[LW * k + c] = keep ? v : T(0);
+        }
+      }
     }
   };
   // 2x2 (y,z) transform of this column's face of node plane `plane`: F[q], q = py + 2 pz.
