@@ -13,7 +13,6 @@ struct Tile {
   static constexpr int NCOL = EY * EZ;
   static constexpr int BLOCK = ((NCOL + 63) / 64) * 64;
   static constexpr int PLANE = NY * NZ;
-  static constexpr int LOADS = (NY * 3 * NZ + BLOCK - 1) / BLOCK;
 };
 constexpr int MV_PREV_ZERO = 1;
 constexpr int MV_W_F32 = 2;       // the AXPY operand w of an FP64 pass is stored in FP32

@@ -20,8 +20,9 @@
 //    + 12 FP64 operations (forward, blocks, modulus, inverse, carry) instead of 240 with an
 //    8-lane shuffle butterfly.
 //  * Node planes are staged through LDS in their HBM order (rows of (x, y, z) triples: lane k
-//    loads and stores flat element k, so both the loads and the LDS stores are unit-stride)
-//    with a one-plane register prefetch.  Each element column stores its four corner
+//    loads node k of the staged plane with one wide load and stores its three values, so the
+//    loads are contiguous across a wave and the LDS stores conflict-free) with a one-plane
+//    register prefetch.  Each element column stores its four corner
 //    contributions in LDS slots numbered by its thread, and the owner of a node gathers them
 //    from its four incident columns in a fixed order (unit-stride writes and reads across a
 //    wave; results are bitwise reproducible).
@@ -119,8 +120,7 @@ __device__ __forceinline__ float reciprocal(float x) { return __builtin_amdgcn_r
 // chunk) of the grid described by `a`.
 template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP>
 __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int logical) {
-  constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE, LOADS = TL::LOADS;
-  constexpr int ROWLEN = 3 * NZ, NLOAD = TL::NY * ROWLEN;
+  constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE;
   __shared__ T su[2][3 * PLANE];      // staged node planes ix & 1 (input), AoS rows as in HBM
   // corner contributions and column modulus sums are indexed by the writing thread (unit stride
   // for the writes and for the owner's four reads), plus one zero pad slot past every writer
@@ -174,34 +174,22 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 
   // ---- loader geometry (plane independent): this thread's LSLOTS slots of the staged plane
   // image (NY rows of NZ nodes, 3 values per node, contiguous along a row), their in-plane DOF
-  // offsets and validity.  FP32 passes load whole nodes (three adjacent values, one wide load per
-  // slot); FP64 passes load one value per slot, flat over the row image.
-  constexpr bool NODE_LD = sizeof(T) == 4;
+  // offsets and validity.  A slot is a whole node, so its three adjacent values are read by one
+  // wide load (single values over the flat row image, coalesced as well, made the solves 2-3 %
+  // slower: three times the vector memory instructions).
   constexpr int NNODE = TL::NY * NZ;
-  constexpr int LSLOTS = NODE_LD ? (NNODE + BLOCK - 1) / BLOCK : LOADS;
-  constexpr int LW = NODE_LD ? 3 : 1;               // values per slot
-  constexpr int NSLOT = NODE_LD ? NNODE : NLOAD;    // slots of the image
+  constexpr int LSLOTS = (NNODE + BLOCK - 1) / BLOCK;
   int ld_off[LSLOTS];
   unsigned ld_okbits = 0;
 #pragma unroll
   for (int j = 0; j < LSLOTS; ++j) {
     const int k = tid + j * BLOCK;
-    if constexpr (NODE_LD) {
-      const int row = k / NZ;
-      const int iy = ybase + row;
-      const int iz = zbase + (k - row * NZ);
-      const bool ok = k < (tye + 2) * NZ && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
-      ld_okbits |= unsigned(ok) << j;
-      ld_off[j] = ok ? 3 * (iy * nz1 + iz) : 0;
-    } else {
-      const int row = k / ROWLEN;
-      const int col = k - row * ROWLEN;
-      const int iy = ybase + row;
-      const int iz = zbase + col / 3;
-      const bool ok = k < (tye + 2) * ROWLEN && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
-      ld_okbits |= unsigned(ok) << j;
-      ld_off[j] = ok ? 3 * (iy * nz1 + zbase) + col : 0;
-    }
+    const int row = k / NZ;
+    const int iy = ybase + row;
+    const int iz = zbase + (k - row * NZ);
+    const bool ok = k < (tye + 2) * NZ && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
+    ld_okbits |= unsigned(ok) << j;
+    ld_off[j] = ok ? 3 * (iy * nz1 + iz) : 0;
   }
   const int e_off = col_ok ? ey * nz + ez : 0;
   const size_t pstride = size_t(3) * ny1 * nz1;     // DOFs per node plane
@@ -211,9 +199,9 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // ix-1: out-of-range slots load a clamped in-bounds address, and the zeroing of those slots
   // is deferred to commit(), so no instruction consumes a loaded value before the next step
   // (the batch stays in flight through this step's compute).
-  T pre[LSLOTS][LW];
-  T pre_w[COMBINE ? LSLOTS : 1][LW];
-  uint8_t pre_m[LSLOTS][LW];
+  T pre[LSLOTS][3];
+  T pre_w[COMBINE ? LSLOTS : 1][3];
+  uint8_t pre_m[LSLOTS][3];
   unsigned pre_keep = 0;
   T pre_E;
   bool pre_E_ok = false;
@@ -226,12 +214,12 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 #pragma unroll
     for (int j = 0; j < LSLOTS; ++j) {
       // (the last slot row is partly beyond the image: waves wholly past it skip the load)
-      if (j < LSLOTS - 1 || tid + j * BLOCK < NSLOT) {
+      if (j < LSLOTS - 1 || tid + j * BLOCK < NNODE) {
         const size_t dof = base + ld_off[j];
 #pragma unroll
-        for (int c = 0; c < LW; ++c) pre[j][c] = load_as<S, T>(u, dof + c);
+        for (int c = 0; c < 3; ++c) pre[j][c] = load_as<S, T>(u, dof + c);
 #pragma unroll
-        for (int c = 0; c < LW; ++c) {
+        for (int c = 0; c < 3; ++c) {
           if (COMBINE) pre_w[j][c] = w_at(dof + c);
           if (MASK == MASK_ARRAY) pre_m[j][c] = a.mask[dof + c];
         }
@@ -253,14 +241,14 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 #pragma unroll
     for (int j = 0; j < LSLOTS; ++j) {
       const int k = tid + j * BLOCK;
-      if (j < LSLOTS - 1 || k < NSLOT) {
+      if (j < LSLOTS - 1 || k < NNODE) {
 #pragma unroll
-        for (int c = 0; c < LW; ++c) {
+        for (int c = 0; c < 3; ++c) {
           bool keep = (pre_keep >> j) & 1u;
           if (MASK == MASK_ARRAY) keep = keep && pre_m[j][c] != 0;
           const T v = AXPY ? pre[j][c] + sw * pre_w[j][c]
                       : (LOADOP == LOAD_MUL ? sw * (pre[j][c] * pre_w[j][c]) : pre[j][c]);
-          flat[LW * k + c] = keep ? v : T(0);
+          flat[3 * k + c] = keep ? v : T(0);
         }
       }
     }
