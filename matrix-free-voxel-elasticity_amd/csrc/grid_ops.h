@@ -34,18 +34,18 @@ struct RestrictCfg {
   static constexpr int RB = RB_;   // coarse rows per block
   static constexpr int CZ = CZ_;   // coarse nodes per z segment (at most)
   static constexpr int NT = NT_;   // threads per workgroup
-  static constexpr int SLOTS = ((2 * RB + 1) * 3 * (2 * CZ + 1) + NT - 1) / NT;   // fine values per thread
-  static constexpr int U = U_ < SLOTS ? U_ : SLOTS;                             // slots with loads in flight
+  static constexpr int SLOTS = ((2 * RB + 1) * (2 * CZ + 1) + NT - 1) / NT;     // fine nodes per thread
+  static constexpr int U = U_ < SLOTS ? U_ : SLOTS;                             // nodes with loads in flight
 };
 template <int PB_, int FZ_, int NT_>
 struct ProlongCfg {
   static constexpr int PB = PB_;   // fine rows per block (even)
   static constexpr int FZ = FZ_;   // fine nodes per z segment (at most; even)
   static constexpr int NT = NT_;
-  static constexpr int PF = (PB * 3 * FZ + NT - 1) / NT;                          // fine values per thread
-  static constexpr int CS = ((PB / 2 + 1) * 3 * (FZ / 2 + 1) + NT - 1) / NT;      // coarse values per thread
+  static constexpr int PF = (PB * FZ + NT - 1) / NT;                              // fine nodes per thread
+  static constexpr int CS = ((PB / 2 + 1) * (FZ / 2 + 1) + NT - 1) / NT;          // coarse nodes per thread
 };
-using TR_RESTRICT = RestrictCfg<4, 80, 256, 4>;
+using TR_RESTRICT = RestrictCfg<4, 80, 256, 6>;
 using TR_PROLONG = ProlongCfg<4, 160, 256>;
 
 // Walks the slots s = start, start + step, ... of a block laid out as rows of `len` values,
@@ -104,15 +104,16 @@ __host__ __device__ inline TransferGeom prolong_geom(int nx, int ny, int nz, int
 
 // Rows per block (host side; the kernels take the result as an argument).  RB / PB, or more
 // where the z segment is short (narrow grids, coarse levels, the small problems of a batch), so
-// that a block still stages about 2 U NT fine values (restriction) or fills its PF fine and CS
-// coarse values per thread (prolongation: rows 3 seg <= PF NT and (rows / 2 + 1) 3 (seg / 2 + 1)
-// <= CS NT; PB meets both since seg <= FZ).  The rows grow only while the launch keeps at least
-// min_blocks blocks (0: no limit, for a batch whose problems fill the chip together).
+// that a block still stages about U NT fine nodes (restriction: one round of loads) or fills its
+// PF fine and CS coarse nodes per thread (prolongation: rows seg <= PF NT and (rows / 2 + 1)
+// (seg / 2 + 1) <= CS NT; PB meets both since seg <= FZ).  The rows grow only while the launch
+// keeps at least min_blocks blocks (0: no limit, for a batch whose problems fill the chip
+// together).
 template <typename C>
 inline int restrict_rows(int nx, int ny, int nz, int min_blocks) {
   const TransferGeom g = restrict_geom<C>(nx, ny, nz, C::RB);
-  const int flen = 3 * (2 * g.seg + 1);
-  int rows = ((2 * C::U * C::NT) / flen - 1) / 2;
+  const int fnodes = 2 * g.seg + 1;
+  int rows = ((C::U * C::NT) / fnodes - 1) / 2;
   while (rows > C::RB && restrict_geom<C>(nx, ny, nz, rows).nblocks < min_blocks) --rows;
   return rows > C::RB ? rows : C::RB;
 }
@@ -120,8 +121,8 @@ inline int restrict_rows(int nx, int ny, int nz, int min_blocks) {
 template <typename C>
 inline int prolong_rows(int nx, int ny, int nz, int min_blocks) {
   const TransferGeom g = prolong_geom<C>(nx, ny, nz, C::PB);
-  const int by_fine = (C::PF * C::NT) / (3 * g.seg);
-  const int by_coarse = 2 * ((C::CS * C::NT) / (3 * (g.seg / 2 + 1))) - 2;
+  const int by_fine = (C::PF * C::NT) / g.seg;
+  const int by_coarse = 2 * ((C::CS * C::NT) / (g.seg / 2 + 1)) - 2;
   int rows = (by_fine < by_coarse ? by_fine : by_coarse) & ~1;
   while (rows > C::PB && prolong_geom<C>(nx, ny, nz, rows).nblocks < min_blocks) rows -= 2;
   return rows > C::PB ? rows : C::PB;
@@ -142,39 +143,67 @@ __device__ __forceinline__ void restrict_block(const T* __restrict__ rf, T* __re
   const int Iy0 = yb * g.rows, Iz0 = zb * g.seg;
   const int nrow = min(g.rows, g.ny1 - Iy0);       // coarse rows of the block
   const int ncz = min(g.seg, g.nz1 - Iz0);         // coarse nodes of the segment
-  const int flen = 3 * (2 * ncz + 1);              // fine nodes 2 Iz0 - 1 .. 2 (Iz0 + ncz) - 1
-  const int nslot = (2 * nrow + 1) * flen;         // fine rows 2 Iy0 - 1 .. 2 (Iy0 + nrow) - 1
-  const int fz3 = 3 * (nz + 1);
-  const size_t fplane = size_t(ny + 1) * fz3;
-  const int q0 = 3 * (2 * Iz0 - 1);                // row offset of the segment's first value
+  const int fnodes = 2 * ncz + 1;                  // fine nodes 2 Iz0 - 1 .. 2 (Iz0 + ncz) - 1
+  const int flen = 3 * fnodes;
+  const int nslot = (2 * nrow + 1) * fnodes;       // fine rows 2 Iy0 - 1 .. 2 (Iy0 + nrow) - 1
+  const int fz1 = nz + 1;
+  const size_t fplane = size_t(ny + 1) * 3 * fz1;
+  const int n0 = 2 * Iz0 - 1;                      // the segment's first fine node
   const int ixm = 2 * Ix - 1, ix0 = 2 * Ix, ixp = 2 * Ix + 1;
   const bool okm = ixm >= 0 && !(fmask_kind == MASK_X0 && ixm == 0);
   const bool ok0 = !(fmask_kind == MASK_X0 && ix0 == 0);
   const bool okp = ixp <= nx;
-  auto fetch = [&](size_t dof) -> T {
-    const T v = rf[dof];
-    return (fmask_kind == MASK_ARRAY && !fmask[dof]) ? T(0) : v;
-  };
-  RowCursor cur(threadIdx.x, NT, flen);
+  // a slot is one fine node of the block's rows: its three values are read by adjacent loads
+  // (one wide load per node and plane), every load of a round in flight together
+  RowCursor cur(threadIdx.x, NT, fnodes);
   for (int s0 = threadIdx.x; s0 < nslot; s0 += U * NT) {
-    T vm[U], v0[U], vp[U];
+    T vm[U][3], v0[U][3], vp[U][3];
+    size_t off[U];
+    bool in[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u * NT;
-      vm[u] = v0[u] = vp[u] = T(0);
-      const int iy = 2 * Iy0 - 1 + cur.r, q = q0 + cur.c;
+      const int iy = 2 * Iy0 - 1 + cur.r, iz = n0 + cur.c;
       cur.next();
-      if (s < nslot && iy >= 0 && iy <= ny && q >= 0 && q < fz3) {
-        const size_t off = size_t(iy) * fz3 + q;
-        if (okm) vm[u] = fetch(size_t(ixm) * fplane + off);
-        if (ok0) v0[u] = fetch(size_t(ix0) * fplane + off);
-        if (okp) vp[u] = fetch(size_t(ixp) * fplane + off);
+      in[u] = s < nslot && iy >= 0 && iy <= ny && iz >= 0 && iz <= nz;
+      off[u] = in[u] ? (size_t(iy) * fz1 + iz) * 3 : 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vm[u][c] = v0[u][c] = vp[u][c] = T(0);
+      if (in[u]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (okm) vm[u][c] = rf[size_t(ixm) * fplane + off[u] + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (ok0) v0[u][c] = rf[size_t(ix0) * fplane + off[u] + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (okp) vp[u][c] = rf[size_t(ixp) * fplane + off[u] + c];
+        }
+      }
+    }
+    if (fmask_kind == MASK_ARRAY) {                // the fine mask, applied to the loaded values
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (in[u]) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if (okm && !fmask[size_t(ixm) * fplane + off[u] + c]) vm[u][c] = T(0);
+            if (ok0 && !fmask[size_t(ix0) * fplane + off[u] + c]) v0[u][c] = T(0);
+            if (okp && !fmask[size_t(ixp) * fplane + off[u] + c]) vp[u][c] = T(0);
+          }
+        }
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u * NT;
-      if (s < nslot) xl[s] = T(0.5) * vm[u] + v0[u] + T(0.5) * vp[u];
+      if (s < nslot) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xl[3 * s + c] = T(0.5) * vm[u][c] + v0[u][c] + T(0.5) * vp[u][c];
+      }
     }
   }
   __syncthreads();
@@ -208,71 +237,97 @@ __device__ __forceinline__ void prolong_block(const T* __restrict__ ec, T* __res
   const int ix = blk / (g.nzb * g.nyb);
   const int iy0 = yb * g.rows, iz0 = zb * g.seg;  // both even
   const int nrow = min(g.rows, g.ny1 - iy0);
-  const int flen = 3 * min(g.seg, g.nz1 - iz0);
+  const int fn = min(g.seg, g.nz1 - iz0);          // fine nodes of the segment
   const int cy1 = ny / 2 + 1, cz1 = nz / 2 + 1;
   const int Y0 = iy0 >> 1, Z0 = iz0 >> 1;
   const int crow = min(g.rows / 2 + 1, cy1 - Y0);
-  const int clen = 3 * min(g.seg / 2 + 1, cz1 - Z0);
+  const int cn = min(g.seg / 2 + 1, cz1 - Z0);     // coarse nodes of a combined row
+  const int clen = 3 * cn;
   const int fz3 = 3 * g.nz1;
   const size_t cz3 = size_t(3) * cz1;
   const size_t fbase = (size_t(ix) * g.ny1 + iy0) * fz3 + 3 * size_t(iz0);
-  // every load of the block is issued before the first wait: the fine values this thread
-  // updates, then the coarse values of the x-combined rows
-  T xold[PF];
-  int fo[PF];                                      // offsets of this thread's fine values
+  // every load of the block is issued before the first wait: the fine nodes this thread
+  // updates, then the coarse nodes of the x-combined rows; a slot is one node, its three
+  // values read (and the fine ones written) by adjacent accesses, one wide access per node
+  T xold[PF][3];
+  int fo[PF];                                      // offsets of this thread's fine nodes
   {
-    RowCursor fc(threadIdx.x, NT, flen);
+    RowCursor fc(threadIdx.x, NT, fn);
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
-      fo[j] = fc.r < nrow ? fc.r * fz3 + fc.c : -1;
+      fo[j] = fc.r < nrow ? fc.r * fz3 + 3 * fc.c : -1;
       fc.next();
     }
   }
 #pragma unroll
-  for (int j = 0; j < PF; ++j) xold[j] = (!assign && fo[j] >= 0) ? xf[fbase + fo[j]] : T(0);
+  for (int j = 0; j < PF; ++j) {
+    const bool ld = !assign && fo[j] >= 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xold[j][c] = T(0);
+    if (ld) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xold[j][c] = xf[fbase + fo[j] + c];
+    }
+  }
   const int x0 = ix >> 1, x1 = (ix + 1) >> 1;
   const bool xodd = ix & 1;
   const T* c0 = ec + (size_t(x0) * cy1 + Y0) * cz3 + 3 * size_t(Z0);
   const T* c1 = ec + (size_t(x1) * cy1 + Y0) * cz3 + 3 * size_t(Z0);
-  const int ncs = crow * clen;                     // <= CS * NT
-  T ca[CS], cb[CS];
+  const int ncs = crow * cn;                       // <= CS * NT
+  T ca[CS][3], cb[CS][3];
   {
-    RowCursor cc(threadIdx.x, NT, clen);
+    RowCursor cc(threadIdx.x, NT, cn);
 #pragma unroll
     for (int u = 0; u < CS; ++u) {
       const int s = threadIdx.x + u * NT;
-      const size_t off = size_t(cc.r) * cz3 + cc.c;
+      const size_t off = size_t(cc.r) * cz3 + 3 * cc.c;
       cc.next();
-      ca[u] = s < ncs ? c0[off] : T(0);
-      cb[u] = (s < ncs && xodd) ? c1[off] : T(0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ca[u][c] = cb[u][c] = T(0);
+      if (s < ncs) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ca[u][c] = c0[off + c];
+      }
+      if (s < ncs && xodd) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cb[u][c] = c1[off + c];
+      }
     }
   }
 #pragma unroll
   for (int u = 0; u < CS; ++u) {
     const int s = threadIdx.x + u * NT;
-    if (s < ncs) cl[s] = xodd ? T(0.5) * (ca[u] + cb[u]) : ca[u];
+    if (s < ncs) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cl[3 * s + c] = xodd ? T(0.5) * (ca[u][c] + cb[u][c]) : ca[u][c];
+    }
   }
   __syncthreads();
   const bool plane_free = !(mask_kind == MASK_X0 && ix == 0);
-  RowCursor fc(threadIdx.x, NT, flen);
+  RowCursor fc(threadIdx.x, NT, fn);
 #pragma unroll
   for (int j = 0; j < PF; ++j) {
-    const int r = fc.r, c = fc.c;
+    const int r = fc.r, zl = fc.c;
     fc.next();
     if (fo[j] >= 0) {
-      const int zl = c / 3, k = c - 3 * (c / 3);
-      const T* p = cl + (r >> 1) * clen + 3 * (zl >> 1) + k;
-      T v = p[0];
-      if (zl & 1) v = T(0.5) * (v + p[3]);
-      if (r & 1) {
-        T w = p[clen];
-        if (zl & 1) w = T(0.5) * (w + p[clen + 3]);
-        v = T(0.5) * (v + w);
+      T out[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T* p = cl + (r >> 1) * clen + 3 * (zl >> 1) + k;
+        T v = p[0];
+        if (zl & 1) v = T(0.5) * (v + p[3]);
+        if (r & 1) {
+          T w = p[clen];
+          if (zl & 1) w = T(0.5) * (w + p[clen + 3]);
+          v = T(0.5) * (v + w);
+        }
+        const size_t fi = fbase + fo[j] + k;
+        bool free_dof = plane_free;
+        if (mask_kind == MASK_ARRAY) free_dof = mask[fi] != 0;
+        out[k] = (assign ? T(0) : xold[j][k]) + (free_dof ? v : T(0));
       }
-      const size_t fi = fbase + fo[j];
-      bool free_dof = plane_free;
-      if (mask_kind == MASK_ARRAY) free_dof = mask[fi] != 0;
-      xf[fi] = (assign ? T(0) : xold[j]) + (free_dof ? v : T(0));
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xf[fbase + fo[j] + k] = out[k];
     }
   }
 }
