@@ -196,45 +196,73 @@ __global__ void __launch_bounds__(BLAS_BLOCK) cvt_dotdiff_kernel(size_t n, const
   }
 }
 
-// Dense symmetric matrix-vector product y = mask * (A r): one wave per row (coarsest level).
-// y = mask * (A r): one wave per row, 4 rows per workgroup.  A is row-major with leading
+// Dense matrix-vector product y = mask * (A r) of the coarsest level: each wave takes
+// DENSE_ROWS consecutive rows (a chunk of r a lane reads serves all of them), 4 waves per
+// workgroup, and issues the loads of DENSE_UNROLL chunks before it accumulates them (one row per
+// wave with eight chunks in flight measured fastest: 17.4 us for the 4,131-row FP32 inverse, 2-3
+// rows per wave or fewer chunks in flight were no faster).  A is row-major with leading
 // dimension ld (a multiple of 4, rows 16-byte aligned), read with 16-byte loads; the ld - n
 // padding columns are never read.  Accumulation in the matrix type (FP64 for the FP64 coarsest
-// inverse, also when the level vectors are FP32).
+// inverse, also when the level vectors are FP32), one accumulator per row and lane in column
+// order.
+constexpr int DENSE_ROWS = 1, DENSE_UNROLL = 8;
 template <typename TA, typename TV>
 __global__ void __launch_bounds__(256) dense_mv_kernel(int n, int ld, const TA* __restrict__ A,
                                                       const TV* __restrict__ r, TV* __restrict__ y,
                                                       const uint8_t* __restrict__ mask) {
   constexpr int V = 16 / int(sizeof(TA));
+  constexpr int R = DENSE_ROWS, UN = DENSE_UNROLL;
   using Vec = typename std::conditional<sizeof(TA) == 8, double2, float4>::type;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * R;
   const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const TA* a = A + size_t(row) * ld;
-  const int nfull = (n / V) * V;
-  TA acc0 = TA(0), acc1 = TA(0);
-  int j = lane * V;
-#pragma unroll 2
-  for (; j + 64 * V < nfull; j += 128 * V) {      // two independent 16-byte loads in flight
-    const Vec x0 = *reinterpret_cast<const Vec*>(a + j);
-    const Vec x1 = *reinterpret_cast<const Vec*>(a + j + 64 * V);
-    const TA* e0 = reinterpret_cast<const TA*>(&x0);
-    const TA* e1 = reinterpret_cast<const TA*>(&x1);
+  if (row0 >= n) return;
+  const TA* a[R];
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-      acc0 += e0[k] * TA(r[j + k]);
-      acc1 += e1[k] * TA(r[j + 64 * V + k]);
+  for (int q = 0; q < R; ++q) a[q] = A + size_t(min(row0 + q, n - 1)) * ld;   // rows past n: not stored
+  const int nfull = (n / V) * V;
+  TA acc[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) acc[q] = TA(0);
+  int j = lane * V;
+  for (; j + (UN - 1) * 64 * V < nfull; j += UN * 64 * V) {
+    Vec x[UN][R];
+    TA rv[UN][V];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) x[u][q] = *reinterpret_cast<const Vec*>(a[q] + j + u * 64 * V);
+#pragma unroll
+      for (int k = 0; k < V; ++k) rv[u][k] = TA(r[j + u * 64 * V + k]);
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u)
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const TA* e = reinterpret_cast<const TA*>(&x[u][q]);
+#pragma unroll
+        for (int k = 0; k < V; ++k) acc[q] += e[k] * rv[u][k];
+      }
+  }
+  for (; j < nfull; j += 64 * V) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const Vec x = *reinterpret_cast<const Vec*>(a[q] + j);
+      const TA* e = reinterpret_cast<const TA*>(&x);
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[q] += e[k] * TA(r[j + k]);
     }
   }
-  if (j < nfull) {
-    const Vec x0 = *reinterpret_cast<const Vec*>(a + j);
-    const TA* e0 = reinterpret_cast<const TA*>(&x0);
+  for (int t = nfull + lane; t < n; t += 64) {
+    const TA rt = TA(r[t]);
 #pragma unroll
-    for (int k = 0; k < V; ++k) acc0 += e0[k] * TA(r[j + k]);
+    for (int q = 0; q < R; ++q) acc[q] += a[q][t] * rt;
   }
-  for (int t = nfull + lane; t < n; t += 64) acc1 += a[t] * TA(r[t]);
-  TA acc = wave_sum(acc0 + acc1);
-  if (lane == 0) y[row] = (mask && !mask[row]) ? TV(0) : TV(acc);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const TA sum = wave_sum(acc[q]);
+    const int row = row0 + q;
+    if (lane == 0 && row < n) y[row] = (mask && !mask[row]) ? TV(0) : TV(sum);
+  }
 }
 
 // --------------------------------------------------------------------------- launchers
@@ -321,7 +349,9 @@ template int cvt_dotdiff_launch<float, float>(size_t, const float*, const float*
 
 template <typename TA, typename TV>
 int dense_mv_launch(int n, int ld, const TA* A, const TV* r, TV* y, const uint8_t* mask, hipStream_t st) {
-  hipLaunchKernelGGL((dense_mv_kernel<TA, TV>), dim3((n + 3) / 4), dim3(256), 0, st, n, ld, A, r, y, mask);
+  const int rows_per_block = 4 * DENSE_ROWS;
+  hipLaunchKernelGGL((dense_mv_kernel<TA, TV>), dim3((n + rows_per_block - 1) / rows_per_block), dim3(256), 0, st, n, ld,
+                     A, r, y, mask);
   return hip_status(hipGetLastError(), "dense_mv_kernel");
 }
 template int dense_mv_launch<double, double>(int, int, const double*, const double*, double*, const uint8_t*, hipStream_t);
