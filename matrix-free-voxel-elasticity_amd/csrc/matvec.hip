@@ -426,6 +426,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         const T esum = ((se[w3] + se[w1]) + se[w2]) + se[w0];
         rsum = reciprocal(esum);
       }
+      T outv[3], dv[3] = {T(0), T(0), T(0)};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const T ku = ((part[c][0] + part[c][1]) + part[c][2]) + part[c][3];
@@ -454,7 +455,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
             out = uo + a.omega * z;
           } else if (MODE == MODE_CHEB) {
             const T d = a.cheb_a * (a.cheb_c == T(0) ? z : z + a.cheb_c * ep_p[c]);
-            a.d[dof] = d;
+            dv[c] = d;
             out = uo + d;
           } else {
             out = uo + a.cheb_a * (a.cheb_c == T(0) ? z : z + a.cheb_c * (uo - ep_p[c]));
@@ -469,9 +470,25 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
           dot_acc += rb * zo;
           dot_acc2 += rb * (zo - double(ep_z[c]));
         }
-        if ((MODE != MODE_DOT && MODE != MODE_RESID) || a.y) a.y[dof] = out;   // DOT / RESID: y optional
-        if (COMBINE) a.u_out[dof] = uo;
-        if (MODE == MODE_RESID && a.y32) a.y32[dof] = float(out);
+        outv[c] = out;
+      }
+      // each output's three components by adjacent stores (one wide store per node)
+      const size_t d0 = 3 * node;
+      if (MODE == MODE_CHEB) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.d[d0 + c] = dv[c];
+      }
+      if ((MODE != MODE_DOT && MODE != MODE_RESID) || a.y) {   // DOT / RESID: y optional
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.y[d0 + c] = outv[c];
+      }
+      if (COMBINE) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.u_out[d0 + c] = ep_u[c];
+      }
+      if (MODE == MODE_RESID && a.y32) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.y32[d0 + c] = float(outv[c]);
       }
     }
 #pragma unroll
