@@ -19,10 +19,10 @@
 //    half of element ex-1 before the (y,z) inverse.  Per element this is 48 + 72 + 24 + 48
 //    + 12 FP64 operations (forward, blocks, modulus, inverse, carry) instead of 240 with an
 //    8-lane shuffle butterfly.
-//  * Node planes are staged through LDS in their HBM order (rows of (x, y, z) triples: lane k
-//    loads node k of the staged plane with one wide load and stores its three values, so the
-//    loads are contiguous across a wave and the LDS stores conflict-free) with a one-plane
-//    register prefetch.  Each element column stores its four corner
+//  * Node planes are staged through LDS with a one-plane register prefetch: lane k loads node k
+//    of the staged plane (rows of (x, y, z) triples, contiguous across a wave) with one wide load
+//    and stores its three values into three per-component arrays, so the stores and the reads
+//    of neighbouring nodes are unit-stride in LDS.  Each element column stores its four corner
 //    contributions in LDS slots numbered by its thread, and the owner of a node gathers them
 //    from its four incident columns in a fixed order (unit-stride writes and reads across a
 //    wave; results are bitwise reproducible).
@@ -121,7 +121,10 @@ __device__ __forceinline__ float reciprocal(float x) { return __builtin_amdgcn_r
 template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP>
 __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int logical) {
   constexpr int NZ = TL::NZ, EZ = TL::EZ, BLOCK = TL::BLOCK, PLANE = TL::PLANE;
-  __shared__ T su[2][3 * PLANE];      // staged node planes ix & 1 (input), AoS rows as in HBM
+  // staged node planes ix & 1 (input), one array per component (node-major within it): reads
+  // of neighbouring nodes by neighbouring lanes are unit-stride (FP64 values at the 24-byte
+  // stride of the HBM layout put two lanes of a wave on one LDS bank)
+  __shared__ T su[2][3][PLANE];
   // corner contributions and column modulus sums are indexed by the writing thread (unit stride
   // for the writes and for the owner's four reads), plus one zero pad slot past every writer
   constexpr int SCN = BLOCK + EZ + 2;
@@ -237,7 +240,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     // a register, the compiler may keep it live past the next load and then copy it, waiting for
     // every load in flight)
     sE[tid] = pre_E_ok ? pre_E : T(0);
-    T* flat = &su[ix & 1][0];
+    T (*pl)[PLANE] = su[ix & 1];
 #pragma unroll
     for (int j = 0; j < LSLOTS; ++j) {
       const int k = tid + j * BLOCK;
@@ -248,7 +251,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
           if (MASK == MASK_ARRAY) keep = keep && pre_m[j][c] != 0;
           const T v = AXPY ? pre[j][c] + sw * pre_w[j][c]
                       : (LOADOP == LOAD_MUL ? sw * (pre[j][c] * pre_w[j][c]) : pre[j][c]);
-          flat[3 * k + c] = keep ? v : T(0);
+          pl[c][k] = keep ? v : T(0);
         }
       }
     }
@@ -260,10 +263,10 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     T w[3][4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      w[c][0] = su[pb][3 * i00 + c];
-      w[c][1] = su[pb][3 * (i00 + 1) + c];
-      w[c][2] = su[pb][3 * (i00 + NZ) + c];
-      w[c][3] = su[pb][3 * (i00 + NZ + 1) + c];
+      w[c][0] = su[pb][c][i00];
+      w[c][1] = su[pb][c][i00 + 1];
+      w[c][2] = su[pb][c][i00 + NZ];
+      w[c][3] = su[pb][c][i00 + NZ + 1];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -408,7 +411,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     // step overwrite this plane buffer
     if (NEED_U && MASK != MASK_ARRAY && finalize && node_own) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ep_u[c] = su[ex & 1][3 * jn_own + c];
+      for (int c = 0; c < 3; ++c) ep_u[c] = su[ex & 1][c][jn_own];
     }
     __syncthreads();
     if (finalize && node_own) {
