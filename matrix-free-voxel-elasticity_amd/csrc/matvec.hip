@@ -203,7 +203,9 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // is deferred to commit(), so no instruction consumes a loaded value before the next step
   // (the batch stays in flight through this step's compute).
   T pre[LSLOTS][3];
-  T pre_w[COMBINE ? LSLOTS : 1][3];
+  // (an FP32 operand w stays FP32 in its prefetch registers until the commit converts it)
+  using WT = std::conditional_t<LOADOP == LOAD_AXPY32, float, T>;
+  WT pre_w[COMBINE ? LSLOTS : 1][3];
   uint8_t pre_m[LSLOTS][3];
   unsigned pre_keep = 0;
   T pre_E;
@@ -223,7 +225,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         for (int c = 0; c < 3; ++c) pre[j][c] = load_as<S, T>(u, dof + c);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          if (COMBINE) pre_w[j][c] = w_at(dof + c);
+          if (COMBINE) pre_w[j][c] = LOADOP == LOAD_AXPY32 ? WT(wf[dof + c]) : WT(wv[dof + c]);
           if (MASK == MASK_ARRAY) pre_m[j][c] = a.mask[dof + c];
         }
       }
@@ -249,8 +251,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         for (int c = 0; c < 3; ++c) {
           bool keep = (pre_keep >> j) & 1u;
           if (MASK == MASK_ARRAY) keep = keep && pre_m[j][c] != 0;
-          const T v = AXPY ? pre[j][c] + sw * pre_w[j][c]
-                      : (LOADOP == LOAD_MUL ? sw * (pre[j][c] * pre_w[j][c]) : pre[j][c]);
+          const T v = AXPY ? pre[j][c] + sw * T(pre_w[j][c])
+                      : (LOADOP == LOAD_MUL ? sw * (pre[j][c] * T(pre_w[j][c])) : pre[j][c]);
           pl[c][k] = keep ? v : T(0);
         }
       }
@@ -519,8 +521,13 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   }
 }
 
+// Resident waves per SIMD the register allocation must allow: 3 for FP64 (at most 168 VGPRs), 4 for FP32
+// (at most 128).  A workgroup of 5 waves (the 8 x 32 and 16 x 16 tiles) would otherwise let the compiler
+// use up to 256 VGPRs, and past 168 only one such workgroup fits on a CU instead of two.
+template <typename T> constexpr int kMinWaves = sizeof(T) == 8 ? 3 : 4;
+
 template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP = LOAD_NONE>
-__global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
+__global__ void __launch_bounds__(TL::BLOCK, kMinWaves<T>) parity_xmarch(MatvecParams<T> a) {
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   if (logical >= a.n_tiles) return;
   xmarch_tile<T, S, MODE, MASK, TL, CUBE, LOADOP>(a, logical);
@@ -529,7 +536,7 @@ __global__ void __launch_bounds__(TL::BLOCK) parity_xmarch(MatvecParams<T> a) {
 // Batched product: problem i owns the logical tiles [offs[i], offs[i+1]) of one launch and
 // reads its parameters (pointers, grid, plan, epilogue scalars, coefficients) from descs[i].
 template <typename T, typename S, int MODE, int MASK, typename TL, bool CUBE, int LOADOP = LOAD_NONE>
-__global__ void __launch_bounds__(TL::BLOCK) parity_xmarch_batch(const MatvecParams<T>* __restrict__ descs,
+__global__ void __launch_bounds__(TL::BLOCK, kMinWaves<T>) parity_xmarch_batch(const MatvecParams<T>* __restrict__ descs,
                                                                  const int* __restrict__ offs, int n) {
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   if (logical >= offs[n]) return;
