@@ -61,10 +61,25 @@ __global__ void __launch_bounds__(SEG_BLOCK) seg_reduce_kernel(const long long* 
   const int i = find_segment(boff, n, b);
   const long long lo = eoff[i] + (long long)(b - boff[i]) * SEG_CHUNK;
   const long long hi = min(lo + SEG_CHUNK, eoff[i + 1]);
+  // each thread takes SEG_ITEMS adjacent elements (wide loads, all in flight together)
+  const long long k0 = lo + (long long)threadIdx.x * SEG_ITEMS;
   double acc = 0.0;
-  for (long long k = lo + threadIdx.x; k < hi; k += SEG_BLOCK) {
-    const double xv = double(x[k]);
-    acc += KIND == RED_NORM2 ? xv * xv : xv * double(y[k]);
+  if (k0 + SEG_ITEMS <= hi) {
+    T xv[SEG_ITEMS], yv[SEG_ITEMS];
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t) xv[t] = x[k0 + t];
+    if (KIND != RED_NORM2) {
+#pragma unroll
+      for (int t = 0; t < SEG_ITEMS; ++t) yv[t] = y[k0 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t)
+      acc += KIND == RED_NORM2 ? double(xv[t]) * double(xv[t]) : double(xv[t]) * double(yv[t]);
+  } else {
+    for (long long k = k0; k < hi; ++k) {
+      const double xv = double(x[k]);
+      acc += KIND == RED_NORM2 ? xv * xv : xv * double(y[k]);
+    }
   }
   const double s = block_sum<double, SEG_BLOCK>(acc, red);
   if (threadIdx.x == 0) partial[b] = s;
@@ -81,13 +96,33 @@ __global__ void __launch_bounds__(SEG_BLOCK) seg_dotdiff_kernel(const long long*
   const int i = find_segment(boff, n, b);
   const long long lo = eoff[i] + (long long)(b - boff[i]) * SEG_CHUNK;
   const long long hi = min(lo + SEG_CHUNK, eoff[i + 1]);
+  // each thread takes SEG_ITEMS adjacent elements (wide loads and stores, all loads in flight)
+  const long long k0 = lo + (long long)threadIdx.x * SEG_ITEMS;
   double s0 = 0.0, s1 = 0.0;
-  for (long long k = lo + threadIdx.x; k < hi; k += SEG_BLOCK) {
-    const Tv zv = Tv(zin[k]);
-    const double z = double(zv), rk = double(r[k]);
-    znew[k] = zv;
-    s0 += rk * z;
-    s1 += rk * (z - double(zold[k]));
+  if (k0 + SEG_ITEMS <= hi) {
+    Tv zv[SEG_ITEMS], rv[SEG_ITEMS], ov[SEG_ITEMS];
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t) zv[t] = Tv(zin[k0 + t]);
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t) rv[t] = r[k0 + t];
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t) ov[t] = zold[k0 + t];
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t) znew[k0 + t] = zv[t];
+#pragma unroll
+    for (int t = 0; t < SEG_ITEMS; ++t) {
+      const double z = double(zv[t]), rk = double(rv[t]);
+      s0 += rk * z;
+      s1 += rk * (z - double(ov[t]));
+    }
+  } else {
+    for (long long k = k0; k < hi; ++k) {
+      const Tv zk = Tv(zin[k]);
+      const double z = double(zk), rk = double(r[k]);
+      znew[k] = zk;
+      s0 += rk * z;
+      s1 += rk * (z - double(zold[k]));
+    }
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
@@ -165,8 +200,9 @@ struct DenseDesc {
   TV* y[DENSE_RHS];
 };
 
-// 4 rows per workgroup, one wave per row, as dense_mv_kernel (16-byte loads of A); each right-hand
-// side accumulates in the same order as a single-vector pass, so grouping does not change results.
+// 4 rows per workgroup, one wave per row, as dense_mv_kernel (16-byte loads of A, several in flight);
+// each right-hand side accumulates in the same order as a single-vector pass (chunk after chunk in
+// column order, then the wave sum), so grouping does not change results.
 template <typename TA, typename TV>
 __global__ void __launch_bounds__(256) dense_batch_kernel(const DenseDesc<TA, TV>* __restrict__ d,
                                                           const int* __restrict__ offs, int nb) {
@@ -183,7 +219,28 @@ __global__ void __launch_bounds__(256) dense_batch_kernel(const DenseDesc<TA, TV
   TA acc[DENSE_RHS];
 #pragma unroll
   for (int c = 0; c < DENSE_RHS; ++c) acc[c] = TA(0);
-  for (int j = lane * V; j < nfull; j += 64 * V) {
+  // the loads of UN chunks of the row in flight before they are accumulated (each right-hand
+  // side's sum keeps the column order of one chunk after the other)
+  constexpr int UN = 4;
+  int j = lane * V;
+  for (; j + (UN - 1) * 64 * V < nfull; j += UN * 64 * V) {
+    Vec x[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) x[u] = *reinterpret_cast<const Vec*>(a + j + u * 64 * V);
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const TA* e = reinterpret_cast<const TA*>(&x[u]);
+#pragma unroll
+      for (int c = 0; c < DENSE_RHS; ++c) {
+        if (c < k) {
+          const TV* r = q.r[c] + j + u * 64 * V;
+#pragma unroll
+          for (int t = 0; t < V; ++t) acc[c] += e[t] * TA(r[t]);
+        }
+      }
+    }
+  }
+  for (; j < nfull; j += 64 * V) {
     const Vec x = *reinterpret_cast<const Vec*>(a + j);
     const TA* e = reinterpret_cast<const TA*>(&x);
 #pragma unroll
