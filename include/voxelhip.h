@@ -266,6 +266,13 @@ int vh_cone_filter(int dtype, const void* x, const void* pre_div, const void* po
  * experiments/session_h/run_w1b.py:82-94 (with w = H(1/s)/n the filtered volume). */
 int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w, double lmid,
                 double move, void* xnew, double* partial, int nparts, void* stream);
+/* count (1 .. 16) OC bisection probes in one pass: partial[k * nparts + b] is block b's FP64 sum of
+ * xnew(lmids[k]) * w, bitwise the partials vh_oc_probe returns for that multiplier (xnew is not
+ * written).  Replaces: consecutive probes of the bisection of oc_top88_gpu
+ * experiments/session_h/run_w1b.py:82-94 (its midpoints are known in advance along a branch of
+ * the bisection, so several of them are evaluated in one read of the design). */
+int vh_oc_probe_multi(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w,
+                      const double* lmids, int count, double move, double* partial, int nparts, void* stream);
 
 /* Heaviside projection of the filtered density rho_bar (FP64, n elements): rho = (tanh(beta eta)
  * + tanh(beta (rho_bar - eta))) / (tanh(beta eta) + tanh(beta (1 - eta))) (identity for beta <=

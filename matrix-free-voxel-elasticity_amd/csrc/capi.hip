@@ -683,6 +683,28 @@ int vh_oc_probe(int dtype, size_t n, const void* x, const void* dc, const void* 
                                 static_cast<float*>(xnew), partial, nparts, st);
 }
 
+int vh_oc_probe_multi(int dtype, size_t n, const void* x, const void* dc, const void* dv, const void* w,
+                      const double* lmids, int count, double move, double* partial, int nparts, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && x && dc && partial && lmids && n >= 1 && nparts >= 1,
+             "vh_oc_probe_multi: bad arguments");
+  VH_REQUIRE(count >= 1 && count <= OC_MAX_PROBES, "vh_oc_probe_multi: %d multipliers (1 .. %d)", count,
+             OC_MAX_PROBES);
+  OcLambdas lm{};
+  lm.count = count;
+  for (int k = 0; k < count; ++k) {
+    VH_REQUIRE(lmids[k] > 0, "vh_oc_probe_multi: multiplier %d must be positive", k);
+    lm.v[k] = lmids[k];
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64)
+    return oc_probe_multi_launch<double>(n, static_cast<const double*>(x), static_cast<const double*>(dc),
+                                         static_cast<const double*>(dv), static_cast<const double*>(w), lm, move,
+                                         partial, nparts, st);
+  return oc_probe_multi_launch<float>(n, static_cast<const float*>(x), static_cast<const float*>(dc),
+                                      static_cast<const float*>(dv), static_cast<const float*>(w), lm, float(move),
+                                      partial, nparts, st);
+}
+
 int vh_heaviside(size_t n, const double* rho_bar, double beta, double eta, double penal, double E0, double Emin,
                  double* rho, double* drho, double* E, double* partial, int nparts, void* stream) {
   VH_REQUIRE(rho_bar && n >= 1 && (rho || drho || E || partial), "vh_heaviside: bad arguments");

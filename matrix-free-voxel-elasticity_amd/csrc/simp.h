@@ -16,6 +16,15 @@ template <typename T> int cone_launch(const T* x, const T* pre_div, const T* pos
                                       hipStream_t st);
 template <typename T> int oc_probe_launch(size_t n, const T* x, const T* dc, const T* dv, const T* w, T lmid, T move,
                                           T* xnew, double* partial, int nparts, hipStream_t st);
+// up to OC_MAX_PROBES bisection multipliers evaluated in one pass (oc_probe_multi_kernel)
+constexpr int OC_MAX_PROBES = 16;
+struct OcLambdas {
+  int count;
+  double v[OC_MAX_PROBES];
+};
+template <typename T> int oc_probe_multi_launch(size_t n, const T* x, const T* dc, const T* dv, const T* w,
+                                                const OcLambdas& lm, T move, double* partial, int nparts,
+                                                hipStream_t st);
 
 int heaviside_launch(size_t n, const double* rb, double beta, double eta, double tb, double den, double penal,
                      double E0, double Emin, double* rho, double* drho, double* E, double* partial, int nparts,

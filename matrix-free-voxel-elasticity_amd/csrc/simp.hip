@@ -59,6 +59,48 @@ __global__ void __launch_bounds__(256) oc_probe_kernel(size_t n, const T* __rest
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Several bisection probes in one pass over the design: partial[k * nparts + block] is the block's
+// sum of xnew(lmid_k) * w, the per-element value and the in-block sum order exactly those of
+// oc_probe_kernel (a probe's partials are bitwise the single probe's); xnew is not written.
+template <typename T>
+__global__ void __launch_bounds__(256) oc_probe_multi_kernel(size_t n, const T* __restrict__ x,
+                                                            const T* __restrict__ dc, const T* __restrict__ dv,
+                                                            const T* __restrict__ w, OcLambdas lm, T move,
+                                                            double* __restrict__ partial) {
+  __shared__ double red[OC_MAX_PROBES][4];
+  double acc[OC_MAX_PROBES];
+#pragma unroll
+  for (int k = 0; k < OC_MAX_PROBES; ++k) acc[k] = 0.0;
+  for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const T xi = x[i];
+    const T q = -dc[i] / (dv ? dv[i] : T(1));
+    const T lo = xi - move > T(0) ? xi - move : T(0);
+    const T hi = xi + move < T(1) ? xi + move : T(1);
+    const double wi = double(w ? w[i] : T(1));
+#pragma unroll
+    for (int k = 0; k < OC_MAX_PROBES; ++k) {
+      if (k < lm.count) {
+        const T ratio = q / T(lm.v[k]);
+        T v = xi * sqrt(ratio > T(0) ? ratio : T(0));
+        v = v < lo ? lo : (v > hi ? hi : v);
+        acc[k] += double(v) * wi;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < OC_MAX_PROBES; ++k) {
+    if (k < lm.count) {
+      const double a = wave_sum(acc[k]);
+      if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = a;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < lm.count) {
+    const int k = threadIdx.x;
+    partial[size_t(k) * gridDim.x + blockIdx.x] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+  }
+}
+
 // Heaviside projection of the filtered density (simp_r2.py:388-404) with its derivative, the
 // SIMP modulus of the projected field and per-block sums of the projected field (its volume):
 //   rho = (tb + tanh(beta (rb - eta))) / den,  drho/drb = beta / (den cosh^2(beta (rb - eta))),
@@ -139,6 +181,17 @@ int oc_probe_launch(size_t n, const T* x, const T* dc, const T* dv, const T* w, 
   return hip_status(hipGetLastError(), "oc_probe_kernel");
 }
 
+template <typename T>
+int oc_probe_multi_launch(size_t n, const T* x, const T* dc, const T* dv, const T* w, const OcLambdas& lm, T move,
+                          double* partial, int nparts, hipStream_t st) {
+  hipLaunchKernelGGL(oc_probe_multi_kernel<T>, dim3(nparts), dim3(256), 0, st, n, x, dc, dv, w, lm, move, partial);
+  return hip_status(hipGetLastError(), "oc_probe_multi_kernel");
+}
+
+template int oc_probe_multi_launch<double>(size_t, const double*, const double*, const double*, const double*,
+                                           const OcLambdas&, double, double*, int, hipStream_t);
+template int oc_probe_multi_launch<float>(size_t, const float*, const float*, const float*, const float*,
+                                          const OcLambdas&, float, double*, int, hipStream_t);
 template int cone_launch<double>(const double*, const double*, const double*, double*, const ConeArgs<double>&, hipStream_t);
 template int cone_launch<float>(const float*, const float*, const float*, float*, const ConeArgs<float>&, hipStream_t);
 template int oc_probe_launch<double>(size_t, const double*, const double*, const double*, const double*, double, double,

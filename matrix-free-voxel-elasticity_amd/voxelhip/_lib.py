@@ -85,6 +85,7 @@ SIGNATURES = {
     "vh_convert": [_i, _i, _sz, _vp, _vp, _d, _vp, _i, _i, _vp],
     "vh_cone_filter": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp],
     "vh_oc_probe": [_i, _sz, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _i, _vp],
+    "vh_oc_probe_multi": [_i, _sz, _vp, _vp, _vp, _vp, _dp, _i, _d, _vp, _i, _vp],
     "vh_product_batch_create": [_i, _i, _i, _i, ctypes.POINTER(ProductItem), ctypes.POINTER(_vp),
                                 ctypes.POINTER(_i)],
     "vh_segments_create": [_i, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_vp)],

@@ -398,3 +398,16 @@ def oc_design(rho, dc, dv, lmid, move, rho_lb, rho_new):
 def oc_probe(x, dc, dv, w, lmid, move, xnew, partial, nparts):
     L.call("vh_oc_probe", dtype_code(x), int(x.numel()), _p(x), _p(dc), _p(dv), _p(w), float(lmid),
            float(move), _p(xnew), _p(partial), int(nparts), _stream())
+
+
+OC_MAX_PROBES = 16
+
+
+def oc_probe_multi(x, dc, dv, w, lmids, move, partial, nparts):
+    """Volume partials of len(lmids) <= OC_MAX_PROBES bisection probes in one pass:
+    partial[k * nparts + b] as oc_probe's partial[b] for lmids[k] (no xnew)."""
+    lm = (ctypes.c_double * len(lmids))(*[float(v) for v in lmids])
+    if partial.numel() < len(lmids) * nparts:
+        raise ValueError("partial holds fewer than len(lmids) * nparts values")
+    L.call("vh_oc_probe_multi", dtype_code(x), int(x.numel()), _p(x), _p(dc), _p(dv), _p(w), lm, len(lmids),
+           float(move), _p(partial), int(nparts), _stream())

@@ -85,23 +85,61 @@ class ConeFilter:
 
 
 def oc_top88(x, dc, dv, volfrac, move, filt=None, l2=1e9, tol=1e-3):
-    """Returns x_new of the last bisection probe, as top88 does."""
+    """Returns x_new of the last bisection probe, as top88 does.
+
+    The bisection's probes are the ones of the reference loop, in its order and with its
+    midpoints 0.5 (l2 + l1); they are evaluated several at a time: while l1 = 0 the next midpoints
+    are l2/2, l2/4, ... whatever the volumes, and afterwards the midpoints of the next levels of
+    the bisection tree (every branch) are known in advance.  One pass of vh_oc_probe_multi gives
+    the volumes of up to 16 of them; the host walks the reference loop on those volumes and stops
+    where it stops.  The last probe's x_new is then computed by one vh_oc_probe pass."""
     n = x.numel()
     nparts = min(1024, max(1, (n + 255) // 256))
-    partial = torch.zeros(nparts, dtype=torch.float64, device=x.device)
+    K = ops.OC_MAX_PROBES
+    partial = torch.zeros(K * nparts, dtype=torch.float64, device=x.device)
     w = filt.w_vol if filt is not None else None
-    xnew = torch.empty_like(x)
-    l1 = 0.0
-    probes = 0
-    while (l2 - l1) / (l1 + l2) > tol:
-        lmid = 0.5 * (l2 + l1)
-        ops.oc_probe(x, dc, dv, w, lmid, move, xnew, partial, nparts)
-        vol = float(partial.sum()) if w is not None else float(partial.sum()) / n
-        probes += 1
-        if vol > volfrac:
-            l1 = lmid
-        else:
+
+    def volumes(lams):
+        ops.oc_probe_multi(x, dc, dv, w, lams, move, partial, nparts)
+        v = partial[:len(lams) * nparts].view(len(lams), nparts).sum(dim=1).tolist()
+        return v if w is not None else [vk / n for vk in v]
+
+    l1, l2 = 0.0, float(l2)
+    lmid = None
+    # while l1 = 0 the probes halve l2 until the first volume above the target
+    while l1 == 0.0 and (l2 - l1) / (l1 + l2) > tol:
+        lams, lb = [], l2
+        for _ in range(K):
+            lb = 0.5 * (lb + l1)
+            lams.append(lb)
+        for lm_, vol in zip(lams, volumes(lams)):
+            lmid = lm_
+            if vol > volfrac:
+                l1 = lmid
+                break
             l2 = lmid
+    # then the tree of the next levels of midpoints (depth 4: 15 probes per pass)
+    depth = 4
+    while (l2 - l1) / (l1 + l2) > tol:
+        nodes = [(l1, l2)]                      # breadth-first: node j has children 2j+1, 2j+2
+        lams = []
+        for j in range(2 ** depth - 1):
+            a, b = nodes[j]
+            m = 0.5 * (b + a)
+            lams.append(m)
+            nodes.append((a, m))                # volume <= target: l2 = m
+            nodes.append((m, b))                # volume > target:  l1 = m
+        vols = volumes(lams)
+        j = 0
+        while j < len(lams) and (l2 - l1) / (l1 + l2) > tol:
+            lmid = lams[j]
+            if vols[j] > volfrac:
+                l1, j = lmid, 2 * j + 2
+            else:
+                l2, j = lmid, 2 * j + 1
+    xnew = torch.empty_like(x)
+    if lmid is not None:
+        ops.oc_probe(x, dc, dv, w, lmid, move, xnew, partial, nparts)
     return xnew
 
 

@@ -119,6 +119,62 @@ def test_oc_update_matches_the_oracle():
     assert rel(xd, xo) < 1e-12
 
 
+def _oc_top88_one_probe_at_a_time(x, dc, dv, volfrac, move, w, l2=1e9, tol=1e-3):
+    """The bisection of experiments/session_h/run_w1b.py:82-94 with one vh_oc_probe per midpoint
+    (the volume of a probe reduced as oc_top88 reduces it); returns (x_new, midpoints)."""
+    from voxelhip import ops
+    n = x.numel()
+    nparts = min(1024, max(1, (n + 255) // 256))
+    partial = torch.zeros(nparts, dtype=torch.float64, device=x.device)
+    xnew = torch.empty_like(x)
+    l1, l2, mids = 0.0, float(l2), []
+    while (l2 - l1) / (l1 + l2) > tol:
+        lmid = 0.5 * (l2 + l1)
+        ops.oc_probe(x, dc, dv, w, lmid, move, xnew, partial, nparts)
+        vol = partial.view(1, nparts).sum(dim=1).item()
+        vol = vol if w is not None else vol / n
+        mids.append(lmid)
+        if vol > volfrac:
+            l1 = lmid
+        else:
+            l2 = lmid
+    return xnew, mids
+
+
+@pytest.mark.parametrize("dtype,filtered,scale", [(torch.float64, True, 1e-9), (torch.float64, False, 10.0),
+                                                  (torch.float32, True, 1e-3)])
+def test_oc_bisection_evaluated_several_probes_at_a_time(dtype, filtered, scale):
+    """oc_top88 evaluates the reference loop's midpoints up to 16 per pass (vh_oc_probe_multi):
+    the multi-probe partials are bitwise the single probe's, and the design is bitwise the one of
+    the loop that probes one midpoint at a time (sensitivities scaled so that the multiplier lies
+    far below or near the initial bracket's top: up to 60 halvings before the first volume above
+    the target)."""
+    from voxelhip import ops
+    from voxelhip.simp import ConeFilter, oc_top88
+    shape = (40, 24, 20)
+    n = int(np.prod(shape))
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = (0.2 + 0.3 * torch.rand(n, generator=g, device="cuda", dtype=torch.float64)).to(dtype)
+    dc = (-scale * torch.rand(n, generator=g, device="cuda", dtype=torch.float64)).to(dtype)
+    f = ConeFilter(shape, 1.5, dtype=dtype) if filtered else None
+    dv = f.transpose(torch.ones(n, dtype=dtype, device="cuda")) if filtered else None
+    w = f.w_vol if filtered else None
+    nparts = min(1024, max(1, (n + 255) // 256))
+    lams = [10.0 ** (-k) for k in range(16)]
+    multi = torch.zeros(16 * nparts, dtype=torch.float64, device="cuda")
+    ops.oc_probe_multi(x, dc, dv, w, lams, 0.2, multi, nparts)
+    one = torch.zeros(nparts, dtype=torch.float64, device="cuda")
+    xn = torch.empty_like(x)
+    for k, lm in enumerate(lams):
+        ops.oc_probe(x, dc, dv, w, lm, 0.2, xn, one, nparts)
+        assert torch.equal(multi[k * nparts:(k + 1) * nparts], one), k
+    ref, mids = _oc_top88_one_probe_at_a_time(x, dc, dv, 0.3, 0.2, w)
+    print("case oc bisection", dtype, filtered, scale, len(mids), "probes", flush=True)
+    assert len(mids) > 20
+    got = oc_top88(x, dc, dv, 0.3, 0.2, filt=f)
+    assert torch.equal(got, ref)
+
+
 def _problem(shape, ke, mask, force, precision="fp64", tol=1e-12):
     """The SIMP state problem on the GMG engine, solved far below the FD step's truncation."""
     from voxelhip.simp import SimpProblem
