@@ -126,9 +126,10 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // stride of the HBM layout put two lanes of a wave on one LDS bank)
   __shared__ T su[2][3][PLANE];
   // corner contributions and column modulus sums are indexed by the writing thread (unit stride
-  // for the writes and for the owner's four reads), plus one zero pad slot past every writer
+  // for the writes and for the owner's reads), plus one zero pad slot past every writer; corner
+  // (1, 1) of a column is the node its own thread owns and stays in registers
   constexpr int SCN = BLOCK + EZ + 2;
-  __shared__ T sc[4][3][SCN];         // corner contributions of each element column
+  __shared__ T sc[3][3][SCN];         // corners (0,0), (0,1), (1,0) of each element column
   __shared__ T se[SCN];               // per element column: E(ex-1) + E(ex), for D^-1 on the fly
   __shared__ T sE[BLOCK];             // this thread's prefetched modulus, parked across the barrier
   __shared__ double red[BLOCK / 64];
@@ -161,7 +162,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // zero the contribution slots once: slots without a writer column must read as zero
   {
     T* flat = &sc[0][0][0];
-    for (int i = tid; i < 12 * SCN; i += BLOCK) flat[i] = T(0);
+    for (int i = tid; i < 9 * SCN; i += BLOCK) flat[i] = T(0);
     if (dinv_from_E)
       for (int i = tid; i < SCN; i += BLOCK) se[i] = T(0);
   }
@@ -306,10 +307,11 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 #pragma unroll
   for (int c = 0; c < 3; ++c) ep_b[c] = ep_p[c] = ep_di[c] = ep_u[c] = ep_z[c] = T(0);
   const int jn_own = (ly + 1) * NZ + (lz + 1);
-  // the element columns (ly+1, lz+1), (ly+1, lz), (ly, lz+1), (ly, lz) around the owned node, as
-  // thread numbers; past the last column of a row (lz + 1 == EZ) there is none: the zero pad
+  // the element columns (ly+1, lz+1), (ly+1, lz), (ly, lz+1) around the owned node besides this
+  // thread's own (ly, lz), as thread numbers; past the last column of a row (lz + 1 == EZ) there is
+  // none: the zero pad
   const bool zlast = lz + 1 >= EZ;
-  const int w0 = zlast ? SCN - 1 : tid + EZ + 1, w1 = tid + EZ, w2 = zlast ? SCN - 1 : tid + 1, w3 = tid;
+  const int w0 = zlast ? SCN - 1 : tid + EZ + 1, w1 = tid + EZ, w2 = zlast ? SCN - 1 : tid + 1;
   // (each operand's three components are read by adjacent loads of one basic block, so they merge
   //  into one wide load per node; the previous Chebyshev iterate is read unconditionally and
   //  discarded when the step does not use it)
@@ -396,6 +398,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     }
 
     const bool finalize = ex >= ox0;     // node plane ex complete: carried right half + left half
+    T own[3];                            // corner (1, 1): this thread's owned node (ly+1, lz+1)
     if (finalize && col_in_tile) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -405,7 +408,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         sc[0][c][tid] = A0 + B0;                       // corner (b=0, c=0): node (ly, lz)
         sc[1][c][tid] = A0 - B0;                       // corner (0, 1): node (ly, lz+1)
         sc[2][c][tid] = A1 + B1;                       // corner (1, 0): node (ly+1, lz)
-        sc[3][c][tid] = A1 - B1;                       // corner (1, 1): node (ly+1, lz+1)
+        own[c] = A1 - B1;                              // corner (1, 1): node (ly+1, lz+1)
       }
       if (dinv_from_E) se[tid] = Eprev + Ee;
     }
@@ -424,11 +427,11 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
         part[c][0] = sc[0][c][w0];
         part[c][1] = sc[1][c][w1];
         part[c][2] = sc[2][c][w2];
-        part[c][3] = sc[3][c][w3];
+        part[c][3] = own[c];
       }
       T rsum = T(0);     // 1 / (sum of the moduli of the node's incident elements)
       if (dinv_from_E) { // incident columns (ly|ly+1, lz|lz+1) of the owned node (ly+1, lz+1)
-        const T esum = ((se[w3] + se[w1]) + se[w2]) + se[w0];
+        const T esum = ((Eprev + Ee + se[w1]) + se[w2]) + se[w0];
         rsum = reciprocal(esum);
       }
       T outv[3], dv[3] = {T(0), T(0), T(0)};
