@@ -203,15 +203,17 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   // ix-1: out-of-range slots load a clamped in-bounds address, and the zeroing of those slots
   // is deferred to commit(), so no instruction consumes a loaded value before the next step
   // (the batch stays in flight through this step's compute).
-  T pre[LSLOTS][3];
   // (an FP32 operand w stays FP32 in its prefetch registers until the commit converts it)
   using WT = std::conditional_t<LOADOP == LOAD_AXPY32, float, T>;
-  WT pre_w[COMBINE ? LSLOTS : 1][3];
-  uint8_t pre_m[LSLOTS][3];
-  unsigned pre_keep = 0;
-  T pre_E;
-  bool pre_E_ok = false;
-  auto prefetch = [&](int ix) {
+  struct Staged {
+    T v[LSLOTS][3];
+    WT w[COMBINE ? LSLOTS : 1][3];
+    uint8_t m[LSLOTS][3];
+    unsigned keep;
+    T E;
+    bool E_ok;
+  };
+  auto prefetch = [&](Staged& st, int ix) {
     // (the x = 0 clamp is applied to whole faces in face(), so the staged plane keeps the
     //  unmasked input the epilogue needs at fixed DOFs)
     const bool plane_ok = ix >= 0 && ix <= nx;
@@ -223,26 +225,26 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
       if (j < LSLOTS - 1 || tid + j * BLOCK < NNODE) {
         const size_t dof = base + ld_off[j];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pre[j][c] = load_as<S, T>(u, dof + c);
+        for (int c = 0; c < 3; ++c) st.v[j][c] = load_as<S, T>(u, dof + c);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          if (COMBINE) pre_w[j][c] = LOADOP == LOAD_AXPY32 ? WT(wf[dof + c]) : WT(wv[dof + c]);
-          if (MASK == MASK_ARRAY) pre_m[j][c] = a.mask[dof + c];
+          if (COMBINE) st.w[j][c] = LOADOP == LOAD_AXPY32 ? WT(wf[dof + c]) : WT(wv[dof + c]);
+          if (MASK == MASK_ARRAY) st.m[j][c] = a.mask[dof + c];
         }
       }
       keep |= unsigned(plane_ok && ((ld_okbits >> j) & 1u)) << j;
     }
-    pre_keep = keep;
+    st.keep = keep;
     const int ex = ix - 1;
-    pre_E_ok = col_ok && ex >= 0 && ex < nx;
-    pre_E = E[(pre_E_ok ? size_t(ex) * estride : 0) + e_off];
+    st.E_ok = col_ok && ex >= 0 && ex < nx;
+    st.E = E[(st.E_ok ? size_t(ex) * estride : 0) + e_off];
   };
-  auto commit = [&](int ix) {
+  auto commit = [&](const Staged& st, int ix) {
     // the prefetched modulus goes through LDS like the plane: a store before the barrier cannot
     // be sunk below the next prefetch, so its register is free to receive the next load (kept in
     // a register, the compiler may keep it live past the next load and then copy it, waiting for
     // every load in flight)
-    sE[tid] = pre_E_ok ? pre_E : T(0);
+    sE[tid] = st.E_ok ? st.E : T(0);
     T (*pl)[PLANE] = su[ix & 1];
 #pragma unroll
     for (int j = 0; j < LSLOTS; ++j) {
@@ -250,10 +252,10 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
       if (j < LSLOTS - 1 || k < NNODE) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          bool keep = (pre_keep >> j) & 1u;
-          if (MASK == MASK_ARRAY) keep = keep && pre_m[j][c] != 0;
-          const T v = AXPY ? pre[j][c] + sw * T(pre_w[j][c])
-                      : (LOADOP == LOAD_MUL ? sw * (pre[j][c] * T(pre_w[j][c])) : pre[j][c]);
+          bool keep = (st.keep >> j) & 1u;
+          if (MASK == MASK_ARRAY) keep = keep && st.m[j][c] != 0;
+          const T v = AXPY ? st.v[j][c] + sw * T(st.w[j][c])
+                      : (LOADOP == LOAD_MUL ? sw * (st.v[j][c] * T(st.w[j][c])) : st.v[j][c]);
           pl[c][k] = keep ? v : T(0);
         }
       }
@@ -353,26 +355,31 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
     }
   };
 
-  prefetch(ox0 - 1);
-  __syncthreads();                 // sc zeroed
-  commit(ox0 - 1);
-  __syncthreads();
-  prefetch(ox0);                   // plane ox0 and the modulus of element ox0 - 1
+  // Prologue: planes ox0-1 and ox0 (and the modulus of element ox0 - 1) loaded together, so the
+  // launch waits for one memory latency before its first step, not two.  The first commit of the
+  // loop writes the other plane buffer, and the corner slots are first written two barriers on.
+  Staged st;
+  {
+    Staged st0;
+    prefetch(st0, ox0 - 1);
+    prefetch(st, ox0);
+    commit(st0, ox0 - 1);
+  }
+  __syncthreads();                 // plane ox0 - 1 staged
   face(Fl, ox0 - 1);
-  __syncthreads();                 // face() reads of plane ox0-1 before the first commit
 
   // Per step: commit plane ex+1 | barrier | prefetch ex+2 | face + element ex | contributions
   // | barrier | gather.  The second barrier also orders this step's face() reads before the
   // next step's commit, and the gather before the next step's contribution writes.
   for (int ex = ox0 - 1; ex < ox1; ++ex) {
-    commit(ex + 1);
+    commit(st, ex + 1);
     __syncthreads();
     const T Ee = sE[tid];
     // this step's epilogue operands first: loads complete in issue order, so the gather's wait
     // for them then leaves the prefetch of plane ex + 2 in flight
     fetch_epilogue(ex);
     __builtin_amdgcn_sched_barrier(0);
-    prefetch(ex + 2);
+    prefetch(st, ex + 2);
     if (RECOMPUTE_FL) face(Fl, ex);
     face(Fr, ex + 1);
 
