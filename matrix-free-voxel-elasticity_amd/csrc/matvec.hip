@@ -134,9 +134,10 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   __shared__ T sE[BLOCK];             // this thread's prefetched modulus, parked across the barrier
   __shared__ double red[BLOCK / 64];
   constexpr bool NEEDS_DINV = MODE == MODE_JACOBI || MODE == MODE_CHEB || MODE == MODE_CHEB3 || MODE == MODE_CHEB3D;
-  // FP64 smoothing passes recompute the left face from the still-staged plane instead of
-  // carrying it in 24 registers, which keeps them within the 168 VGPRs of 3 waves per SIMD
-  constexpr bool RECOMPUTE_FL = sizeof(T) == 8 && NEEDS_DINV;
+  // FP64 smoothing and residual passes, and FP64 passes that combine two vectors at load time,
+  // recompute the left face from the still-staged plane instead of carrying it in 24 registers,
+  // which keeps them within the 168 VGPRs of 3 waves per SIMD
+  constexpr bool RECOMPUTE_FL = sizeof(T) == 8 && (NEEDS_DINV || MODE == MODE_RESID || LOADOP != LOAD_NONE);
   const bool dinv_from_E = NEEDS_DINV && a.dinv == nullptr;
 
   const int tz = logical % a.tiles_z;
