@@ -251,7 +251,8 @@ def _cheb_params(lam, lower_frac):
 class Multigrid:
     """V-cycle preconditioner (src/voxel_mg.py:133-178).  `products` is a list of per-level
     product objects (finest first) exposing raw(u, E) and diagonal(E); `coarse_factor` is the
-    lower Cholesky factor of the coarsest constrained operator (host or device)."""
+    lower Cholesky factor of the coarsest constrained operator (host or device), or None when the
+    caller sets the coarsest inverse later (set_coarse_inverse, before the first V-cycle)."""
 
     def __init__(self, xp, shapes, products, E_fine, mask_fine, coarse_factor, dtype,
                  pre=4, post=4, omegas=None, smoother="jacobi", lambdas=None,
@@ -295,9 +296,10 @@ class Multigrid:
         self.coarse_dtype = self.dtype if coarse_dtype is None else _tdt(coarse_dtype)
         if self.coarse_dtype not in (self.dtype, torch.float64):
             raise ValueError("coarse_dtype must be the level precision or float64")
-        Lf = coarse_factor if isinstance(coarse_factor, torch.Tensor) else torch.from_numpy(np.asarray(coarse_factor))
-        Lf = Lf.to(dev, torch.float64)
-        self.coarse_inverse = ops.padded_matrix(torch.cholesky_inverse(Lf), self.coarse_dtype)
+        self.coarse_inverse = None
+        if coarse_factor is not None:
+            Lf = coarse_factor if isinstance(coarse_factor, torch.Tensor) else torch.from_numpy(np.asarray(coarse_factor))
+            self.set_coarse_inverse(torch.cholesky_inverse(Lf.to(dev, torch.float64)))
         cl = self.levels[-1]
         self._coarse_mask = ops.Mask(L.MASK_ARRAY, (cl.mask != 0).to(torch.uint8).contiguous())
         # work vectors per level: x, x2 (ping-pong), res, d (Chebyshev), rhs (levels > 0)
@@ -309,6 +311,10 @@ class Multigrid:
             if not (self.levels[0].fused and not self.levels[0].stencil):
                 raise ValueError("bf16 storage needs the parity product on the finest level")
             self._xb = torch.zeros(self.levels[0].n, dtype=torch.bfloat16, device=dev)
+
+    def set_coarse_inverse(self, inverse64):
+        """The coarsest level's explicit inverse (FP64 device matrix), stored for dense_mv."""
+        self.coarse_inverse = ops.padded_matrix(inverse64, self.coarse_dtype)
 
     def tune(self, tuner):
         """Launch plans of every smoothing level's passes (smoother and residual)."""
