@@ -95,7 +95,8 @@ typedef struct vh_epilogue {
   /* Load-time combination (voxel-cube blocks), the combined input written to u_out at every
    * DOF:  DOT / RESID read u + s w with s = sdev[sidx] (device FP64 scalar);  CHEB3 reads
    * load_scale * (u (.) w) (elementwise product: with u = b, w = dinv and load_scale = 1/theta
-   * this is the first Chebyshev step from a zero iterate, so the pass applies steps 1 and 2).
+   * this is the first Chebyshev step from a zero iterate, so the pass applies steps 1 and 2; the
+   * previous iterate is zero: VH_PREV_ZERO or cheb_c = 0 is required, and it is not read).
    * RESID also writes a float32 copy of y to y32 and, with partial != NULL, per-tile sums of y^2. */
   const void* w;
   const double* sdev;

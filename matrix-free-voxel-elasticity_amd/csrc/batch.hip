@@ -443,6 +443,8 @@ int vh_product_batch_create(int dtype, int mask_kind, int tile, int n, const vh_
     VH_REQUIRE(!combine || ((it.ep.sdev || mode == MODE_CHEB3) && it.ep.u_out && it.ep.u_out != it.u &&
                             it.ep.u_out != it.y && it.ep.w != it.y && it.ep.u_out != it.ep.w && it.ep.u_out != it.ep.b),
                "vh_product_batch_create: item %d combination operands", i);
+    VH_REQUIRE(!combine || mode != MODE_CHEB3 || (it.ep.flags & MV_PREV_ZERO) || it.ep.cheb_c == 0.0,
+               "vh_product_batch_create: item %d: a CHEB3 pass with the load-time product starts from zero", i);
     VH_REQUIRE(!it.ep.y32 || mode == MODE_RESID, "vh_product_batch_create: item %d y32", i);
     if ((mode == MODE_JACOBI || mode == MODE_CHEB3) && !it.ep.dinv)
       VH_REQUIRE(it.ep.kdiag[0] > 0 && it.ep.kdiag[1] > 0 && it.ep.kdiag[2] > 0,

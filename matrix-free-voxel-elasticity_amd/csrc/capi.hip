@@ -339,6 +339,9 @@ int vh_matvec_fused(int dtype, int storage, const double* blocks72, const void* 
     VH_REQUIRE(!ep->w || (ep->u_out && ep->u_out != u && ep->u_out != y && ep->w != y && ep->u_out != ep->w &&
                           ep->u_out != ep->b),
                "vh_matvec: the load-time combination needs a distinct u_out");
+    VH_REQUIRE(!ep->w || m != MODE_CHEB3 || (ep->flags & MV_PREV_ZERO) || ep->cheb_c == 0.0,
+               "vh_matvec: a CHEB3 pass with the load-time product is the first step from a zero iterate "
+               "(VH_PREV_ZERO or cheb_c = 0)");
     VH_REQUIRE(!ep->y32 || m == MODE_RESID, "vh_matvec: y32 is an output of the RESID pass");
     VH_REQUIRE((ep->flags & ~(MV_PREV_ZERO | MV_W_F32)) == 0, "vh_matvec: unknown flags %d", ep->flags);
     VH_REQUIRE(!(ep->flags & MV_W_F32) || (dtype == F64 && storage == F64 && m == MODE_RESID && ep->w),

@@ -317,7 +317,8 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
   const int w0 = zlast ? SCN - 1 : tid + EZ + 1, w1 = tid + EZ, w2 = zlast ? SCN - 1 : tid + 1;
   // (each operand's three components are read by adjacent loads of one basic block, so they merge
   //  into one wide load per node; the previous Chebyshev iterate is read unconditionally and
-  //  discarded when the step does not use it)
+  //  discarded when the step does not use it, except by the first step from a zero iterate (the
+  //  MUL load), which never uses it)
   const T* __restrict__ prev_src = (MODE == MODE_CHEB3 || MODE == MODE_CHEB3D) ? (a.prev ? a.prev : a.y) : nullptr;
   const bool prev_used = a.cheb_c != T(0) && !(a.flags & MV_PREV_ZERO);
   auto fetch_epilogue = [&](int ix) {
@@ -340,7 +341,7 @@ __device__ __forceinline__ void xmarch_tile(const MatvecParams<T>& a, const int 
 #pragma unroll
       for (int c = 0; c < 3; ++c) ep_p[c] = a.d[3 * node + c];
     }
-    if (MODE == MODE_CHEB3 || MODE == MODE_CHEB3D) {
+    if ((MODE == MODE_CHEB3 || MODE == MODE_CHEB3D) && LOADOP != LOAD_MUL) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) ep_p[c] = prev_src[3 * node + c];
 #pragma unroll

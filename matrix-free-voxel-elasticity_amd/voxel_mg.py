@@ -321,6 +321,8 @@ class Multigrid:
         sm = L.MODE_CHEB3 if self.smoother == "chebyshev" else L.MODE_JACOBI
         for lv in self.levels[:-1]:
             passes = [(sm, False), (L.MODE_RESID, False)]
+            if sm == L.MODE_CHEB3:
+                passes.append((L.MODE_JACOBI, False))    # the first post-smoothing step (c = 0)
             if sm == L.MODE_CHEB3 and self.cheb_variant == "saad":
                 passes.append((sm, True))          # the first step, fused with x0 = D^-1 r / theta
             lv.tune(tuner, passes)
@@ -382,7 +384,9 @@ class Multigrid:
             elif out is not None and sweeps == 1:
                 return last_pass(x, other, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
             else:
-                self._pass(level, x, other, mode=L.MODE_CHEB3, b=r, cheb_a=1.0 / theta, cheb_c=0.0, **dk)
+                # (c = 0: the step is the damped Jacobi sweep with omega = 1 / theta, the same
+                #  arithmetic, and that pass does not read the previous iterate)
+                self._pass(level, x, other, mode=L.MODE_JACOBI, b=r, omega=1.0 / theta, **dk)
                 x, other = other, x
             for k in range(start, sweeps):
                 c = 0.25 * delta * delta * alpha

@@ -309,12 +309,13 @@ class BatchedGMGSolver:
                 it.cmask_kind, it.cmask = M.levels[d + 1].maskobj.kind, M.levels[d + 1].maskobj.ptr
                 r_items.append(it)
             restr = transfer_batch(pd, True, r_items)
-            # post: first step with c = 0 from the prolongated iterate, then the recurrence
+            # post: first step with c = 0 from the prolongated iterate (the damped Jacobi sweep with
+            # omega = 1 / theta, which does not read the previous iterate), then the recurrence
             post = []
             items = []
             for k in ks:
-                items.append(self._item(lv(k), _ptr(bufs[cur], o(k)), _ptr(bufs[oth], o(k)), L.MODE_CHEB3,
-                                        b=_ptr(self.RHS[d], o(k)), cheb_a=1.0 / th[k][0], cheb_c=0.0,
+                items.append(self._item(lv(k), _ptr(bufs[cur], o(k)), _ptr(bufs[oth], o(k)), L.MODE_JACOBI,
+                                        b=_ptr(self.RHS[d], o(k)), omega=1.0 / th[k][0],
                                         kdiag=lv(k).product.kdiag))
             post.append(product_batch(pd, mk, self.tile, items)[0])
             post_in = cur

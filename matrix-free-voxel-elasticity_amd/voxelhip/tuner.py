@@ -127,6 +127,8 @@ class LaunchTuner:
             kw.update(zold=vec(), prev=vec())
         if combine:
             kw.update(w=w, sdev=sdev, sidx=0, u_out=uo)
+            if mode == L.MODE_CHEB3:         # the first Chebyshev step starts from a zero iterate
+                kw["prev_zero"] = True
         res = {}
         for t, lx in cands:
             res[(t, lx)] = self._time(lambda: ops.matvec(level.product.blocks, level.E, u, y, level.shape,

@@ -1,5 +1,6 @@
 """CPU checks of the C-ABI library: it loads, exports every symbol include/voxelhip.h declares,
 and its host-only entry point (parity-block factorisation) agrees with the oracle."""
+import ctypes
 import re
 from pathlib import Path
 
@@ -100,6 +101,31 @@ def test_device_entry_points_fail_loudly_without_a_gpu():
         pytest.skip("GPU present")
     with pytest.raises(RuntimeError):
         vk.emulate("modal8", (2, 2, 2), vk.quadrature([1, 1, 1], 0.3), np.ones(8), np.zeros(81))
+
+
+def test_first_chebyshev_step_must_start_from_zero():
+    """The CHEB3 pass with the load-time product is the first step from a zero iterate (it does not
+    read the previous one): the entry points reject it unless VH_PREV_ZERO is set or cheb_c = 0.
+    (Argument checks only: the rejected calls return before touching a device.)"""
+    from voxelhip import _lib as L
+    lib = L.load()
+    fake = ctypes.c_void_p(4096)
+    blocks = (ctypes.c_double * 72)()
+    ep = L.Epilogue()
+    ep.mode = L.MODE_CHEB3
+    ep.b, ep.w, ep.u_out = 8192, 12288, 16384
+    ep.cheb_a, ep.cheb_c, ep.load_scale = 0.5, 0.3, 1.0
+    ep.kdiag[0] = ep.kdiag[1] = ep.kdiag[2] = 1.0
+    args = (L.F64, L.F64, ctypes.addressof(blocks), fake, fake, ctypes.c_void_p(20480), 4, 4, 4, L.MASK_X0, None, ctypes.byref(ep), None)
+    assert lib.vh_matvec_fused(*args) == -2
+    assert "first step from a zero iterate" in lib.vh_last_error().decode()
+    item = L.ProductItem()
+    item.blocks72, item.E, item.u, item.y = ctypes.addressof(blocks), 4096, 4096, 20480
+    item.nx = item.ny = item.nz = 4
+    item.ep = ep
+    h = ctypes.c_void_p()
+    assert lib.vh_product_batch_create(L.F64, L.MASK_X0, 0, 1, ctypes.byref(item), ctypes.byref(h), None) == -2
+    assert "starts from zero" in lib.vh_last_error().decode()
 
 
 def test_host_index_helpers_match_the_oracle():

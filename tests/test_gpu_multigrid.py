@@ -370,6 +370,31 @@ def test_galerkin_hierarchy_vcycle_matches_the_oracle(KE):
     assert rel(M(torch.from_numpy(r).cuda()), Mo(r)) < 1e-11
 
 
+def test_coarsest_factorisation_beside_the_bounds(KE):
+    """The coarsest Cholesky factorisation and inverse run on a second stream while the spectrum
+    estimates run on the current one: the hierarchy's coarsest inverse equals the host inverse of
+    the same operator, and an operator that is not positive definite still raises LinAlgError
+    (checked when the two streams join)."""
+    from voxelhip import ops
+    from voxelhip.engine import build_hierarchy
+    shapes = [(16, 8, 8), (8, 4, 4), (4, 2, 2)]
+    rho, E, mask, _ = sr.cantilever(shapes[0], seed=3, rmin=1.5)
+    E64 = torch.from_numpy(E).cuda()
+    mo = ops.Mask.from_values(mask, shapes[0])
+    n = 3 * ops.n_nodes(shapes[-1])
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)                       # symmetric positive definite
+    _, M = build_hierarchy(shapes[0], KE, mo, E64, shapes, torch.float64, smoother="chebyshev", degree=3,
+                           coarse_matrix=torch.from_numpy(A).cuda())
+    inv = M.coarse_inverse[:, :n].double().cpu().numpy()
+    assert np.abs(inv - np.linalg.inv(A)).max() <= 1e-12 * np.abs(np.linalg.inv(A)).max()
+    A[5, 5] = -1.0                                    # leading minor 6 negative
+    with pytest.raises(torch.linalg.LinAlgError):
+        build_hierarchy(shapes[0], KE, mo, E64, shapes, torch.float64, smoother="chebyshev", degree=3,
+                        coarse_matrix=torch.from_numpy(A).cuda())
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_full_galerkin_solve(KE, precision):
     from voxelhip.engine import GMGSolver
