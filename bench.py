@@ -150,7 +150,7 @@ def simp_leg(ke, shape=(384, 192, 192), steps=100, seed=5):
 
 
 # the reference's own study solves (inputs regenerated from its base designs in tests/golden),
-# with its published times on other GPUs
+# with its published times on other GPUs (the paper tables cited per case)
 REFERENCE_WORKLOADS = {
     "q64x1": {"design": "optimized-q64 (session D)", "omega_seed": 2026092320, "iterations": 13,
               "published": {"fp64_s": 0.1037, "mixed_s": 0.0626, "gpu": "RTX 4090",
