@@ -257,6 +257,11 @@ int vh_axpby(int dtype, size_t n, double a, const void* x, const void* w, double
  * cast of the fine smoother's input in donor_yang_gmg multigrid_v4.py:851-856. */
 int vh_convert(int dtype_in, int dtype_out, size_t n, const void* x, void* y, double scale, const double* dscale,
                int inv_scale, int accumulate, void* stream);
+/* y[i] = 2 u_i - 1 in [-1, 1) with u_i = (s_i >> 11) 2^-53, s_i the (i+1)-th output of SplitMix64 seeded
+ * with `seed` (Steele, Lea, Flood, OOPSLA 2014).  No counterpart in the reference: the deterministic
+ * start vector of the Chebyshev spectrum estimate, drawn on the device (the reference's seeded host
+ * normals cost about 0.8 host seconds per 100 M values), reproducible bit for bit by integer arithmetic. */
+int vh_fill_splitmix(int dtype, size_t n, unsigned long long seed, void* y, void* stream);
 
 /* Cone filter y = post_div^-1 H (x / pre_div) with radius rmin (pre_div/post_div optional).
  * Replaces: cone_stencil_fp64 filter_r2.py:40-76, ConeFilter.forward/transpose :171-178. */

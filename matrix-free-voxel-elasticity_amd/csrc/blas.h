@@ -67,6 +67,7 @@ template <typename T> int cg_update_launch(size_t n, T* x, const T* p, T* r, con
 template <typename T> int jacobi_dot_launch(size_t n, const T* r, const T* dinv, T* z, double* partial, hipStream_t st);
 template <typename T> int xpby_launch(size_t n, const T* z, T* p, const double* s, T* zcopy, hipStream_t st);
 template <typename T> int axpby_launch(size_t n, T a, const T* x, const T* w, T b, T* y, hipStream_t st);
+template <typename T> int fill_splitmix_launch(size_t n, unsigned long long seed, T* y, hipStream_t st);
 template <typename Ti, typename To> int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dscale,
                                                        int inv_scale, int accumulate, hipStream_t st);
 int convert_bf16_launch(size_t n, const float* x, uint16_t* y, double scale, const double* dscale, int inv_scale,

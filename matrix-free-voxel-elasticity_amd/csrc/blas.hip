@@ -153,6 +153,25 @@ __global__ void __launch_bounds__(BLAS_BLOCK) convert_kernel(size_t n, const Ti*
   }
 }
 
+// y[i] = 2 u_i - 1 in [-1, 1), u_i = (s_i >> 11) * 2^-53 with s_i the (i+1)-th output of the
+// SplitMix64 generator seeded with `seed` (Steele, Lea and Flood, "Fast splittable pseudorandom
+// number generators", OOPSLA 2014): s_i = mix(seed + (i + 1) * 0x9E3779B97F4A7C15).  Integer
+// arithmetic throughout, so any restatement reproduces every value bit for bit.
+__device__ __forceinline__ unsigned long long splitmix64_at(unsigned long long seed, unsigned long long i) {
+  unsigned long long z = seed + (i + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK) fill_splitmix_kernel(size_t n, unsigned long long seed, T* __restrict__ y) {
+  for (size_t i = blockIdx.x * size_t(BLAS_BLOCK) + threadIdx.x; i < n; i += size_t(gridDim.x) * BLAS_BLOCK) {
+    const double u = double(splitmix64_at(seed, i) >> 11) * 0x1.0p-53;
+    y[i] = T(2.0 * u - 1.0);
+  }
+}
+
 // y = bfloat16(x * scale), rounded to nearest even: BF16 storage of an FP32 vector.
 __global__ void __launch_bounds__(BLAS_BLOCK) convert_bf16_kernel(size_t n, const float* __restrict__ x,
                                                                  uint16_t* __restrict__ y, double scale,
@@ -319,6 +338,14 @@ int axpby_launch(size_t n, T a, const T* x, const T* w, T b, T* y, hipStream_t s
   hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, a, x, w, b, y);
   return hip_status(hipGetLastError(), "axpby_kernel");
 }
+
+template <typename T>
+int fill_splitmix_launch(size_t n, unsigned long long seed, T* y, hipStream_t st) {
+  hipLaunchKernelGGL(fill_splitmix_kernel<T>, dim3(grid_for(n)), dim3(BLAS_BLOCK), 0, st, n, seed, y);
+  return hip_status(hipGetLastError(), "fill_splitmix_kernel");
+}
+template int fill_splitmix_launch<double>(size_t, unsigned long long, double*, hipStream_t);
+template int fill_splitmix_launch<float>(size_t, unsigned long long, float*, hipStream_t);
 
 template <typename Ti, typename To>
 int convert_launch(size_t n, const Ti* x, To* y, double scale, const double* dscale, int inv_scale, int accumulate,

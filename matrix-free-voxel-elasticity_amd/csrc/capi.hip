@@ -641,6 +641,13 @@ int vh_convert(int dtype_in, int dtype_out, size_t n, const void* x, void* y, do
   return convert_launch<float, float>(n, static_cast<const float*>(x), static_cast<float*>(y), scale, dscale, inv_scale, accumulate, st);
 }
 
+int vh_fill_splitmix(int dtype, size_t n, unsigned long long seed, void* y, void* stream) {
+  VH_REQUIRE(dtype_ok(dtype) && y && n >= 1, "vh_fill_splitmix: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == F64) return fill_splitmix_launch<double>(n, seed, static_cast<double*>(y), st);
+  return fill_splitmix_launch<float>(n, seed, static_cast<float*>(y), st);
+}
+
 int vh_cone_filter(int dtype, const void* x, const void* pre_div, const void* post_div, void* y, int nx, int ny, int nz,
                    double rmin, void* stream) {
   VH_REQUIRE(dtype_ok(dtype) && x && y && x != y && dims_ok(nx, ny, nz), "vh_cone_filter: bad arguments");

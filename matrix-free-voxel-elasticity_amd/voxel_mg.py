@@ -545,6 +545,20 @@ def _start_vector(n, seed, dtype):
     return v
 
 
+_BOUND_STARTS = {}
+
+
+def _bound_start(n, seed, dtype):
+    """The start vector of the Chebyshev bound estimate: uniform on [-1, 1) from SplitMix64
+    (ops.fill_splitmix), drawn on the device and kept per (n, seed, dtype)."""
+    key = (int(n), int(seed), dtype)
+    v = _BOUND_STARTS.get(key)
+    if v is None:
+        v = ops.fill_splitmix(torch.empty(int(n), dtype=torch.float64, device=voxelhip.device()), seed).to(dtype)
+        _BOUND_STARTS[key] = v
+    return v
+
+
 def spectral_omegas(xp, levels, iterations=30, seed=20260923):
     """omega_l = 4 / (3 lambda_l) for every non-coarsest level, lambda_l from a seeded power
     iteration on D^-1 A restricted to free DOFs (src/voxel_mg.py:228-244).  Returns (omegas, lambdas)."""
@@ -571,8 +585,9 @@ def spectral_omegas(xp, levels, iterations=30, seed=20260923):
 def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923, start=None, keep=None):
     """Upper bounds of the spectrum of D^-1 A per smoothing level for the Chebyshev smoother:
     safety x the D-weighted Rayleigh quotient v.Av / v.Dv after `iterations` power steps
-    v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from the seeded
-    random vector of src/voxel_mg.py:234-235 over the free DOFs.  (spectral_omegas' norm ratio
+    v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from a seeded
+    uniform vector over the free DOFs (SplitMix64 on the device, _bound_start; the donor's all-ones
+    start is a smooth vector, far from the top of the spectrum).  (spectral_omegas' norm ratio
     approaches lambda_max from below slowly on smooth moduli - 30 steps give 2.81 for 3.09 on a
     uniform 256 x 128 x 128 grid - and a Chebyshev interval that ends below the spectrum
     amplifies the top modes; the Rayleigh quotient converges twice as fast and the 10 % margin
@@ -582,7 +597,7 @@ def chebyshev_bounds(xp, levels, iterations=30, safety=1.1, seed=20260923, start
     lams = []
     for li, Lv in enumerate(levels[:-1]):
         d = Lv.product.diagonal(Lv.E, mask=Lv.maskobj)               # D, 1 at fixed DOFs
-        w = (start[li] if start is not None else _start_vector(Lv.n, seed, Lv.dtype)).clone()
+        w = (start[li] if start is not None else _bound_start(Lv.n, seed, Lv.dtype)).clone()
         w = ops.axpby(1.0, w, 0.0, w, w=Lv.mask)
         red = _Reducer(Lv.n, w.device)
         v, t = torch.empty_like(w), torch.empty_like(w)

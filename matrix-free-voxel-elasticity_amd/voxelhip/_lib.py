@@ -83,6 +83,7 @@ SIGNATURES = {
     "vh_xpby": [_i, _sz, _vp, _vp, _vp, _vp, _vp],
     "vh_axpby": [_i, _sz, _d, _vp, _vp, _d, _vp, _vp],
     "vh_convert": [_i, _i, _sz, _vp, _vp, _d, _vp, _i, _i, _vp],
+    "vh_fill_splitmix": [_i, _sz, ctypes.c_ulonglong, _vp, _vp],
     "vh_cone_filter": [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp],
     "vh_oc_probe": [_i, _sz, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _i, _vp],
     "vh_oc_probe_multi": [_i, _sz, _vp, _vp, _vp, _vp, _dp, _i, _d, _vp, _i, _vp],

@@ -357,6 +357,12 @@ def axpby(a, x, b, y, w=None):
     return y
 
 
+def fill_splitmix(y, seed):
+    """y[i] = 2 u_i - 1, u_i from the (i+1)-th SplitMix64 output of `seed` (vh_fill_splitmix)."""
+    L.call("vh_fill_splitmix", _DT[y.dtype], y.numel(), int(seed) & (2 ** 64 - 1), y.data_ptr(), _stream())
+    return y
+
+
 def convert(x, y, scale=1.0, dscale=None, inv_scale=False, accumulate=False):
     _dev(x, "x"); _dev(y, "y")
     L.call("vh_convert", dtype_code(x), dtype_code(y), int(x.numel()), _p(x), _p(y), float(scale),

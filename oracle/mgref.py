@@ -306,17 +306,30 @@ def spectral_omegas(levels, iterations=30, seed=20260923):
     return omegas, lambdas
 
 
+def splitmix_uniform(n, seed):
+    """2 u_i - 1 on [-1, 1), u_i = (s_i >> 11) 2^-53 with s_i the (i+1)-th output of the SplitMix64
+    generator seeded with `seed` (Steele, Lea, Flood, "Fast splittable pseudorandom number
+    generators", OOPSLA 2014): s_i = mix(seed + (i + 1) 0x9E3779B97F4A7C15), mix the generator's
+    finaliser.  Unsigned 64-bit arithmetic wraps, as the generator's does."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(int(seed) & (2 ** 64 - 1)) + (np.arange(1, int(n) + 1, dtype=np.uint64)
+                                                     * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
+
+
 def chebyshev_bounds(levels, iterations=30, safety=1.1, seed=20260923):
     """Upper bounds of the spectrum of D^-1 A per smoothing level for the Chebyshev smoother:
     safety x the D-weighted Rayleigh quotient v.Av / v.Dv after `iterations` power steps
-    v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from the seeded
-    random vector of src/voxel_mg.py:234-235 (the donor's all-ones start is a smooth vector, far
-    from the top of the spectrum), over the free DOFs."""
+    v <- D^-1 A v / |D^-1 A v|_D (donor_yang_gmg multigrid_v4.py:273-295), started from a seeded
+    uniform vector (splitmix_uniform; the donor's all-ones start is a smooth vector, far from the
+    top of the spectrum), over the free DOFs."""
     lams = []
     for L in levels[:-1]:
-        rng = np.random.default_rng(seed)
         d = np.where(L.mask != 0, 1.0 / L.inv_diag, 1.0).astype(L.dtype)
-        v = rng.standard_normal(L.mask.size).astype(L.dtype) * L.mask
+        v = splitmix_uniform(L.mask.size, seed).astype(L.dtype) * L.mask
         v = v / np.sqrt(float(v @ (d * v)))
         for _ in range(iterations):
             w = L.inv_diag * (L.apply(v) * L.mask)

@@ -370,6 +370,19 @@ def test_galerkin_hierarchy_vcycle_matches_the_oracle(KE):
     assert rel(M(torch.from_numpy(r).cuda()), Mo(r)) < 1e-11
 
 
+def test_device_start_vector_equals_the_oracle_restatement():
+    """vh_fill_splitmix (the Chebyshev bound estimate's start vector) reproduces the oracle's
+    SplitMix64 restatement bit for bit in FP64, and its FP32 rounding in FP32 (5 M values, so the
+    counter passes 2^22 and the products wrap)."""
+    from voxelhip import ops
+    n, seed = 5_000_003, 20260923
+    ref = mg_ref.splitmix_uniform(n, seed)
+    y = ops.fill_splitmix(torch.empty(n, dtype=torch.float64, device="cuda"), seed)
+    assert np.array_equal(y.cpu().numpy(), ref)
+    y32 = ops.fill_splitmix(torch.empty(n, dtype=torch.float32, device="cuda"), seed)
+    assert np.array_equal(y32.cpu().numpy(), ref.astype(np.float32))
+
+
 def test_coarsest_factorisation_beside_the_bounds(KE):
     """The coarsest Cholesky factorisation and inverse run on a second stream while the spectrum
     estimates run on the current one: the hierarchy's coarsest inverse equals the host inverse of

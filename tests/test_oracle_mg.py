@@ -300,6 +300,20 @@ def test_chebyshev_multigrid_converges(golden, KE):
     assert ok and it <= 40, (it, rel)
 
 
+def test_splitmix_start_vector_reproduces_the_published_generator():
+    """splitmix_uniform maps the SplitMix64 stream (Steele, Lea, Flood 2014) to [-1, 1): its first
+    values are the generator's published outputs for seed 0 (0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4)
+    and seed 1234567 (6457827717110365317, 3203168211198807973, 9817491932198370423,
+    4593380528125082431, 16408922859458223821), top 53 bits scaled by 2^-53."""
+    to_unit = lambda s: 2.0 * ((s >> 11) * 2.0 ** -53) - 1.0  # noqa: E731
+    assert np.array_equal(mg.splitmix_uniform(2, 0), [to_unit(0xE220A8397B1DCDAF), to_unit(0x6E789E6AA1B965F4)])
+    pub = [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+           16408922859458223821]
+    assert np.array_equal(mg.splitmix_uniform(5, 1234567), [to_unit(v) for v in pub])
+    v = mg.splitmix_uniform(200_000, 20260923)
+    assert v.min() >= -1.0 and v.max() < 1.0 and abs(v.mean()) < 0.01 and abs(v.var() - 1.0 / 3.0) < 0.01
+
+
 def test_chebyshev_bounds_enclose_the_spectrum(golden, KE):
     """The Chebyshev upper bound against the dense spectrum of D^-1 A (free DOFs) on the smoke
     state and on a uniform-modulus grid: the Rayleigh estimate is within 5 % of lambda_max, and
