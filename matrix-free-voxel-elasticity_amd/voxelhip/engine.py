@@ -474,42 +474,17 @@ def build_hierarchy(shape, ke, mask, E64, shapes, pdtype, *, smoother="chebyshev
     else:
         kw = dict(pre=pre, post=post)
     kw.update(coarse_sweeps=coarse_degree, coarse_from=coarse_level, storage=storage)
-    M = mg.Multigrid(None, shapes, prods, E64, mask, None, pdtype, coarse_dtype=coarse_dtype, **kw)
-    # The coarsest factorisation and explicit inverse (rocSOLVER: a chain of small launches) run on
-    # a second stream of the same priority while the spectrum estimates below keep the device busy;
-    # both read only the assembled operator.  cholesky_ex leaves its check on the device until the
-    # join.
-    cur = torch.cuda.current_stream(dev)
-    side = _side_stream(dev)
-    side.wait_stream(cur)
-    with torch.cuda.stream(side):
-        Lc, info = torch.linalg.cholesky_ex(coarse_matrix)
-        inverse = torch.cholesky_inverse(Lc)
-    coarse_matrix.record_stream(side)
+    # (the coarsest factorisation stays on the current stream: run on a second stream beside the
+    #  spectrum estimates, torch's rocSOLVER Cholesky made some solves differ from run to run)
+    Lc = torch.linalg.cholesky(coarse_matrix)
+    M = mg.Multigrid(None, shapes, prods, E64, mask, Lc, pdtype, coarse_dtype=coarse_dtype, **kw)
     # Jacobi: the reference's spectral omegas; Chebyshev: safe upper bounds of the spectrum
     if smoother == "chebyshev":
         M.lambdas = mg.chebyshev_bounds(None, M.levels, iterations=bound_iterations, start=bound_start,
                                         keep=bound_keep)
     else:
         M.omegas, M.lambdas = mg.spectral_omegas(None, M.levels, iterations=omega_iterations, seed=omega_seed)
-    cur.wait_stream(side)
-    for t in (Lc, info, inverse):
-        t.record_stream(cur)
-    if int(info) != 0:
-        raise torch.linalg.LinAlgError(f"the coarsest operator is not positive definite (leading minor {int(info)})")
-    M.set_coarse_inverse(inverse)
     return fine, M
-
-
-_SIDE_STREAMS = {}
-
-
-def _side_stream(dev):
-    """One extra stream per device (default priority), created once."""
-    key = torch.device(dev).index if torch.device(dev).index is not None else torch.cuda.current_device()
-    if key not in _SIDE_STREAMS:
-        _SIDE_STREAMS[key] = torch.cuda.Stream(device=key)
-    return _SIDE_STREAMS[key]
 
 
 def galerkin_probes(fshape, apply, dev):

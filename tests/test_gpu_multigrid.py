@@ -383,11 +383,9 @@ def test_device_start_vector_equals_the_oracle_restatement():
     assert np.array_equal(y32.cpu().numpy(), ref.astype(np.float32))
 
 
-def test_coarsest_factorisation_beside_the_bounds(KE):
-    """The coarsest Cholesky factorisation and inverse run on a second stream while the spectrum
-    estimates run on the current one: the hierarchy's coarsest inverse equals the host inverse of
-    the same operator, and an operator that is not positive definite still raises LinAlgError
-    (checked when the two streams join)."""
+def test_coarsest_inverse_and_an_indefinite_coarsest_operator(KE):
+    """The hierarchy's coarsest inverse equals the host inverse of the same operator, and an
+    operator that is not positive definite raises LinAlgError."""
     from voxelhip import ops
     from voxelhip.engine import build_hierarchy
     shapes = [(16, 8, 8), (8, 4, 4), (4, 2, 2)]
