@@ -6,7 +6,10 @@
 //
 // OC probe (experiments/session_h/run_w1b.py:46-62, 82-94): for a trial multiplier l,
 // x_new = clip(x sqrt(max(-dc/dv/l, 0)), max(x - move, 0), min(x + move, 1)) and the filtered
-// volume mean(H x_new / s).  Because H is symmetric, mean(H x_new / s) = sum_j x_new_j w_j / n
+// volume mean(H x_new / s).  The square root is split as (x sqrt(max(-dc/dv, 0))) l^(-1/2), the
+// first factor formed once per element and l^(-1/2) once per probe: a probe then costs one
+// multiply and the clamps instead of an IEEE divide and square root (values within 2 ulp of the
+// unsplit form; the single- and multi-probe kernels evaluate the same expression).  Because H is symmetric, mean(H x_new / s) = sum_j x_new_j w_j / n
 // with w = H (1/s), so one probe is a single fused elementwise pass plus a deterministic
 // reduction instead of a filter application.
 #include <algorithm>
@@ -43,10 +46,11 @@ __global__ void __launch_bounds__(256) oc_probe_kernel(size_t n, const T* __rest
                                                       T move, T* __restrict__ xnew, double* __restrict__ partial) {
   __shared__ double red[4];
   double acc = 0.0;
+  const T rl = T(1.0 / sqrt(double(lmid)));
   for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
     const T xi = x[i];
-    const T ratio = -dc[i] / (dv ? dv[i] : T(1)) / lmid;
-    T v = xi * sqrt(ratio > T(0) ? ratio : T(0));
+    const T q = -dc[i] / (dv ? dv[i] : T(1));
+    T v = (xi * sqrt(q > T(0) ? q : T(0))) * rl;
     const T lo = xi - move > T(0) ? xi - move : T(0);
     const T hi = xi + move < T(1) ? xi + move : T(1);
     v = v < lo ? lo : (v > hi ? hi : v);
@@ -69,19 +73,23 @@ __global__ void __launch_bounds__(256) oc_probe_multi_kernel(size_t n, const T* 
                                                             double* __restrict__ partial) {
   __shared__ double red[OC_MAX_PROBES][4];
   double acc[OC_MAX_PROBES];
+  T rl[OC_MAX_PROBES];
 #pragma unroll
-  for (int k = 0; k < OC_MAX_PROBES; ++k) acc[k] = 0.0;
+  for (int k = 0; k < OC_MAX_PROBES; ++k) {
+    acc[k] = 0.0;
+    rl[k] = k < lm.count ? T(1.0 / sqrt(double(T(lm.v[k])))) : T(0);   // (l rounded as the single probe's)
+  }
   for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
     const T xi = x[i];
     const T q = -dc[i] / (dv ? dv[i] : T(1));
+    const T xs = xi * sqrt(q > T(0) ? q : T(0));
     const T lo = xi - move > T(0) ? xi - move : T(0);
     const T hi = xi + move < T(1) ? xi + move : T(1);
     const double wi = double(w ? w[i] : T(1));
 #pragma unroll
     for (int k = 0; k < OC_MAX_PROBES; ++k) {
       if (k < lm.count) {
-        const T ratio = q / T(lm.v[k]);
-        T v = xi * sqrt(ratio > T(0) ? ratio : T(0));
+        T v = xs * rl[k];
         v = v < lo ? lo : (v > hi ? hi : v);
         acc[k] += double(v) * wi;
       }
