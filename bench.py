@@ -396,7 +396,7 @@ def main():
     if args.no_cpu_baseline:
         args.cpu_baseline = "none"
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
     import torch
